@@ -1,0 +1,94 @@
+/* CPU oracle of the D2Q9 discrete-Boltzmann time step on continuous Q_p
+ * elements (ivan-pi/lbm-fem), restated from the reference's algorithm.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker of the HIP path: the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark may load it;
+ * the product (lbm-fem_amd/) never does, and must not fall back to it.
+ *
+ * Everything is FP64 on host arrays. A population vector is Q = 9 blocks of
+ * n_nodes doubles (block a at offset a * n_nodes); an increment or right-hand
+ * side is 8 blocks (block a - 1 for population a). Node numbering is
+ * lexicographic on the (pN+1) x (pN+1) grid, x fastest. Periodic images of the
+ * x = L / y = L node lines are kept as constrained (slave) entries.
+ */
+#ifndef LBFEM_ORACLE_H
+#define LBFEM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_disc ora_disc;
+typedef struct ora_scheme ora_scheme;
+
+enum { ORA_TGV = 0, ORA_COUETTE = 1, ORA_CAVITY = 2 };
+enum { ORA_LEELIN = 0, ORA_BARDOW = 1 };
+enum { ORA_TG2 = 0, ORA_TG3 = 1, ORA_TG3_SPLIT = 2 };
+enum { ORA_MASS_CG = 0, ORA_MASS_LUMPED = 1, ORA_MASS_RICHARDSON = 2 };
+enum { ORA_OK = 0, ORA_EINVAL = 1, ORA_ENOCONV = 2 };
+
+/* Test case + discretization (test_cases.h, discretization.h:78-144). */
+ora_disc *ora_disc_create(int n_cells, int degree, int test_case, double mach, int mode1, int mode2,
+                          double stretch);
+void      ora_disc_destroy(ora_disc *d);
+int       ora_disc_n_nodes(const ora_disc *d);
+int       ora_disc_n_per_dir(const ora_disc *d);
+int       ora_disc_n_periodic(const ora_disc *d);
+double    ora_disc_h_min(const ora_disc *d);
+double    ora_disc_time_step(const ora_disc *d, double cfl);
+void      ora_disc_nodes(const ora_disc *d, double *xy);              /* 2 n */
+void      ora_disc_vertices(const ora_disc *d, double *x, double *y); /* N+1 each */
+void      ora_disc_lumped_mass(const ora_disc *d, double *ml);        /* n */
+void      ora_disc_mass_diagonal(const ora_disc *d, double *diag);    /* n */
+void      ora_disc_constrained(const ora_disc *d, int *flag);         /* n, 1 = periodic slave */
+int       ora_disc_n_walls(const ora_disc *d);
+void      ora_disc_walls(const ora_disc *d, int *nodes, double *velocity); /* n_walls, 2 n_walls */
+
+/* TestCase (test_case.h:23-83, test_cases.h). */
+void   ora_case_set_reynolds(ora_disc *d, double Re);
+double ora_case_relaxation_time(const ora_disc *d);
+double ora_case_reference_time(const ora_disc *d);
+double ora_case_U0(const ora_disc *d);
+double ora_case_energy_scale(const ora_disc *d);
+void   ora_case_initial_populations(const ora_disc *d, double lambda, double *f); /* 9 n */
+void   ora_case_exact(const ora_disc *d, double t, double *rho_u);                 /* 3 n */
+
+/* Element-level pins: the 1-D GLL nodes and the (p+1)^2 x (p+1)^2 element
+ * matrices M, C_e, K_e of one h x h cell in deal.II lexicographic order. */
+void ora_gll_nodes(int degree, double *x);
+void ora_element_matrices(int degree, double h, double ex, double ey, double *M, double *C, double *K);
+
+/* Operators (collision.h, advection.h, mass.h). */
+void ora_collide_bgk(const ora_disc *d, double *g, double omega);
+void ora_compute_equilibrium(const ora_disc *d, const double *f, double *feq);
+void ora_predictor_corrector(const ora_disc *d, double *f, const double *feq, const double *incr, double theta);
+/* e: 9 directions (18 doubles). leelin selects LeeLinAdvection (feq required). */
+void ora_advection_apply(const ora_disc *d, double *rhs, const double *f, const double *feq, const double *e,
+                         double dt, double lambda, int leelin, int remove_mass_flux);
+/* y = (M + c K_e) x, constrained rows the identity. */
+void ora_streaming_vmult(const ora_disc *d, double *y, const double *x, double c, double ex, double ey);
+/* x <- A^{-1} r; CG starts from x. Returns ORA_OK / ORA_ENOCONV; *its = CG steps. */
+int ora_mass_solve(const ora_disc *d, double *x, const double *r, int mass_type, int richardson, double cg_tol,
+                   int cg_max_its, double tg3_coef, double ex, double ey, int *its);
+
+/* Schemes (schemes.h, streaming.h). */
+ora_scheme *ora_scheme_create(const ora_disc *d, int scheme, int streaming, int mass_type, int richardson,
+                              double cg_tol, int cg_max_its);
+void        ora_scheme_destroy(ora_scheme *s);
+void        ora_scheme_set_time_step(ora_scheme *s, double dt, double lambda);
+void        ora_scheme_set_initial_populations(ora_scheme *s);
+void        ora_scheme_set_populations(ora_scheme *s, const double *f);
+void        ora_scheme_get_populations(const ora_scheme *s, double *f);
+int         ora_scheme_step(ora_scheme *s, int n_steps); /* ORA_OK or ORA_ENOCONV */
+long long   ora_scheme_cg_iterations(const ora_scheme *s);
+/* the increments of the last step's sweep s (8 blocks), and x[1] */
+void ora_scheme_get_increments(const ora_scheme *s, int which, double *x);
+
+/* Diagnostics sums (cgdbe.cc:353-368) over the non-slave nodes:
+ * |u-u_ex|^2, |u_ex|^2, |u|^2, rho, area, |u-u_prev| (u_prev given, 2 n). */
+void ora_diagnostics(const ora_disc *d, const double *f, double t, const double *u_prev, double *sums);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,400 @@
+"""Pins of the CPU oracle against what the reference publishes or what the
+mathematics fixes. CPU only. Each test names the reference source it is
+anchored on (paths relative to the reference repository).
+"""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def frac_matrix(rows):
+    return np.array([[float(Fraction(v)) for v in r] for r in rows])
+
+
+# ----------------------------------------------------------------- D2Q9 (A1)
+def test_d2q9_lattice_tensors_and_round_trip():
+    """include/lbfem/d2q9.h:80-92: isotropy to 4th order; moments(equilibrium(1.1, 0.05, -0.02))."""
+    E = O.D2Q9_E
+    W = O.W
+    assert abs(W.sum() - 1) < 1e-15
+    assert np.allclose(W @ E, 0, atol=1e-15)
+    T2 = np.einsum("a,ai,aj->ij", W, E, E)
+    assert np.allclose(T2, np.eye(2) / 3, atol=1e-15)
+    T3 = np.einsum("a,ai,aj,ak->ijk", W, E, E, E)
+    assert np.allclose(T3, 0, atol=1e-15)
+    T4 = np.einsum("a,ai,aj,ak,al->ijkl", W, E, E, E, E)
+    d = np.eye(2)
+    iso = (np.einsum("ij,kl->ijkl", d, d) + np.einsum("ik,jl->ijkl", d, d) + np.einsum("il,jk->ijkl", d, d)) / 9
+    assert np.allclose(T4, iso, atol=1e-15)
+    # the oracle's own equilibrium on a one-cell disc: idempotent, with the moments it was built from
+    disc = O.Disc(1, 1, O.CAVITY, mach=0.1, stretch=0.0)
+    feq0 = O.equilibrium(np.full(disc.n, 1.1), np.full(disc.n, 0.05), np.full(disc.n, -0.02))
+    disc_f = disc.compute_equilibrium(feq0)
+    # every node of a 1-cell cavity is a wall node at rest: the imposed velocity is 0
+    rho, ux, uy = O.moments(disc_f)
+    assert np.allclose(rho, 1.1, rtol=1e-14) and np.allclose(ux, 0, atol=1e-16)
+    tgv = O.Disc(2, 1, O.TGV)
+    feq0 = O.equilibrium(np.full(tgv.n, 1.1), np.full(tgv.n, 0.05), np.full(tgv.n, -0.02))
+    f = tgv.compute_equilibrium(feq0)
+    rho, ux, uy = O.moments(f)
+    assert np.allclose(rho, 1.1, atol=1e-14) and np.allclose(ux, 0.05, atol=1e-14)
+    assert np.allclose(uy, -0.02, atol=1e-14)
+    assert np.allclose(f, feq0, atol=1e-15)
+
+
+# ------------------------------------------------- element matrices (A6/A9/A11)
+def _fixtures():
+    with open(os.path.join(GOLDEN, "element_matrices.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_element_matrices_match_published_fractions(p):
+    """docs/figures/fem_stencils_q1.tex:18-19,69-70,120-121,171-172,222-223 and
+    docs/figures/fem_stencils_q2_elements.tex:42,231,420 (node order of the sketches)."""
+    fx = _fixtures()["q%d" % p]
+    n = p + 1
+    perm = [oy * n + ox for ox, oy in fx["node_order"]]
+    h = 0.37
+    for name, item in fx["elements"].items():
+        e = (1.0, 1.0) if "(1,1)" in name else (1.0, 0.0)
+        M, C, K = O.element_matrices(p, h, e)
+        got = {"M": M, "C": C, "K": K}[name[0]][np.ix_(perm, perm)]
+        scale = {"h^2": h * h, "h": h, "": 1.0}[item["scale"]]
+        want = frac_matrix(item["matrix"]) * scale
+        assert np.allclose(got, want, rtol=0, atol=1e-14 * max(1.0, np.abs(want).max())), name
+
+
+def _assembled_row(disc, apply, i0, j0):
+    """Row of the assembled operator at node (i0, j0): A[c, j] = (A e_j)[c]."""
+    npd = disc.np1
+    c = j0 * npd + i0
+    st = np.zeros((3, 3))
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            u = np.zeros(disc.n)
+            u[(j0 + dy) * npd + (i0 + dx)] = 1.0
+            st[dy + 1, dx + 1] = apply(u)[c]
+    return st
+
+
+def test_q1_assembled_stencils_match_published():
+    """docs/figures/fem_stencils_q1.tex, 'assembled stencil' column (uniform h x h grid)."""
+    fx = _fixtures()["q1"]["stencils"]
+    disc = O.Disc(8, 1, O.TGV)
+    h = 1.0 / 8
+    M = _assembled_row(disc, lambda u: disc.vmult(u), 4, 4)
+    assert np.allclose(M, frac_matrix(fx["M"]) * h * h, atol=1e-15)
+
+    def split(e):
+        # r(dt) = -dt C g - dt^2/2 K g for TaylorGalerkinAdvection (advection.h:223-237)
+        def r(u, dt):
+            f = np.zeros((9, disc.n))
+            f[1:] = u
+            ee = np.zeros((9, 2))
+            ee[1:] = e
+            return disc.advection(f, dt, 1.0, e=ee, remove_mass_flux=False)[0]
+
+        Cst = _assembled_row(disc, lambda u: -r(u, 1.0) - 0.5 * (2 * r(u, 1.0) - r(u, 2.0)), 4, 4)
+        Kst = _assembled_row(disc, lambda u: 2 * r(u, 1.0) - r(u, 2.0), 4, 4)
+        return Cst, Kst
+
+    for name_c, name_k, e in (("C_{(1,0)}", "K_{(1,0)}", (1.0, 0.0)), ("C_{(1,1)}", "K_{(1,1)}", (1.0, 1.0))):
+        Cst, Kst = split(e)
+        assert np.allclose(Cst, frac_matrix(fx[name_c]) * h, atol=1e-14), name_c
+        assert np.allclose(Kst, frac_matrix(fx[name_k]), atol=1e-13), name_k
+
+
+# ------------------------------------------- Fourier-mode streaming (A6/A9/A12/A14)
+def _sym1d(t):  # analysis/tg_spectral.py:10-11
+    return (2 + np.cos(t)) / 3, 1j * np.sin(t), 2 * (1 - np.cos(t))
+
+
+def _G2d(tx, ty, ex, ey, C, lhs):  # analysis/tg_spectral.py:19-26 and lbm_spectral.py:16-18
+    mx, cx, kx = _sym1d(tx)
+    my, cy, ky = _sym1d(ty)
+    M = mx * my
+    Ce = ex * cx * my + ey * mx * cy
+    Ke = ex ** 2 * kx * my + ey ** 2 * mx * ky - 2 * ex * ey * cx * cy
+    rhs = C * Ce + 0.5 * C ** 2 * Ke
+    den = {"consistent": M, "tg3": M + C ** 2 / 6 * Ke, "lumped": 1.0}[lhs]
+    return 1 - rhs / den
+
+
+@pytest.mark.parametrize("lhs", ["consistent", "lumped", "tg3"])
+def test_streaming_amplification_of_a_fourier_mode(lhs):
+    """docs/analysis.md:23-44, analysis/tg_spectral.py: one TG2/TG3 streaming step on a uniform
+    periodic Q1 grid multiplies exp(i k.x) by G for every direction; mass solves to 1e-14."""
+    N = 16
+    disc = O.Disc(N, 1, O.TGV)
+    h = 1.0 / N
+    C = 0.4
+    dt = C * h
+    xy = disc.nodes()
+    ix = np.rint(xy[:, 0] / h)
+    jy = np.rint(xy[:, 1] / h)
+    mx_, my_ = 3, 5
+    tx, ty = 2 * np.pi * mx_ / N, 2 * np.pi * my_ / N
+    phase = tx * ix + ty * jy
+    f = np.tile(np.cos(phase), (9, 1))
+    rhs = disc.advection(f, dt, 1.0, remove_mass_flux=False)
+    indep = ~disc.constrained()
+    for a in range(8):
+        ex, ey = O.EX[a + 1], O.EY[a + 1]
+        if lhs == "lumped":
+            x, _, st = disc.mass_solve(rhs[a], mass=O.MASS_LUMPED)
+        else:
+            x, _, st = disc.mass_solve(rhs[a], mass=O.MASS_CG, cg_tol=1e-15, cg_max=400,
+                                       tg3=(dt * dt / 6 if lhs == "tg3" else 0.0), e=(ex, ey))
+        assert st == O.OK
+        G = _G2d(tx, ty, ex, ey, C, lhs)
+        want = np.real((G - 1) * np.exp(1j * phase))
+        assert np.allclose(x[indep], want[indep], atol=2e-13), (lhs, a)
+
+
+def test_tg2_odd_even_mode_known_answer():
+    """docs/analysis.md:33-34: G_TG2(pi) = 1 - 6 C^2 along an axis (consistent mass)."""
+    N = 8
+    disc = O.Disc(N, 1, O.TGV)
+    C = 0.3
+    dt = C / N
+    ix = np.rint(disc.nodes()[:, 0] * N)
+    f = np.tile((-1.0) ** ix, (9, 1))
+    rhs = disc.advection(f, dt, 1.0, remove_mass_flux=False)
+    x, _, _ = disc.mass_solve(rhs[0], cg_tol=1e-15, cg_max=400)
+    indep = ~disc.constrained()
+    assert np.allclose(x[indep], (-6 * C * C * (-1.0) ** ix)[indep], atol=1e-13)
+
+
+def test_tg3_at_courant_one_is_the_exact_shift():
+    """docs/analysis.md:39-42: TG3 at C = 1 is the exact upwind shift along an axis, for any field."""
+    N = 12
+    disc = O.Disc(N, 1, O.TGV)
+    dt = 1.0 / N
+    rng = np.random.default_rng(7)
+    npd = disc.np1
+    g = rng.standard_normal((N, N))
+    full = np.zeros((npd, npd))
+    full[:N, :N] = g
+    full[N, :] = full[0, :]
+    full[:, N] = full[:, 0]
+    f = np.tile(full.reshape(-1), (9, 1))
+    rhs = disc.advection(f, dt, 1.0, remove_mass_flux=False)
+    x, _, _ = disc.mass_solve(rhs[0], cg_tol=1e-15, cg_max=400, tg3=dt * dt / 6, e=(1.0, 0.0))
+    shifted = np.roll(g, 1, axis=1) - g  # g(i-1, j) - g(i, j), e = (1, 0)
+    assert np.allclose(x.reshape(npd, npd)[:N, :N], shifted, atol=1e-12)
+
+
+# ------------------------------------------------ coupled Bardow step (A3 + A9)
+@pytest.mark.parametrize("mass", [O.MASS_LUMPED, O.MASS_CG])
+def test_linearised_bardow_step(mass):
+    """analysis/lbm_spectral.py:8-24: about rest, one collide-then-stream step maps a mode
+    v exp(i k.x) to S(theta) [I - omega (I - J)] v, J_ab = w_a (1 + 3 e_a.e_b)."""
+    N = 16
+    disc = O.Disc(N, 1, O.TGV, mach=0.1, modes=(1, 1), reynolds=100)
+    C = 0.25
+    dt = C / N
+    lam = 0.01
+    omega = dt / (lam + 0.5 * dt)
+    sch = O.Scheme(disc, O.BARDOW, O.TG2, mass, cg_tol=1e-15, cg_max=400)
+    sch.set_time_step(dt, lam)
+    xy = disc.nodes()
+    ix, jy = np.rint(xy[:, 0] * N), np.rint(xy[:, 1] * N)
+    tx, ty = 2 * np.pi * 2 / N, 2 * np.pi * 3 / N
+    phase = tx * ix + ty * jy
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(9)
+    eps = 1e-7
+    g0 = O.W[:, None] * np.ones(disc.n) + eps * v[:, None] * np.cos(phase)[None, :]
+    sch.set_populations(g0)
+    sch.step(1)
+    g1 = sch.populations()
+    J = np.array([[O.W[a] * (1 + 3 * (O.EX[a] * O.EX[b] + O.EY[a] * O.EY[b])) for b in range(9)] for a in range(9)])
+    K = np.eye(9) - omega * (np.eye(9) - J)
+    lhs = "lumped" if mass == O.MASS_LUMPED else "consistent"
+    S = np.array([1.0] + [_G2d(tx, ty, O.EX[a], O.EY[a], C, lhs) for a in range(1, 9)])
+    pred = np.real((S[:, None] * (K @ v)[:, None]) * np.exp(1j * phase)[None, :])
+    indep = ~disc.constrained()
+    got = (g1 - O.W[:, None]) / eps
+    assert np.allclose(got[:, indep], pred[:, indep], atol=5e-5)
+
+
+# ------------------------------------------------------ nodal ops (A2-A5)
+def test_bgk_collision_invariants_and_walls():
+    """include/lbfem/collision.h:147-168: rho and rho u kept on fluid nodes, rho u_wall on walls,
+    omega = 1 gives the equilibrium of the imposed velocity."""
+    disc = O.Disc(6, 1, O.CAVITY, mach=0.3, stretch=1.2)
+    rng = np.random.default_rng(1)
+    g = O.W[:, None] * (1 + 0.05 * rng.standard_normal((9, disc.n)))
+    walls, vel = disc.walls()
+    wall = np.zeros(disc.n, bool)
+    wall[walls] = True
+    rho0, ux0, uy0 = O.moments(g)
+    for omega in (0.7, 1.0, 1.6):
+        out = disc.collide_bgk(g, omega)
+        rho, ux, uy = O.moments(out)
+        assert np.allclose(rho, rho0, rtol=1e-14)
+        assert np.allclose(ux[~wall], ux0[~wall], atol=1e-14) and np.allclose(uy[~wall], uy0[~wall], atol=1e-14)
+        assert np.allclose(ux[walls], vel[:, 0], atol=1e-14) and np.allclose(uy[walls], vel[:, 1], atol=1e-14)
+    ueff = np.where(wall, 0, ux0)
+    ueff[walls] = vel[:, 0]
+    veff = np.where(wall, 0, uy0)
+    veff[walls] = vel[:, 1]
+    assert np.allclose(disc.collide_bgk(g, 1.0), O.equilibrium(rho0, ueff, veff), atol=1e-15)
+    # the lid (y = L, x strictly inside) moves, the two top corners belong to the side walls
+    xy = disc.nodes()
+    lid = (np.abs(xy[walls, 1] - 1) < 1e-12) & (xy[walls, 0] > 1e-12) & (xy[walls, 0] < 1 - 1e-12)
+    assert np.allclose(vel[lid, 0], disc.U0) and np.allclose(vel[~lid], 0)
+
+
+def test_lee_lin_predictor_corrector_limits():
+    """include/lbfem/collision.h:115-145: theta = 0 gives f + x; with x = 0 the density is kept."""
+    disc = O.Disc(4, 1, O.COUETTE, mach=0.1)
+    rng = np.random.default_rng(2)
+    f = O.W[:, None] * (1 + 0.05 * rng.standard_normal((9, disc.n)))
+    x = 1e-3 * rng.standard_normal((8, disc.n))
+    feq = disc.compute_equilibrium(f)
+    out = disc.predictor_corrector(f, feq, x, 0.0)
+    want = f.copy()
+    want[1:] += x
+    assert np.allclose(out, want, atol=1e-16)
+    out = disc.predictor_corrector(f, feq, np.zeros_like(x), 1.7)
+    walls, _ = disc.walls()
+    fluid = np.ones(disc.n, bool)
+    fluid[walls] = False
+    assert np.allclose(out.sum(0)[fluid], f.sum(0)[fluid], rtol=1e-14)
+
+
+# ------------------------------------------------ mass matrices and CG (A11, A14)
+def _dense(disc, c=0.0, e=(0.0, 0.0)):
+    A = np.zeros((disc.n, disc.n))
+    for j in range(disc.n):
+        u = np.zeros(disc.n)
+        u[j] = 1.0
+        A[:, j] = disc.vmult(u, c, e)
+    return A
+
+
+@pytest.mark.parametrize("case,p", [(O.TGV, 1), (O.TGV, 2), (O.TGV, 3), (O.COUETTE, 2), (O.CAVITY, 1), (O.CAVITY, 3)])
+def test_cg_matches_direct_solve_and_diagonals(case, p):
+    """mass.h:204-211 (CG, Jacobi), discretization.h:123-125 (lumped, diagonal), brute force."""
+    disc = O.Disc(4, p, case, mach=0.1, stretch=1.2)
+    A = _dense(disc)
+    assert np.allclose(A, A.T, atol=1e-15)
+    cons = disc.constrained()
+    assert np.allclose(np.diag(A), disc.mass_diagonal(), rtol=1e-14)
+    rowsum = A[:, ~cons].sum(axis=1)  # constrained columns never read (slaves resolve to masters)
+    ml = disc.lumped_mass()
+    assert np.allclose(ml[~cons], rowsum[~cons], rtol=1e-13)
+    assert abs(ml[~cons].sum() - 1.0) < 1e-13  # the area of the unit box
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(disc.n)
+    r[cons] = 0
+    x, its, st = disc.mass_solve(r, cg_tol=1e-15, cg_max=400)
+    assert st == O.OK and its > 0
+    assert np.allclose(x, np.linalg.solve(A, r), rtol=1e-10, atol=1e-12)
+    if case == O.TGV:
+        tg3 = 0.3 ** 2 / 6 / 16
+        A3 = _dense(disc, tg3, (1.0, -1.0))
+        x, _, st = disc.mass_solve(r, cg_tol=1e-15, cg_max=400, tg3=tg3, e=(1.0, -1.0))
+        assert np.allclose(x, np.linalg.solve(A3, r), rtol=1e-10, atol=1e-12)
+
+
+def test_jacobi_preconditioned_mass_spectrum_q1():
+    """mass.h:38-42: the Jacobi-preconditioned Q1 mass matrix (uniform) has eigenvalues in [1/4, 9/4]."""
+    disc = O.Disc(6, 1, O.TGV)
+    cons = disc.constrained()
+    A = _dense(disc)[np.ix_(~cons, ~cons)]
+    # periodic rows: fold the slave columns onto their masters is what the cell loop does already
+    dinv = 1 / np.sqrt(np.diag(A))
+    ev = np.linalg.eigvalsh(dinv[:, None] * A * dinv[None, :])
+    assert ev.min() > 0.25 - 1e-12 and ev.max() < 2.25 + 1e-12
+
+
+def test_cg_cold_start_iteration_counts():
+    """mass.h:38-42: a cold start needs about 25 iterations at 1e-8 and 38 at 1e-12."""
+    disc = O.Disc(32, 1, O.TGV)
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal(disc.n)
+    r[disc.constrained()] = 0
+    _, its8, _ = disc.mass_solve(r, cg_tol=1e-8)
+    _, its12, _ = disc.mass_solve(r, cg_tol=1e-12)
+    assert 19 <= its8 <= 31 and 31 <= its12 <= 45, (its8, its12)
+
+
+def test_cg_non_convergence_is_reported():
+    """mass.h:38-43: NoConvergence at the iteration cap (cgdbe.cc:586-590 maps it to exit 1)."""
+    disc = O.Disc(8, 1, O.TGV)
+    r = np.random.default_rng(0).standard_normal(disc.n)
+    r[disc.constrained()] = 0
+    _, its, st = disc.mass_solve(r, cg_tol=1e-30, cg_max=5)
+    assert st == O.ENOCONV and its == 5
+
+
+# ------------------------------------------------ geometry (A17, A18)
+def test_cavity_mesh_clustering_published_h_min():
+    """docs/results.md:267: 65 x 65 nodes, gamma = 1.2 gives h_min = 0.0071; test_cases.h:215-225."""
+    disc = O.Disc(64, 1, O.CAVITY, mach=0.1, stretch=1.2)
+    assert abs(disc.h_min - 0.0071) < 5e-5
+    x, y = disc.vertices()
+    assert np.allclose(x, y) and abs(x[0]) < 1e-15 and abs(x[-1] - 1) < 1e-15
+    assert abs((x[1] - x[0]) - disc.h_min) < 1e-15
+    # Q2 on 32^2 elements: the same 65 nodes per direction, h_min of the element
+    d2 = O.Disc(32, 2, O.CAVITY, mach=0.1, stretch=1.2)
+    assert d2.np1 == 65 and abs(d2.time_step(0.4) - 0.4 * d2.h_min / 4) < 1e-18
+
+
+# ------------------------------------------------ end to end (A16, A19)
+def _run_case(case, scheme, mass, N, Re, cfl, t_end, modes=(1, 4), p=1, cg_tol=1e-8, mach=0.1):
+    """apps/cgdbe/cgdbe.cc:236-245 (dt, step count, refit for unsteady cases)."""
+    disc = O.Disc(N, p, case, mach=mach, modes=modes, reynolds=Re)
+    T = t_end * disc.reference_time
+    dt = disc.time_step(cfl)
+    n = int(np.ceil(T / dt))
+    dt = T / n
+    sch = O.Scheme(disc, scheme, O.TG2, mass, cg_tol=cg_tol)
+    lam = disc.relaxation_time
+    sch.set_time_step(dt, lam)
+    sch.set_initial_populations()
+    sch.step(n)
+    s = disc.diagnostics(sch.populations(), n * dt)
+    return disc, sch, s, n
+
+
+@pytest.mark.parametrize("scheme,mass", [(O.LEELIN, O.MASS_CG), (O.BARDOW, O.MASS_LUMPED), (O.BARDOW, O.MASS_CG)])
+def test_couette_steady_profile_to_solver_tolerance(scheme, mass):
+    """docs/results.md:250-263: 16 elements, Re = 10, t = 2 L^2/nu: with the wall surface term
+    (kept by leelin, mass flux removed by bardow) the steady shear profile is reproduced to the
+    solver tolerance (published 2e-10 / 2e-10 / 4e-10 at CG tolerance 1e-8)."""
+    _, _, s, _ = _run_case(O.COUETTE, scheme, mass, 16, 10.0, 0.25, 2.0)
+    err = np.sqrt(s[0] / s[1])
+    assert err < 2e-9, err
+    assert abs(s[3] / s[4] - 1.0) < 1e-9  # mass drift
+
+
+def test_couette_leelin_transient():
+    """docs/results.md:255: leelin, consistent mass, transient error 7.7e-4 at t = 0.2 L^2/nu.
+    Reproduced at CFL 0.5 (the leelin default of the driver the table predates); the bardow
+    transient entries of that table were made with an earlier wall treatment and are not pinned."""
+    _, _, s, _ = _run_case(O.COUETTE, O.LEELIN, O.MASS_CG, 16, 10.0, 0.5, 0.2)
+    err = np.sqrt(s[0] / s[1])
+    assert abs(err - 7.7e-4) / 7.7e-4 < 0.03, err
+
+
+def test_taylor_green_published_decay():
+    """docs/results.md:36-38: modes 1,1, 64^2, Re = 100, leelin (consistent, CFL 0.25, Ma 0.1,
+    t = t_ref): E/E0 = 0.13513 against 0.13534 exact, velocity error 9.3e-4."""
+    disc, sch, s, n = _run_case(O.TGV, O.LEELIN, O.MASS_CG, 64, 100.0, 0.25, 1.0, modes=(1, 1))
+    E = s[2] / s[4] / disc.energy_scale
+    Eex = s[1] / s[4] / disc.energy_scale
+    err = np.sqrt(s[0] / s[1])
+    assert abs(Eex - 0.13534) < 1e-5
+    assert abs(E - 0.13513) < 2e-5, E
+    assert abs(err - 9.3e-4) < 0.5e-4, err
