@@ -1,0 +1,170 @@
+"""Benchmark of the D2Q9 Q_p time step on one MI355X.
+
+Workload (the plain-method GPU configuration): lid-driven cavity on the unit
+square, U0 = 0.1 (Ma = 0.1732), tanh clustering 1.2, Q1 on 256 x 256 cells,
+Lee-Lin predictor-corrector, Re 400, CFL 0.5, FP64. The headline value is the
+lumped-mass step (one fused kernel per step); the consistent-mass (CG, tol
+1e-10) step, the Q2 128 x 128 mesh and an HBM-resident Q1 sweep (2048^2,
+4096^2 cells, beyond the 256 MB Infinity Cache) are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lbm-fem_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
+Q = 9
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=400.0, cfl=0.5):
+    tc = A.test_cases.make("cavity", mach=0.1 / (1 / 3) ** 0.5, stretch=1.2)  # U0 = Ma c_s = 0.1
+    tc.set_reynolds(Re)
+    disc = A.Discretization(tc, N, p)
+    cls = A.LeeLin if leelin else A.Bardow
+    s = cls(disc, mass=A.MASS_LUMPED if mass is None else mass, cg_tolerance=cg_tol)
+    dt = disc.time_step(cfl)
+    s.set_time_step(dt, tc.relaxation_time())
+    s.set_initial_populations(tc)
+    return disc, s
+
+
+def timed(torch, s, steps, warmup):
+    s.step(warmup)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_time(s, steps):
+    """Average duration of the step's advection-stage kernel(s), from HIP events on the scheme's stream."""
+    s.enable_stage_timers(True)
+    s.step(steps)
+    t = s.stage_times()
+    s.enable_stage_timers(False)
+    return t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is a domain decomposition, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import lbfem_amd as A
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    K, W = args.steps, args.warmup
+
+    # ---- headline: cavity Q1 256^2, Lee-Lin, lumped (fused kernel)
+    disc, s = make_scheme(A, 256, 1)
+    n = disc.n_nodes
+    sec = timed(torch, s, K, W)
+    value = 9 * n * K / sec / 1e6
+    kt = kernel_time(s, K)
+    t_kernel = kt["advection"] / K
+    alg_bytes = 18 * 8 * n
+    achieved = alg_bytes / t_kernel / 1e9
+    log(f"cavity Q1 256^2 leelin lumped: {1e3 * sec / K:.4f} ms/step, {value:.1f} MLUPS, kernel {1e6 * t_kernel:.2f} us, "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    variants = {}
+    # ---- consistent mass (CG tol 1e-10), and Q2 128^2
+    for name, N, p, mass in (("q1_256_leelin_cg", 256, 1, A.MASS_CG), ("q2_128_leelin_lumped", 128, 2, A.MASS_LUMPED),
+                             ("q2_128_leelin_cg", 128, 2, A.MASS_CG), ("q1_256_bardow_lumped", 256, 1, None)):
+        leelin = not name.endswith("bardow_lumped")
+        d2, s2 = make_scheme(A, N, p, leelin=leelin, mass=mass)
+        steps = max(10, K // 4) if mass == A.MASS_CG else K
+        its0 = s2.cg_iterations
+        sec2 = timed(torch, s2, steps, W)
+        its = (s2.cg_iterations - its0) / (8 * (steps + W)) if mass == A.MASS_CG else 0.0
+        variants[name] = {"ms_per_step": 1e3 * sec2 / steps, "MLUPS": 9 * d2.n_nodes * steps / sec2 / 1e6,
+                          "cg_its_per_solve": round(its, 2)}
+        log(name, variants[name])
+        del s2, d2
+
+    # ---- HBM-resident sweep (working set > 256 MB Infinity Cache)
+    sweep = []
+    if not args.no_sweep:
+        for N, steps in ((2048, 40), (4096, 12)):
+            d3, s3 = make_scheme(A, N, 1)
+            sec3 = timed(torch, s3, steps, 3)
+            kt3 = kernel_time(s3, steps)
+            tk = kt3["advection"] / steps
+            gbs = 18 * 8 * d3.n_nodes / tk / 1e9
+            sweep.append({"cells": f"{N}x{N}", "degree": 1, "n_nodes": d3.n_nodes, "ms_per_step": 1e3 * sec3 / steps,
+                          "MLUPS": 9 * d3.n_nodes * steps / sec3 / 1e6, "kernel_us": 1e6 * tk,
+                          "achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)})
+            log("sweep", sweep[-1])
+            del s3, d3
+
+    # ---- CPU baseline: the oracle (C++ restatement), one core, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        od = O.Disc(256, 1, O.CAVITY, mach=0.1 / (1 / 3) ** 0.5, stretch=1.2, reynolds=400.0)
+        osch = O.Scheme(od, O.LEELIN, O.TG2, O.MASS_LUMPED)
+        osch.set_time_step(od.time_step(0.5), od.relaxation_time)
+        osch.set_initial_populations()
+        osch.step(1)
+        cpu_steps = 40
+        t0 = time.perf_counter()
+        osch.step(cpu_steps)
+        ct = time.perf_counter() - t0
+        cpu = {"value": round(9 * od.n * cpu_steps / ct / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": f"{cpu_steps} steps of the same workload (cavity Q1 256^2, Lee-Lin, lumped), "
+                         f"{ct:.1f} s on one host core"}
+        log("cpu", cpu)
+
+    out = {
+        "metric": "MLUPS (node·pop updates/s)",
+        "value": round(value, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * sec / K, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (lid-driven cavity from rest, seeded by the flow definition)",
+        "config": {"workload": "lid-driven cavity, Q1 256x256 cells (66049 nodes), Lee-Lin, lumped mass, "
+                               "Re 400, CFL 0.5, U0 0.1, stretch 1.2",
+                   "n_nodes": n, "mnups": round(value / 9, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_tile<1,32,16,kStepLeeLin>", "kernel_us": round(1e6 * t_kernel, 3),
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+        "variants": variants,
+        "hbm_sweep": sweep,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/* lbfem_hip: the D2Q9 discrete-Boltzmann time step on continuous Q_p
+ * elements (ivan-pi/lbm-fem) on one MI355X (gfx950), behind a C ABI.
+ *
+ * The entry points mirror the reference's operator and scheme API, so that a
+ * driver written against include/lbfem/ of the reference calls this library
+ * in their place (each declaration names the reference interface it
+ * replaces). Conventions:
+ *   - every call returns an int status (LBFEM_OK, LBFEM_EINVAL for what the
+ *     reference rejects with AssertThrow, LBFEM_ENOCONV for
+ *     SolverControl::NoConvergence, LBFEM_EHIP for a HIP runtime error);
+ *     lbfem_last_error() gives the message. No C++ exception crosses the ABI.
+ *   - a population vector is Q = 9 blocks of n_nodes values, block a at
+ *     offset a * stride (the reference's BlockVector, docs/method.md:165); an
+ *     increment or right-hand side has 8 blocks (block a - 1 <-> population a).
+ *   - nodes are numbered lexicographically on the (pN+1) x (pN+1) grid, x
+ *     fastest; the periodic images of the x = L and y = L lines are kept as
+ *     constrained (slave) entries, touched by the nodal operations only, as
+ *     deal.II's periodicity constraints do (discretization.h:110-112).
+ *   - host-buffer entry points copy; device-pointer entry points take a
+ *     hipStream_t (as void*, NULL = the null stream) and do not synchronize.
+ *   - handles own their device memory and are not thread-safe (the reference
+ *     is single-threaded per rank, discretization.h:115).
+ */
+#ifndef LBFEM_HIP_H
+#define LBFEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { LBFEM_OK = 0, LBFEM_EINVAL = 1, LBFEM_ENOCONV = 2, LBFEM_EHIP = 3 };
+enum { LBFEM_LEELIN = 0, LBFEM_BARDOW = 1 };                      /* cgdbe.cc:56-60 */
+enum { LBFEM_TG2 = 0, LBFEM_TG3 = 1, LBFEM_TG3_SPLIT = 2 };        /* streaming.h:25-30 */
+enum { LBFEM_MASS_CG = 0, LBFEM_MASS_LUMPED = 1, LBFEM_MASS_RICHARDSON = 2 }; /* mass.h:26-31 */
+enum { LBFEM_FP64 = 0, LBFEM_MIXED = 1 };
+
+typedef struct lbfem_disc   lbfem_disc;
+typedef struct lbfem_scheme lbfem_scheme;
+
+/* Copies the message of the last failed call of this thread (NUL-terminated). */
+size_t lbfem_last_error(char *buf, size_t len);
+/* Number of visible HIP devices (0 without a GPU). */
+int lbfem_device_count(int *count);
+
+/* ---------------------------------------------------------- Discretization
+ * Replaces lbfem::Discretization::reinit (include/lbfem/discretization.h:78-144)
+ * for the structured box meshes of the reference: N x N Q_p cells on
+ * [0, L]^2, periodic in the first n_periodic directions, vertices optionally
+ * moved by the tanh clustering of LidDrivenCavity::transform_mesh
+ * (include/lbfem/test_cases.h:215-225) when stretch > 0. */
+typedef struct
+{
+  int    n_cells;    /* N, elements per direction (MeshSettings::n_cells) */
+  int    degree;     /* p in 1..3 (fe_degree, discretization.h:45) */
+  int    n_periodic; /* 0, 1 (x) or 2 (TestCase::n_periodic) */
+  double L;          /* box size (TestCase::L) */
+  double stretch;    /* tanh clustering gamma, 0 = uniform */
+} lbfem_mesh_desc;
+
+typedef struct
+{
+  int64_t n_nodes;       /* (pN+1)^2, slaves included (the reference's n_dofs) */
+  int64_t n_independent; /* non-slave nodes (Discretization::independent) */
+  int64_t stride;        /* block stride of device population vectors */
+  int32_t n_per_dir;     /* pN + 1 */
+  int32_t n_walls;
+  double  h_min;         /* Discretization::h_min */
+} lbfem_disc_info;
+
+int lbfem_disc_create(const lbfem_mesh_desc *desc, lbfem_disc **disc);
+int lbfem_disc_destroy(lbfem_disc *disc);
+int lbfem_disc_get_info(const lbfem_disc *disc, lbfem_disc_info *info);
+/* Discretization::node (support points), 2 doubles per node. */
+int lbfem_disc_node_coordinates(const lbfem_disc *disc, double *xy);
+/* 1 for periodic slave nodes (constraints.is_constrained), else 0. */
+int lbfem_disc_constrained(const lbfem_disc *disc, int32_t *flag);
+/* Discretization::node_weight (row-sum lumped mass, constrained entries 1) and
+ * the diagonal of M used by the Jacobi preconditioner (discretization.h:123-125). */
+int lbfem_disc_lumped_mass(const lbfem_disc *disc, double *ml);
+int lbfem_disc_mass_diagonal(const lbfem_disc *disc, double *diag);
+/* Discretization::walls (discretization.h:63-67, 138-142): the wall nodes and
+ * the velocity of the wall each is on. Wall nodes must lie on the boundary of
+ * the box in a non-periodic direction. Replaces any earlier list. */
+int lbfem_disc_set_walls(lbfem_disc *disc, int32_t n, const int32_t *nodes, const double *velocity);
+/* Discretization::time_step (discretization.h:158-162): cfl * h_min / p^2. */
+int lbfem_disc_time_step(const lbfem_disc *disc, double cfl, double *dt);
+
+/* ---------------------------------------------------------- operators
+ * Device pointers, stream as void* (hipStream_t). */
+
+/* lbfem::AdvectionOperator::apply (include/lbfem/advection.h:49-58) with the
+ * two implementations of the reference: kind LBFEM_BARDOW is
+ * TaylorGalerkinAdvection (advection.h:211-246), LBFEM_LEELIN is
+ * LeeLinAdvection (advection.h:256-293; feq required). rhs: 8 blocks, f and
+ * feq: 9 blocks, all with the disc's stride. e: the 9 directions of the sweep
+ * (18 doubles, host). remove_mass_flux: the wall surface-term option
+ * (advection.h:182-199). */
+int lbfem_advection_apply(const lbfem_disc *disc, int kind, double *rhs, const double *f, const double *feq,
+                          const double *e, double dt, double lambda, int remove_mass_flux, void *stream);
+
+/* lbfem::collide_bgk, compute_equilibrium, predictor_corrector
+ * (include/lbfem/collision.h:105-168), over every node, walls imposing
+ * their velocity. incr: 8 blocks. */
+int lbfem_collide_bgk(const lbfem_disc *disc, double *g, double omega, void *stream);
+int lbfem_compute_equilibrium(const lbfem_disc *disc, const double *f, double *feq, void *stream);
+int lbfem_predictor_corrector(const lbfem_disc *disc, double *f, const double *feq, const double *incr, double theta,
+                              void *stream);
+
+/* y = (M + c K_e) x (lbfem::StreamingMatrix::vmult, mass.h:78-83), constrained
+ * rows the identity. */
+int lbfem_streaming_vmult(const lbfem_disc *disc, double *y, const double *x, double c, const double *e,
+                          void *stream);
+
+/* lbfem::MassSettings (include/lbfem/mass.h:33-48). */
+typedef struct
+{
+  int    type;              /* LBFEM_MASS_* */
+  int    richardson;        /* passes for LBFEM_MASS_RICHARDSON */
+  double cg_tolerance;      /* relative to |r| */
+  int    cg_max_iterations; /* NoConvergence when hit */
+  int    fused;             /* accepted for interface parity; the CG updates are always fused here */
+} lbfem_mass_settings;
+
+/* lbfem::MassSolver::solve (include/lbfem/mass.h:184-213): x <- A^{-1} r for
+ * one population, A = M + tg3_coefficient K_e(e); CG starts from x.
+ * *iterations receives the CG steps (0 for the lumped variants). Synchronizes
+ * the stream (the stopping test reads the residual). */
+int lbfem_mass_solve(const lbfem_disc *disc, const lbfem_mass_settings *settings, double *x, const double *r,
+                     const double *e, double tg3_coefficient, int *iterations, void *stream);
+
+/* ---------------------------------------------------------- schemes
+ * lbfem::Scheme, LeeLin, Bardow (include/lbfem/schemes.h:31-142) with
+ * lbfem::TaylorGalerkin streaming (include/lbfem/streaming.h:38-130). The
+ * scheme owns its populations on the device. */
+typedef struct
+{
+  int                 scheme;    /* LBFEM_LEELIN | LBFEM_BARDOW */
+  int                 streaming; /* LBFEM_TG2 | LBFEM_TG3 | LBFEM_TG3_SPLIT */
+  lbfem_mass_settings mass;
+  int                 precision; /* LBFEM_FP64 | LBFEM_MIXED */
+} lbfem_scheme_settings;
+
+int lbfem_scheme_create(const lbfem_disc *disc, const lbfem_scheme_settings *settings, lbfem_scheme **scheme);
+int lbfem_scheme_destroy(lbfem_scheme *scheme);
+/* Scheme::set_time_step (schemes.h:36-41): {dt, lambda = nu / c_s^2}. */
+int lbfem_scheme_set_time_step(lbfem_scheme *scheme, double dt, double lambda);
+/* Scheme::set_initial_populations (schemes.h:50-59, Bardow's transform
+ * schemes.h:128-141 applied): f0 are the test case's populations at t = 0,
+ * 9 blocks of n_nodes doubles on the host. */
+int lbfem_scheme_set_initial_populations(lbfem_scheme *scheme, const double *f0);
+/* Raw access to Scheme::f (f for LeeLin, g for Bardow), host, 9 x n_nodes. */
+int lbfem_scheme_set_populations(lbfem_scheme *scheme, const double *f);
+int lbfem_scheme_get_populations(const lbfem_scheme *scheme, double *f);
+/* The scheme's device population vector (9 blocks, disc stride) for
+ * device-side callers; valid until the next step. */
+int lbfem_scheme_device_populations(lbfem_scheme *scheme, double **f);
+/* Scheme::step (schemes.h:92-98, 118-124), n_steps times, on the scheme's
+ * stream; returns after the work is queued unless CG needs host checks. */
+int lbfem_scheme_step(lbfem_scheme *scheme, int n_steps);
+int lbfem_scheme_synchronize(lbfem_scheme *scheme);
+/* MassSolver::cg_iterations (mass.h:215), summed over all solves. */
+int lbfem_scheme_cg_iterations(const lbfem_scheme *scheme, int64_t *iterations);
+/* StageTimers (include/lbfem/timers.h:17-23): seconds in collision,
+ * advection, mass, and the total of the timed steps; timing adds events. */
+int lbfem_scheme_enable_stage_timers(lbfem_scheme *scheme, int enable);
+int lbfem_scheme_stage_times(const lbfem_scheme *scheme, double *seconds4);
+
+/* The diagnostics sums of cgdbe (apps/cgdbe/cgdbe.cc:353-368) over the
+ * non-slave nodes with lumped-mass weights, from device populations:
+ * sums[0..5] = |u-u_ex|^2, |u_ex|^2, |u|^2, rho, area, |u-u_prev|;
+ * u_exact and u_prev: device arrays of 2 doubles per node (u_prev updated). */
+int lbfem_diagnostics(const lbfem_disc *disc, const double *f, const double *u_exact, double *u_prev, double *sums,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

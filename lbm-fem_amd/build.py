@@ -1,0 +1,59 @@
+"""Builds liblbfem_hip.so for gfx950 in-tree with hipcc (no JIT, no cache).
+
+    python lbm-fem_amd/build.py            # -> lbm-fem_amd/lbfem_amd/_lib/liblbfem_hip.so
+"""
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT_DIR = os.path.join(HERE, "lbfem_amd", "_lib")
+LIB = os.path.join(OUT_DIR, "liblbfem_hip.so")
+SOURCES = ["api.hip", "tile_kernels.hip", "solver_kernels.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
+    # packed FP32 arithmetic disabled for all device code of this library
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+
+
+def build(verbose=True, jobs=None):
+    os.makedirs(OUT_DIR, exist_ok=True)
+    objs = []
+    procs = []
+    t0 = time.time()
+    for src in SOURCES:
+        obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
+        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        objs.append(obj)
+    failed = False
+    for src, p in procs:
+        out, _ = p.communicate()
+        text = out.decode(errors="replace")
+        # the host pass does not know the packed-fp32 feature; that warning is expected
+        lines = [l for l in text.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+        if p.returncode != 0:
+            failed = True
+            print(f"[build] {src} failed:\n" + "\n".join(lines), file=sys.stderr)
+        elif verbose and lines:
+            print(f"[build] {src}:\n" + "\n".join(lines[:40]), file=sys.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed")
+    tmp = LIB + ".tmp"
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    if verbose:
+        print(f"[build] {LIB} ({time.time() - t0:.1f} s)")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

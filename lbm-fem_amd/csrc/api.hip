@@ -1,0 +1,1037 @@
+// The C ABI of include/lbfem_hip.h: discretization and scheme objects, the
+// sequencing of a time step (include/lbfem/schemes.h, streaming.h, mass.h)
+// over the kernels of tile_kernels.hip and solver_kernels.hip.
+#include "lbfem_hip.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+#include "kernels.hpp"
+
+using namespace lbfem_amd;
+
+namespace
+{
+  thread_local std::string g_error;
+
+  int
+  fail(const int code, const std::string &msg)
+  {
+    g_error = msg;
+    return code;
+  }
+
+  struct HipError : std::runtime_error
+  {
+    using std::runtime_error::runtime_error;
+  };
+
+  void
+  check(const hipError_t e, const char *what)
+  {
+    if (e != hipSuccess)
+      throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+  }
+#define HIP_CHECK(x) check((x), #x)
+
+  template <typename T>
+  struct DevBuf
+  {
+    T     *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf()
+    {
+      if (p)
+        (void)hipFree(p);
+    }
+    void
+    alloc(const size_t count)
+    {
+      if (p)
+        (void)hipFree(p);
+      p = nullptr;
+      n = count;
+      if (count)
+        HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void
+    upload(const std::vector<T> &v)
+    {
+      alloc(v.size());
+      if (!v.empty())
+        HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+  };
+
+  // D2Q9 directions (include/lbfem/d2q9.h:18-19)
+  constexpr double kE[Q][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}, {-1, -1}, {0, -1}, {1, -1}};
+
+  constexpr int kMaxCgBlocks = 1 << 16;
+} // namespace
+
+struct lbfem_disc
+{
+  lbfem_mesh_desc          desc{};
+  std::unique_ptr<Basis1D> B;
+  std::unique_ptr<Axis1D>  ax, ay;
+  int                      p = 1, N = 1, P = 1, np = 2;
+  int64_t                  n = 0, stride = 0;
+  double                   h_min = 0.;
+  DevBuf<double>           axbuf, aybuf, wall_vel;
+  DevBuf<int32_t>          wall_side[4];
+  std::vector<int32_t>     wall_nodes;
+  std::vector<double>      wall_velocity;
+  Geo                      G{};
+  // scratch of the operator-level entry points
+  DevBuf<CgState>  cg_state;
+  DevBuf<double>   partial, s_r, s_p0, s_p1, s_q, s_ax;
+  DevBuf<unsigned> counter;
+
+  void
+  upload_axis(const Axis1D &a, DevBuf<double> &buf, AxisDev &dev)
+  {
+    const size_t        w = size_t(a.P + 1) * (2 * a.p + 1), m = size_t(a.P + 1);
+    std::vector<double> host;
+    host.reserve(4 * w + 3 * m);
+    for (const auto *v : {&a.M, &a.D, &a.DT, &a.K})
+      host.insert(host.end(), v->begin(), v->end());
+    for (size_t i = 0; i < m; ++i)
+      host.push_back(1. / a.row_sum[i]);
+    host.insert(host.end(), a.row_sum.begin(), a.row_sum.end());
+    host.insert(host.end(), a.diagonal.begin(), a.diagonal.end());
+    buf.upload(host);
+    dev.M        = buf.p;
+    dev.D        = buf.p + w;
+    dev.DT       = buf.p + 2 * w;
+    dev.K        = buf.p + 3 * w;
+    dev.ml_inv   = buf.p + 4 * w;
+    dev.ml       = buf.p + 4 * w + m;
+    dev.diag     = buf.p + 4 * w + 2 * m;
+    dev.periodic = a.periodic;
+    for (int l = 0; l < 4; ++l)
+      {
+        dev.b0[l] = l <= a.p ? a.b0[l] : 0.;
+        dev.b1[l] = l <= a.p ? a.b1[l] : 0.;
+      }
+  }
+
+  void
+  ensure_scratch()
+  {
+    if (cg_state.p)
+      return;
+    cg_state.alloc(NM);
+    partial.alloc(size_t(3 * NM) * kMaxCgBlocks);
+    counter.alloc(1);
+    HIP_CHECK(hipMemset(counter.p, 0, sizeof(unsigned)));
+  }
+};
+
+struct lbfem_scheme
+{
+  const lbfem_disc     *d = nullptr;
+  lbfem_scheme_settings set{};
+  double                dt = 0., lambda = 0.;
+  hipStream_t           stream = nullptr;
+  DevBuf<double>        f[2];
+  int                   cur = 0;
+  DevBuf<double>        rhs, x[2], r, pb[2], q, ax_tmp;
+  DevBuf<CgState>       state;
+  DevBuf<double>        partial;
+  DevBuf<unsigned>      counter;
+  CgState              *host_state = nullptr; // pinned
+  int64_t               cg_iterations = 0;
+  std::vector<std::array<double, 2 * Q>> sweeps;
+  double                tg3         = 0.;
+  bool                  extrapolate = false;
+  int                   last_its    = 1;
+  // stage timers (include/lbfem/timers.h): collision, advection, mass, total
+  bool                                   timing = false;
+  std::vector<std::pair<hipEvent_t, int>> marks;
+  double                                 stage[4] = {0., 0., 0., 0.};
+
+  ~lbfem_scheme()
+  {
+    for (auto &m : marks)
+      (void)hipEventDestroy(m.first);
+    if (host_state)
+      (void)hipHostFree(host_state);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+
+  void
+  mark(const int stage_id)
+  {
+    if (!timing)
+      return;
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(e, stream));
+    marks.push_back({e, stage_id});
+  }
+
+  void
+  resolve_marks()
+  {
+    if (marks.empty())
+      return;
+    HIP_CHECK(hipEventSynchronize(marks.back().first));
+    for (size_t k = 1; k < marks.size(); ++k)
+      {
+        if (marks[k].second < 0) // a step boundary opens a new interval
+          continue;
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, marks[k - 1].first, marks[k].first));
+        stage[marks[k].second] += 1e-3 * ms;
+        stage[3] += 1e-3 * ms;
+      }
+    for (auto &m : marks)
+      (void)hipEventDestroy(m.first);
+    marks.clear();
+  }
+};
+
+namespace
+{
+  // ------------------------------------------------------------------ helpers
+  TileParams
+  tile_params(const lbfem_scheme &s, const std::array<double, 2 * Q> &e, const bool leelin)
+  {
+    TileParams S{};
+    for (int a = 0; a < Q; ++a)
+      {
+        S.e[a][0] = e[2 * a];
+        S.e[a][1] = e[2 * a + 1];
+      }
+    S.dt = s.dt;
+    if (leelin) // advection.h:268-270
+      {
+        S.b = -0.5 * s.dt * s.dt / s.lambda;
+        S.a = -s.dt - S.b;
+      }
+    else // advection.h:223
+      {
+        S.a = -s.dt;
+        S.b = 0.;
+      }
+    S.c                = -0.5 * s.dt * s.dt;
+    S.omega            = s.dt / (s.lambda + 0.5 * s.dt); // schemes.h:122
+    S.theta            = s.dt / s.lambda;                // schemes.h:97
+    S.remove_mass_flux = leelin ? 0 : 1;                 // advection.h:288, schemes.h:115
+    return S;
+  }
+
+  bool
+  moves(const std::array<double, 2 * Q> &e, const int a) // streaming.h:118-122
+  {
+    return e[2 * (a + 1)] != 0. || e[2 * (a + 1) + 1] != 0.;
+  }
+
+  CgWork
+  cg_work(const lbfem_disc &d, CgState *state, double *partial, unsigned *counter, const lbfem_mass_settings &m)
+  {
+    CgWork W{};
+    W.max_its    = m.cg_max_iterations;
+    W.tol_rel    = m.cg_tolerance;
+    W.state      = state;
+    W.partial    = partial;
+    W.counter    = counter;
+    W.max_blocks = kMaxCgBlocks;
+    (void)d;
+    return W;
+  }
+
+  // CG on the vectors of W (include/lbfem/mass.h:204-211): returns LBFEM_OK or
+  // LBFEM_ENOCONV; adds the iterations of all vectors to *its.
+  int
+  run_cg(const lbfem_disc &d, const CgWork &W, double *x, const double *b, double *r, double *p0, double *p1,
+         double *q, CgState *host_state, hipStream_t stream, int64_t *its, const int check_from)
+  {
+    const Geo &G = d.G;
+    HIP_CHECK(cg_start_launch(G, W, x, b, r, stream));
+    double    *pb[2] = {p0, p1};
+    const auto done  = [&](bool &finished) -> int {
+      HIP_CHECK(hipMemcpyAsync(host_state, W.state, W.nv * sizeof(CgState), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      bool active = false, failed = false;
+      for (int v = 0; v < W.nv; ++v)
+        {
+          active |= host_state[v].status == 0;
+          failed |= host_state[v].status == 2;
+        }
+      finished = !active;
+      if (!finished)
+        return LBFEM_OK;
+      for (int v = 0; v < W.nv; ++v)
+        *its += host_state[v].its;
+      if (failed)
+        return fail(LBFEM_ENOCONV,
+                    "CG: no convergence within " + std::to_string(W.max_its) + " iterations (SolverControl::NoConvergence)");
+      return LBFEM_OK;
+    };
+    // iterations of converged vectors are no-ops, so the first checks can be skipped
+    for (int it = 1; it <= W.max_its; ++it)
+      {
+        HIP_CHECK(cg_direction_launch(G, W, it, pb[it % 2], r, pb[(it + 1) % 2], q, stream));
+        HIP_CHECK(cg_update_launch(G, W, it, x, r, pb[(it + 1) % 2], q, stream));
+        if (it >= check_from)
+          {
+            bool      finished = false;
+            const int st       = done(finished);
+            if (finished)
+              return st;
+          }
+      }
+    bool finished = false;
+    return done(finished); // the update at max_its leaves no vector active
+  }
+
+  // the increments of one sweep: advection, then the mass solves (streaming.h:66-84)
+  int
+  increment(lbfem_scheme &s, const int sw, const bool collide)
+  {
+    const lbfem_disc &d  = *s.d;
+    const Geo        &G  = d.G;
+    const auto       &e  = s.sweeps[sw];
+    const bool        ll = s.set.scheme == LBFEM_LEELIN;
+    TileParams        S  = tile_params(s, e, ll);
+    S.collide            = collide ? 1 : 0;
+    double *fin = s.f[s.cur].p, *gstar = nullptr;
+    if (collide)
+      gstar = s.f[1 - s.cur].p;
+    s.mark(-1);
+    HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, G, S, fin, nullptr, gstar, s.rhs.p, s.stream));
+    s.mark(1);
+    if (collide)
+      s.cur = 1 - s.cur;
+
+    unsigned mask = 0;
+    for (int a = 0; a < NM; ++a)
+      if (moves(e, a))
+        mask |= 1u << a;
+    double   *xs = s.x[sw].p;
+    const int type = s.set.mass.type;
+    if (type == LBFEM_MASS_LUMPED)
+      HIP_CHECK(lumped_solve_launch(G, xs, s.rhs.p, NM, s.stream));
+    else if (type == LBFEM_MASS_RICHARDSON) // mass.h:193-202, x_0 = M_L^-1 r, x += M_L^-1 (r - A x)
+      {
+        HIP_CHECK(lumped_solve_launch(G, xs, s.rhs.p, NM, s.stream));
+        CgWork W = cg_work(d, s.state.p, s.partial.p, s.counter.p, s.set.mass);
+        W.nv     = NM;
+        for (int a = 0; a < NM; ++a)
+          {
+            W.block[a]       = a;
+            W.vec[a].e[0]    = e[2 * (a + 1)];
+            W.vec[a].e[1]    = e[2 * (a + 1) + 1];
+            W.vec[a].coef    = 0.;
+          }
+        for (int k = 0; k < s.set.mass.richardson; ++k)
+          {
+            HIP_CHECK(vmult_launch(G, W, xs, s.ax_tmp.p, s.stream));
+            HIP_CHECK(richardson_update_launch(G, xs, s.rhs.p, s.ax_tmp.p, NM, s.stream));
+          }
+      }
+    else
+      {
+        if (s.extrapolate)
+          HIP_CHECK(extrapolate_launch(G, xs, s.x[1].p, mask, s.stream));
+        CgWork W = cg_work(d, s.state.p, s.partial.p, s.counter.p, s.set.mass);
+        W.nv     = 0;
+        for (int a = 0; a < NM; ++a)
+          if (mask & (1u << a))
+            {
+              W.block[W.nv]      = a;
+              W.vec[W.nv].e[0]   = e[2 * (a + 1)];
+              W.vec[W.nv].e[1]   = e[2 * (a + 1) + 1];
+              W.vec[W.nv].coef   = s.tg3 * s.dt * s.dt;
+              ++W.nv;
+            }
+        int64_t   its = 0;
+        const int st  = run_cg(d, W, xs, s.rhs.p, s.r.p, s.pb[0].p, s.pb[1].p, s.q.p, s.host_state, s.stream, &its,
+                               std::max(1, s.last_its - 1));
+        s.cg_iterations += its;
+        s.last_its = int((its + W.nv - 1) / std::max(1, W.nv));
+        if (st != LBFEM_OK)
+          return st;
+      }
+    s.mark(2);
+    return LBFEM_OK;
+  }
+
+  int
+  step_once(lbfem_scheme &s)
+  {
+    const lbfem_disc &d = *s.d;
+    const Geo        &G = d.G;
+    const bool        ll = s.set.scheme == LBFEM_LEELIN;
+    if (s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2)
+      {
+        // one fused kernel: nodal prologue, advection, M_L^-1, nodal epilogue
+        TileParams S = tile_params(s, s.sweeps[0], ll);
+        S.collide    = 1;
+        s.mark(-1);
+        HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, G, S, s.f[s.cur].p, nullptr, s.f[1 - s.cur].p, nullptr,
+                              s.stream));
+        s.mark(1);
+        s.cur = 1 - s.cur;
+        return LBFEM_OK;
+      }
+    if (ll) // schemes.h:92-98
+      {
+        const int st = increment(s, 0, false);
+        if (st != LBFEM_OK)
+          return st;
+        HIP_CHECK(pc_from_f_launch(G, s.f[s.cur].p, s.x[0].p, s.dt / s.lambda, s.stream));
+        s.mark(0);
+        return LBFEM_OK;
+      }
+    // schemes.h:118-124, streaming.h:88-100: collide (fused into the first advection), stream sweep by sweep
+    for (int sw = 0; sw < int(s.sweeps.size()); ++sw)
+      {
+        const int st = increment(s, sw, sw == 0);
+        if (st != LBFEM_OK)
+          return st;
+        unsigned mask = 0;
+        for (int a = 0; a < NM; ++a)
+          if (moves(s.sweeps[sw], a))
+            mask |= 1u << a;
+        HIP_CHECK(add_increments_launch(G, s.f[s.cur].p, s.x[sw].p, mask, s.stream));
+        s.mark(0);
+      }
+    return LBFEM_OK;
+  }
+
+  template <typename F>
+  int
+  guarded(F &&fn)
+  {
+    try
+      {
+        return fn();
+      }
+    catch (const HipError &e)
+      {
+        return fail(LBFEM_EHIP, e.what());
+      }
+    catch (const std::bad_alloc &)
+      {
+        return fail(LBFEM_EHIP, "out of host memory");
+      }
+    catch (const std::exception &e)
+      {
+        return fail(LBFEM_EINVAL, e.what());
+      }
+  }
+
+  hipStream_t
+  as_stream(void *s)
+  {
+    return reinterpret_cast<hipStream_t>(s);
+  }
+} // namespace
+
+extern "C"
+{
+  size_t
+  lbfem_last_error(char *buf, const size_t len)
+  {
+    if (buf && len)
+      {
+        const size_t k = std::min(len - 1, g_error.size());
+        std::memcpy(buf, g_error.data(), k);
+        buf[k] = '\0';
+      }
+    return g_error.size();
+  }
+
+  int
+  lbfem_device_count(int *count)
+  {
+    return guarded([&]() -> int {
+      int        c = 0;
+      const auto e = hipGetDeviceCount(&c);
+      *count       = e == hipSuccess ? c : 0;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_disc_create(const lbfem_mesh_desc *desc, lbfem_disc **out)
+  {
+    return guarded([&]() -> int {
+      if (!desc || !out)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (desc->n_cells < 2 || desc->degree < 1 || desc->degree > kMaxDegree || desc->n_periodic < 0 ||
+          desc->n_periodic > 2 || !(desc->L > 0.) || desc->stretch < 0.)
+        return fail(LBFEM_EINVAL, "invalid mesh description (N >= 2, degree 1..3, n_periodic 0..2, L > 0)");
+      auto d    = std::make_unique<lbfem_disc>();
+      d->desc   = *desc;
+      d->p      = desc->degree;
+      d->N      = desc->n_cells;
+      d->P      = d->p * d->N;
+      d->np     = d->P + 1;
+      d->n      = int64_t(d->np) * d->np;
+      d->stride = (d->n + 63) / 64 * 64;
+      d->B      = std::make_unique<Basis1D>(d->p);
+      const auto v = box_vertices(d->N, desc->L, desc->stretch);
+      d->ax = std::make_unique<Axis1D>(*d->B, v, desc->n_periodic >= 1);
+      d->ay = std::make_unique<Axis1D>(*d->B, v, desc->n_periodic >= 2);
+      // discretization.h:99-100: h_min = L / N, moved by the clustering
+      d->h_min = v[1] - v[0];
+      d->upload_axis(*d->ax, d->axbuf, d->G.ax);
+      d->upload_axis(*d->ay, d->aybuf, d->G.ay);
+      d->G.p      = d->p;
+      d->G.P      = d->P;
+      d->G.np     = d->np;
+      d->G.stride = d->stride;
+      for (auto &s : d->G.walls.side)
+        s = nullptr;
+      d->G.walls.velocity = nullptr;
+      *out                = d.release();
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_disc_destroy(lbfem_disc *d)
+  {
+    delete d;
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_disc_get_info(const lbfem_disc *d, lbfem_disc_info *info)
+  {
+    if (!d || !info)
+      return fail(LBFEM_EINVAL, "null argument");
+    info->n_nodes   = d->n;
+    const int64_t m = d->ax->periodic ? d->P : d->np, k = d->ay->periodic ? d->P : d->np;
+    info->n_independent = m * k;
+    info->stride        = d->stride;
+    info->n_per_dir     = d->np;
+    info->n_walls       = int32_t(d->wall_nodes.size());
+    info->h_min         = d->h_min;
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_disc_node_coordinates(const lbfem_disc *d, double *xy)
+  {
+    if (!d || !xy)
+      return fail(LBFEM_EINVAL, "null argument");
+    for (int j = 0; j < d->np; ++j)
+      for (int i = 0; i < d->np; ++i)
+        {
+          xy[2 * (int64_t(j) * d->np + i)]     = d->ax->coord[i];
+          xy[2 * (int64_t(j) * d->np + i) + 1] = d->ay->coord[j];
+        }
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_disc_constrained(const lbfem_disc *d, int32_t *flag)
+  {
+    if (!d || !flag)
+      return fail(LBFEM_EINVAL, "null argument");
+    for (int j = 0; j < d->np; ++j)
+      for (int i = 0; i < d->np; ++i)
+        flag[int64_t(j) * d->np + i] = (d->ax->periodic && i == d->P) || (d->ay->periodic && j == d->P);
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_disc_lumped_mass(const lbfem_disc *d, double *ml)
+  {
+    if (!d || !ml)
+      return fail(LBFEM_EINVAL, "null argument");
+    for (int j = 0; j < d->np; ++j)
+      for (int i = 0; i < d->np; ++i)
+        {
+          const bool slave = (d->ax->periodic && i == d->P) || (d->ay->periodic && j == d->P);
+          ml[int64_t(j) * d->np + i] = slave ? 1. : d->ax->row_sum[i] * d->ay->row_sum[j];
+        }
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_disc_mass_diagonal(const lbfem_disc *d, double *diag)
+  {
+    if (!d || !diag)
+      return fail(LBFEM_EINVAL, "null argument");
+    for (int j = 0; j < d->np; ++j)
+      for (int i = 0; i < d->np; ++i)
+        {
+          const bool slave = (d->ax->periodic && i == d->P) || (d->ay->periodic && j == d->P);
+          diag[int64_t(j) * d->np + i] = slave ? 1. : d->ax->diagonal[i] * d->ay->diagonal[j];
+        }
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_disc_set_walls(lbfem_disc *d, const int32_t n, const int32_t *nodes, const double *velocity)
+  {
+    return guarded([&]() -> int {
+      if (!d || n < 0 || (n > 0 && (!nodes || !velocity)))
+        return fail(LBFEM_EINVAL, "null argument");
+      std::vector<int32_t> side[4];
+      bool                 used[4] = {false, false, false, false};
+      for (auto &s : side)
+        s.assign(d->np, -1);
+      for (int32_t k = 0; k < n; ++k)
+        {
+          const int64_t g = nodes[k];
+          if (g < 0 || g >= d->n)
+            return fail(LBFEM_EINVAL, "wall node index out of range");
+          const int i = int(g % d->np), j = int(g / d->np);
+          int       sd = -1;
+          if (!d->ay->periodic && (j == 0 || j == d->P))
+            sd = j == 0 ? 2 : 3;
+          else if (!d->ax->periodic && (i == 0 || i == d->P))
+            sd = i == 0 ? 0 : 1;
+          if (sd < 0)
+            return fail(LBFEM_EINVAL, "wall node " + std::to_string(g) +
+                                        " is not on a non-periodic boundary of the box");
+          side[sd][sd >= 2 ? i : j] = k;
+          used[sd]                  = true;
+        }
+      d->wall_nodes.assign(nodes, nodes + n);
+      d->wall_velocity.assign(velocity, velocity + 2 * size_t(n));
+      d->wall_vel.upload(d->wall_velocity);
+      for (int s = 0; s < 4; ++s)
+        {
+          if (used[s])
+            d->wall_side[s].upload(side[s]);
+          else
+            d->wall_side[s].alloc(0);
+          d->G.walls.side[s] = used[s] ? d->wall_side[s].p : nullptr;
+        }
+      d->G.walls.velocity = d->wall_vel.p;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_disc_time_step(const lbfem_disc *d, const double cfl, double *dt)
+  {
+    if (!d || !dt)
+      return fail(LBFEM_EINVAL, "null argument");
+    *dt = cfl * d->h_min / (d->p * d->p);
+    return LBFEM_OK;
+  }
+
+  // ------------------------------------------------------------------ operators
+  int
+  lbfem_advection_apply(const lbfem_disc *d, const int kind, double *rhs, const double *f, const double *feq,
+                        const double *e, const double dt, const double lambda, const int remove_mass_flux,
+                        void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!d || !rhs || !f || !e)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (kind == LBFEM_LEELIN && !feq)
+        return fail(LBFEM_EINVAL, "LeeLinAdvection needs the equilibria"); // advection.h:267
+      if (kind == LBFEM_LEELIN && !(lambda > 0.))
+        return fail(LBFEM_EINVAL, "lambda must be positive");
+      lbfem_scheme s;
+      s.d      = d;
+      s.dt     = dt;
+      s.lambda = kind == LBFEM_LEELIN ? lambda : 1.;
+      std::array<double, 2 * Q> ee{};
+      std::copy(e, e + 2 * Q, ee.begin());
+      TileParams S       = tile_params(s, ee, kind == LBFEM_LEELIN);
+      S.collide          = 0;
+      S.remove_mass_flux = remove_mass_flux ? 1 : 0;
+      HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, d->G, S, f,
+                            kind == LBFEM_LEELIN ? feq : nullptr, nullptr, rhs, as_stream(stream)));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_collide_bgk(const lbfem_disc *d, double *g, const double omega, void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!d || !g)
+        return fail(LBFEM_EINVAL, "null argument");
+      HIP_CHECK(collide_launch(d->G, g, omega, as_stream(stream)));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_compute_equilibrium(const lbfem_disc *d, const double *f, double *feq, void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!d || !f || !feq)
+        return fail(LBFEM_EINVAL, "null argument");
+      HIP_CHECK(equilibrium_launch(d->G, f, feq, as_stream(stream)));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_predictor_corrector(const lbfem_disc *d, double *f, const double *feq, const double *incr, const double theta,
+                            void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!d || !f || !feq || !incr)
+        return fail(LBFEM_EINVAL, "null argument");
+      HIP_CHECK(predictor_corrector_launch(d->G, f, feq, incr, theta, as_stream(stream)));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_streaming_vmult(const lbfem_disc *d, double *y, const double *x, const double c, const double *e,
+                        void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!d || !y || !x)
+        return fail(LBFEM_EINVAL, "null argument");
+      CgWork W{};
+      W.nv          = 1;
+      W.block[0]    = 0;
+      W.vec[0].coef = c;
+      W.vec[0].e[0] = e ? e[0] : 0.;
+      W.vec[0].e[1] = e ? e[1] : 0.;
+      HIP_CHECK(vmult_launch(d->G, W, x, y, as_stream(stream)));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_mass_solve(const lbfem_disc *dc, const lbfem_mass_settings *m, double *x, const double *r, const double *e,
+                   const double tg3_coefficient, int *iterations, void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!dc || !m || !x || !r)
+        return fail(LBFEM_EINVAL, "null argument");
+      auto       *d = const_cast<lbfem_disc *>(dc); // scratch only
+      hipStream_t st = as_stream(stream);
+      if (iterations)
+        *iterations = 0;
+      if (m->type == LBFEM_MASS_LUMPED)
+        {
+          HIP_CHECK(lumped_solve_launch(d->G, x, r, 1, st));
+          return LBFEM_OK;
+        }
+      if (m->type != LBFEM_MASS_CG && m->type != LBFEM_MASS_RICHARDSON)
+        return fail(LBFEM_EINVAL, "unknown mass type");
+      d->ensure_scratch();
+      if (d->s_r.n < size_t(d->stride))
+        {
+          d->s_r.alloc(d->stride);
+          d->s_p0.alloc(d->stride);
+          d->s_p1.alloc(d->stride);
+          d->s_q.alloc(d->stride);
+          d->s_ax.alloc(d->stride);
+        }
+      CgWork W      = cg_work(*d, d->cg_state.p, d->partial.p, d->counter.p, *m);
+      W.nv          = 1;
+      W.block[0]    = 0;
+      W.vec[0].coef = tg3_coefficient;
+      W.vec[0].e[0] = e ? e[0] : 0.;
+      W.vec[0].e[1] = e ? e[1] : 0.;
+      if (m->type == LBFEM_MASS_RICHARDSON)
+        {
+          HIP_CHECK(lumped_solve_launch(d->G, x, r, 1, st));
+          for (int k = 0; k < m->richardson; ++k)
+            {
+              HIP_CHECK(vmult_launch(d->G, W, x, d->s_ax.p, st));
+              HIP_CHECK(richardson_update_launch(d->G, x, r, d->s_ax.p, 1, st));
+            }
+          return LBFEM_OK;
+        }
+      CgState *hs = nullptr;
+      HIP_CHECK(hipHostMalloc(&hs, sizeof(CgState) * NM));
+      int64_t   its = 0;
+      int       rc  = LBFEM_OK;
+      try
+        {
+          rc = run_cg(*d, W, x, r, d->s_r.p, d->s_p0.p, d->s_p1.p, d->s_q.p, hs, st, &its, 1);
+        }
+      catch (...)
+        {
+          (void)hipHostFree(hs);
+          throw;
+        }
+      (void)hipHostFree(hs);
+      if (iterations)
+        *iterations = int(its);
+      return rc;
+    });
+  }
+
+  // ------------------------------------------------------------------ schemes
+  int
+  lbfem_scheme_create(const lbfem_disc *d, const lbfem_scheme_settings *set, lbfem_scheme **out)
+  {
+    return guarded([&]() -> int {
+      if (!d || !set || !out)
+        return fail(LBFEM_EINVAL, "null argument");
+      const auto &m = set->mass;
+      if (set->scheme != LBFEM_LEELIN && set->scheme != LBFEM_BARDOW)
+        return fail(LBFEM_EINVAL, "unknown scheme");
+      if (set->streaming < LBFEM_TG2 || set->streaming > LBFEM_TG3_SPLIT)
+        return fail(LBFEM_EINVAL, "unknown streaming");
+      if (m.type < LBFEM_MASS_CG || m.type > LBFEM_MASS_RICHARDSON)
+        return fail(LBFEM_EINVAL, "unknown mass type");
+      if (set->streaming != LBFEM_TG2 && m.type != LBFEM_MASS_CG) // streaming.h:51-52
+        return fail(LBFEM_EINVAL, "TG3 streaming needs the consistent mass (Mass::cg)");
+      if (set->scheme == LBFEM_LEELIN && set->streaming != LBFEM_TG2) // schemes.h:88
+        return fail(LBFEM_EINVAL, "LeeLin streams with TG2 only");
+      if (set->precision != LBFEM_FP64)
+        return fail(LBFEM_EINVAL, "only the FP64 form is available");
+      auto s = std::make_unique<lbfem_scheme>();
+      s->d   = d;
+      s->set = *set;
+      HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+      const size_t st = size_t(d->stride);
+      s->f[0].alloc(Q * st);
+      s->f[1].alloc(Q * st);
+      HIP_CHECK(hipMemset(s->f[0].p, 0, Q * st * sizeof(double)));
+      HIP_CHECK(hipMemset(s->f[1].p, 0, Q * st * sizeof(double)));
+      std::array<double, 2 * Q> all{};
+      for (int a = 0; a < Q; ++a)
+        {
+          all[2 * a]     = kE[a][0];
+          all[2 * a + 1] = kE[a][1];
+        }
+      if (set->streaming == LBFEM_TG3_SPLIT) // streaming.h:53-56, 108-115
+        {
+          std::array<double, 2 * Q> ex{}, ey{};
+          for (int a = 0; a < Q; ++a)
+            {
+              ex[2 * a]     = kE[a][0];
+              ey[2 * a + 1] = kE[a][1];
+            }
+          s->sweeps = {ex, ey};
+        }
+      else
+        s->sweeps = {all};
+      s->tg3         = set->streaming == LBFEM_TG2 ? 0. : 1. / 6.; // streaming.h:47
+      s->extrapolate = m.type == LBFEM_MASS_CG && set->streaming != LBFEM_TG3_SPLIT;
+      const bool lumped_fused = m.type == LBFEM_MASS_LUMPED && set->streaming == LBFEM_TG2;
+      if (!lumped_fused)
+        {
+          s->rhs.alloc(NM * st);
+          s->x[0].alloc(NM * st);
+          s->x[1].alloc(NM * st);
+          HIP_CHECK(hipMemset(s->x[0].p, 0, NM * st * sizeof(double)));
+          HIP_CHECK(hipMemset(s->x[1].p, 0, NM * st * sizeof(double)));
+        }
+      if (m.type == LBFEM_MASS_RICHARDSON)
+        s->ax_tmp.alloc(NM * st);
+      if (m.type != LBFEM_MASS_LUMPED)
+        {
+          s->state.alloc(NM);
+          s->partial.alloc(size_t(3 * NM) * kMaxCgBlocks);
+          s->counter.alloc(1);
+          HIP_CHECK(hipMemset(s->counter.p, 0, sizeof(unsigned)));
+        }
+      if (m.type == LBFEM_MASS_CG)
+        {
+          s->r.alloc(NM * st);
+          s->pb[0].alloc(NM * st);
+          s->pb[1].alloc(NM * st);
+          s->q.alloc(NM * st);
+          HIP_CHECK(hipHostMalloc(&s->host_state, sizeof(CgState) * NM));
+        }
+      *out = s.release();
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_destroy(lbfem_scheme *s)
+  {
+    delete s;
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_scheme_set_time_step(lbfem_scheme *s, const double dt, const double lambda)
+  {
+    if (!s)
+      return fail(LBFEM_EINVAL, "null argument");
+    if (!(dt > 0.) || !(lambda > 0.))
+      return fail(LBFEM_EINVAL, "dt and lambda must be positive");
+    s->dt     = dt;
+    s->lambda = lambda;
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_scheme_set_populations(lbfem_scheme *s, const double *f)
+  {
+    return guarded([&]() -> int {
+      if (!s || !f)
+        return fail(LBFEM_EINVAL, "null argument");
+      const int64_t n = s->d->n;
+      for (int a = 0; a < Q; ++a)
+        HIP_CHECK(hipMemcpyAsync(s->f[s->cur].p + a * s->d->stride, f + a * n, n * sizeof(double),
+                                 hipMemcpyHostToDevice, s->stream));
+      HIP_CHECK(hipStreamSynchronize(s->stream));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_set_initial_populations(lbfem_scheme *s, const double *f0)
+  {
+    return guarded([&]() -> int {
+      if (!s || !f0)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (s->set.scheme == LBFEM_LEELIN)
+        return lbfem_scheme_set_populations(s, f0);
+      if (!(s->dt > 0.) || !(s->lambda > 0.))
+        return fail(LBFEM_EINVAL, "set the time step first (schemes.h:128-141 uses dt and lambda)");
+      // g = f + dt/(2 lambda) (f - feq(moments(f))) (schemes.h:128-141), on the host
+      const int64_t       n     = s->d->n;
+      const double        scale = 0.5 * s->dt / s->lambda;
+      std::vector<double> g(Q * size_t(n));
+      for (int64_t i = 0; i < n; ++i)
+        {
+          double fi[Q];
+          for (int a = 0; a < Q; ++a)
+            fi[a] = f0[a * n + i];
+          double rho = 0., mx = 0., my = 0.;
+          for (int a = 0; a < Q; ++a)
+            {
+              rho += fi[a];
+              mx += kE[a][0] * fi[a];
+              my += kE[a][1] * fi[a];
+            }
+          const double ux = mx / rho, uy = my / rho, uu = ux * ux + uy * uy;
+          for (int a = 0; a < Q; ++a)
+            {
+              const double eu = kE[a][0] * ux + kE[a][1] * uy;
+              const double fe = w_of(a) * rho * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * uu);
+              g[a * n + i]    = fi[a] + scale * (fi[a] - fe);
+            }
+        }
+      return lbfem_scheme_set_populations(s, g.data());
+    });
+  }
+
+  int
+  lbfem_scheme_get_populations(const lbfem_scheme *s, double *f)
+  {
+    return guarded([&]() -> int {
+      if (!s || !f)
+        return fail(LBFEM_EINVAL, "null argument");
+      const int64_t n = s->d->n;
+      for (int a = 0; a < Q; ++a)
+        HIP_CHECK(hipMemcpyAsync(f + a * n, s->f[s->cur].p + a * s->d->stride, n * sizeof(double),
+                                 hipMemcpyDeviceToHost, s->stream));
+      HIP_CHECK(hipStreamSynchronize(s->stream));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_device_populations(lbfem_scheme *s, double **f)
+  {
+    if (!s || !f)
+      return fail(LBFEM_EINVAL, "null argument");
+    *f = s->f[s->cur].p;
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_scheme_step(lbfem_scheme *s, const int n_steps)
+  {
+    return guarded([&]() -> int {
+      if (!s || n_steps < 0)
+        return fail(LBFEM_EINVAL, "invalid argument");
+      if (!(s->dt > 0.) || !(s->lambda > 0.))
+        return fail(LBFEM_EINVAL, "set_time_step first");
+      for (int k = 0; k < n_steps; ++k)
+        {
+          const int st = step_once(*s);
+          if (st != LBFEM_OK)
+            return st;
+          if (s->timing && s->marks.size() > 256)
+            s->resolve_marks();
+        }
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_synchronize(lbfem_scheme *s)
+  {
+    return guarded([&]() -> int {
+      if (!s)
+        return fail(LBFEM_EINVAL, "null argument");
+      HIP_CHECK(hipStreamSynchronize(s->stream));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_cg_iterations(const lbfem_scheme *s, int64_t *its)
+  {
+    if (!s || !its)
+      return fail(LBFEM_EINVAL, "null argument");
+    *its = s->cg_iterations;
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_scheme_enable_stage_timers(lbfem_scheme *s, const int enable)
+  {
+    return guarded([&]() -> int {
+      if (!s)
+        return fail(LBFEM_EINVAL, "null argument");
+      s->resolve_marks();
+      s->timing = enable != 0;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_stage_times(const lbfem_scheme *sc, double *seconds4)
+  {
+    return guarded([&]() -> int {
+      if (!sc || !seconds4)
+        return fail(LBFEM_EINVAL, "null argument");
+      auto *s = const_cast<lbfem_scheme *>(sc);
+      s->resolve_marks();
+      for (int k = 0; k < 4; ++k)
+        seconds4[k] = s->stage[k];
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_diagnostics(const lbfem_disc *dc, const double *f, const double *u_exact, double *u_prev, double *sums,
+                    void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!dc || !f || !sums)
+        return fail(LBFEM_EINVAL, "null argument");
+      auto *d = const_cast<lbfem_disc *>(dc);
+      d->ensure_scratch();
+      DevBuf<double> dsum;
+      dsum.alloc(6);
+      hipStream_t st = as_stream(stream);
+      HIP_CHECK(diagnostics_launch(d->G, f, u_exact, u_prev, dsum.p, d->partial.p, d->counter.p, kMaxCgBlocks, st));
+      HIP_CHECK(hipMemcpyAsync(sums, dsum.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      return LBFEM_OK;
+    });
+  }
+}

@@ -1,0 +1,91 @@
+// Host-side launch interface of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace lbfem_amd
+{
+  // modes of the tile kernel
+  enum TileMode : int
+  {
+    kStepBardow   = 0, // collide, advect, lumped solve, g += x  (one fused step)
+    kStepLeeLin   = 1, // equilibria, advect, lumped solve, predictor-corrector
+    kAdvectBardow = 2, // (optional collision, writing g*) + right-hand side
+    kAdvectLeeLin = 3, // right-hand side of LeeLinAdvection (feq given or computed)
+  };
+
+  struct TileParams
+  {
+    double e[Q][2];           // directions of the sweep
+    double dt, a, b, c;       // advection coefficients (advection.h:223, 268-270)
+    double omega;             // BGK rate (schemes.h:122)
+    double theta;             // Lee-Lin dt / lambda (schemes.h:97)
+    int    collide;           // Bardow: collide on load
+    int    remove_mass_flux;  // wall surface term option (advection.h:182-199)
+    int    tiles_x, tiles_y;  // set by the launcher
+  };
+
+  hipError_t tile_launch(int mode, const Geo &G, const TileParams &S, const double *fin, const double *feq,
+                         double *out, double *rhs, hipStream_t stream);
+
+  // nodal operations over every node (collision.h:78-168)
+  hipError_t collide_launch(const Geo &G, double *g, double omega, hipStream_t stream);
+  hipError_t equilibrium_launch(const Geo &G, const double *f, double *feq, hipStream_t stream);
+  hipError_t predictor_corrector_launch(const Geo &G, double *f, const double *feq, const double *incr, double theta,
+                                        hipStream_t stream);
+  // Lee-Lin nodal update after a consistent-mass solve: f <- PC(f, feq(f), x)
+  hipError_t pc_from_f_launch(const Geo &G, double *f, const double *incr, double theta, hipStream_t stream);
+  // f_{a+1} += x_a for the populations flagged in mask (streaming.h:94-98)
+  hipError_t add_increments_launch(const Geo &G, double *f, const double *x, unsigned mask, hipStream_t stream);
+
+  // ---- consistent-mass solves: block CG over up to 8 vectors sharing the mesh
+  // (one SolverCG per vector, include/lbfem/mass.h:204-211, run side by side).
+  struct CgVector
+  {
+    double e[2];  // direction of the K_e term
+    double coef;  // c of A = M + c K_e
+  };
+
+  struct CgState // device, one per vector
+  {
+    double rho, rho_prev, alpha, beta, res, tol;
+    int    its, status; // status: 0 active, 1 converged, 2 failed (NoConvergence)
+  };
+
+  struct CgWork
+  {
+    int       nv;         // vectors
+    int       block[NM];  // block of the increment/rhs arrays vector v lives in
+    int       max_its;
+    double    tol_rel;
+    CgVector  vec[NM];
+    CgState  *state;      // device [nv]
+    double   *partial;    // device [nv * 4 * max_blocks]
+    unsigned *counter;    // device, zero between uses
+    int       max_blocks;
+  };
+
+  // r = A x - b, tolerances, rho = r.D^-1 r (iteration 0 of SolverCG)
+  hipError_t cg_start_launch(const Geo &G, const CgWork &W, const double *x, const double *b, double *r,
+                             hipStream_t stream);
+  // p_new = beta p_old - D^-1 r (p_new = -D^-1 r at it == 1); q = A p_new; alpha = rho / p.q
+  hipError_t cg_direction_launch(const Geo &G, const CgWork &W, int it, const double *p_old, const double *r,
+                                 double *p_new, double *q, hipStream_t stream);
+  // x += alpha p; r += alpha q; res = |r|; stopping test; rho = r.D^-1 r, beta
+  hipError_t cg_update_launch(const Geo &G, const CgWork &W, int it, double *x, double *r, const double *p,
+                              const double *q, hipStream_t stream);
+  // y = (M + c K_e) x for nv vectors (constrained rows the identity)
+  hipError_t vmult_launch(const Geo &G, const CgWork &W, const double *x, double *y, hipStream_t stream);
+  // x_guess = 2 x - x_prev, x_prev = x (the extrapolated start, streaming.h:76-80), masked
+  hipError_t extrapolate_launch(const Geo &G, double *x, double *x_prev, unsigned mask, hipStream_t stream);
+  // lumped-type solves: x = M_L^-1 r, and k Richardson passes x += M_L^-1 (r - A x)
+  hipError_t lumped_solve_launch(const Geo &G, double *x, const double *r, int nv, hipStream_t stream);
+  hipError_t richardson_update_launch(const Geo &G, double *x, const double *r, const double *Ax, int nv,
+                                      hipStream_t stream);
+
+  // diagnostics sums over non-slave nodes (cgdbe.cc:353-368)
+  hipError_t diagnostics_launch(const Geo &G, const double *f, const double *u_exact, double *u_prev, double *sums,
+                                double *partial, unsigned *counter, int max_blocks, hipStream_t stream);
+} // namespace lbfem_amd

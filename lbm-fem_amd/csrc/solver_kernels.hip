@@ -1,0 +1,690 @@
+// Nodal kernels and the consistent-mass solves.
+//
+// The mass matrix of a cartesian Q_p mesh is Mx (x) My, and the TG3 matrix
+// M + c K_e adds Kronecker terms of the same 1-D bands, so a matrix-vector
+// product is the tile pass of the advection kernel with other coefficients.
+// CG runs for up to eight right-hand sides side by side (one per moving
+// population, as the reference's eight sequential SolverCG solves): each
+// vector keeps its own alpha, beta, residual and stopping test in device
+// memory, and a converged vector is skipped from then on, so that every vector
+// takes exactly the iterations its own solve would.
+//
+// Reductions: wavefront shuffles and LDS inside a block, one partial per block,
+// and the last block to finish sums the partials in block order (a fixed
+// order, so results do not depend on scheduling) and updates the scalars.
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace lbfem_amd
+{
+  namespace
+  {
+    constexpr int kThreads = 256;
+
+    template <int p>
+    __device__ __forceinline__ double
+    band(const double *B, const int i, const int o)
+    {
+      return B[i * (2 * p + 1) + o + p];
+    }
+
+    __device__ __forceinline__ int
+    wrap(const int i, const int P, const int periodic)
+    {
+      if (periodic)
+        return ((i % P) + P) % P;
+      return (i < 0 || i > P) ? -1 : i;
+    }
+
+    __device__ __forceinline__ bool
+    is_slave(const Geo &G, const int i, const int j)
+    {
+      return (G.ax.periodic && i == G.P) || (G.ay.periodic && j == G.P);
+    }
+
+    // the Jacobi diagonal diag(M) = diag(Mx) diag(My), 1 on constrained rows
+    __device__ __forceinline__ double
+    mass_diag(const Geo &G, const int i, const int j)
+    {
+      return is_slave(G, i, j) ? 1. : G.ax.diag[i] * G.ay.diag[j];
+    }
+
+    // sum over the block of up to K values per thread; result valid in thread 0
+    template <int K>
+    __device__ __forceinline__ void
+    block_sum(double (&v)[K], double (*lds)[kThreads / 64])
+    {
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        {
+          double s = v[k];
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1)
+            s += __shfl_down(s, off, 64);
+          if (lane == 0)
+            lds[k][wave] = s;
+        }
+      __syncthreads();
+      if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          {
+            double s = 0.;
+#pragma unroll
+            for (int w = 0; w < kThreads / 64; ++w)
+              s += lds[k][w];
+            v[k] = s;
+          }
+      __syncthreads();
+    }
+
+    // Last-block-done: returns true in the one block that sees all partials.
+    __device__ __forceinline__ bool
+    last_block(unsigned *counter, const unsigned nblocks)
+    {
+      __shared__ bool last;
+      __threadfence();
+      __syncthreads();
+      if (threadIdx.x == 0)
+        {
+          const unsigned t = atomicAdd(counter, 1u);
+          last             = (t == nblocks - 1);
+        }
+      __syncthreads();
+      if (last)
+        __threadfence();
+      return last;
+    }
+
+    // fixed-order sum of partial[0..n) by the whole block
+    __device__ __forceinline__ double
+    sum_partials(const double *partial, const int n, double (*lds)[kThreads / 64])
+    {
+      double s[1] = {0.};
+      for (int b = threadIdx.x; b < n; b += kThreads)
+        s[0] += partial[b];
+      block_sum<1>(s, lds);
+      return s[0];
+    }
+  } // namespace
+
+  // ------------------------------------------------------------------ nodal kernels
+  __global__ void __launch_bounds__(kThreads)
+  k_collide(const Geo G, double *g, const double omega)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        double    v[Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          v[a] = g[a * G.stride + k];
+        double     wx = 0., wy = 0.;
+        const bool wall = wall_of(G, i, j, wx, wy);
+        collide_node(v, omega, wall, wx, wy);
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          g[a * G.stride + k] = v[a];
+      }
+  }
+
+  __global__ void __launch_bounds__(kThreads)
+  k_equilibrium(const Geo G, const double *f, double *feq)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        double    v[Q], e[Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          v[a] = f[a * G.stride + k];
+        double     wx = 0., wy = 0.;
+        const bool wall = wall_of(G, i, j, wx, wy);
+        equilibrium_node(v, e, wall, wx, wy);
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          feq[a * G.stride + k] = e[a];
+      }
+  }
+
+  __global__ void __launch_bounds__(kThreads)
+  k_predictor_corrector(const Geo G, double *f, const double *feq, const double *incr, const double theta)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        double    v[Q], e[Q], x[NM];
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          {
+            v[a] = f[a * G.stride + k];
+            e[a] = feq[a * G.stride + k];
+          }
+#pragma unroll
+        for (int a = 0; a < NM; ++a)
+          x[a] = incr[a * G.stride + k];
+        double     wx = 0., wy = 0.;
+        const bool wall = wall_of(G, i, j, wx, wy);
+        predictor_corrector_node(v, e, x, theta, wall, wx, wy);
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          f[a * G.stride + k] = v[a];
+      }
+  }
+
+  // f <- predictor-corrector(f, feq(f), x) in place: the Lee-Lin nodal update
+  // after a consistent-mass solve, with the equilibria of f^n rebuilt
+  // (collision.h:105-145)
+  __global__ void __launch_bounds__(kThreads)
+  k_pc_from_f(const Geo G, double *f, const double *incr, const double theta)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        double    v[Q], e[Q], x[NM];
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          v[a] = f[a * G.stride + k];
+#pragma unroll
+        for (int a = 0; a < NM; ++a)
+          x[a] = incr[a * G.stride + k];
+        double     wx = 0., wy = 0.;
+        const bool wall = wall_of(G, i, j, wx, wy);
+        equilibrium_node(v, e, wall, wx, wy);
+        predictor_corrector_node(v, e, x, theta, wall, wx, wy);
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          f[a * G.stride + k] = v[a];
+      }
+  }
+
+  __global__ void __launch_bounds__(kThreads)
+  k_add_increments(const Geo G, double *f, const double *x, const unsigned mask)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+#pragma unroll
+      for (int a = 0; a < NM; ++a)
+        if (mask & (1u << a))
+          f[(a + 1) * G.stride + k] += x[a * G.stride + k];
+  }
+
+  __global__ void __launch_bounds__(kThreads)
+  k_extrapolate(const Geo G, double *x, double *x_prev, const unsigned mask)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+#pragma unroll
+      for (int a = 0; a < NM; ++a)
+        if (mask & (1u << a))
+          {
+            const double xa = x[a * G.stride + k], xp = x_prev[a * G.stride + k];
+            x[a * G.stride + k]      = -xp + 2. * xa; // sadd(-1, 2, xa), then swap
+            x_prev[a * G.stride + k] = xa;
+          }
+  }
+
+  __global__ void __launch_bounds__(kThreads)
+  k_lumped_solve(const Geo G, double *x, const double *r, const int nv)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int    i = int(k % G.np), j = int(k / G.np);
+        const double m = is_slave(G, i, j) ? 1. : G.ax.ml_inv[i] * G.ay.ml_inv[j];
+        for (int v = 0; v < nv; ++v)
+          x[v * G.stride + k] = m * r[v * G.stride + k];
+      }
+  }
+
+  __global__ void __launch_bounds__(kThreads)
+  k_richardson_update(const Geo G, double *x, const double *r, const double *Ax, const int nv)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int    i = int(k % G.np), j = int(k / G.np);
+        const double m = is_slave(G, i, j) ? 1. : G.ax.ml_inv[i] * G.ay.ml_inv[j];
+        for (int v = 0; v < nv; ++v)
+          x[v * G.stride + k] += m * (r[v * G.stride + k] - Ax[v * G.stride + k]);
+      }
+  }
+
+  // ------------------------------------------------------------------ operator tile
+  // KIND 0: y = A x.  KIND 1 (CG start): r = A x - b, partials |r|^2, |b|^2, r.D^-1 r.
+  // KIND 2 (CG direction): p_new = beta p_old - D^-1 r, q = A p_new, partial p_new.q.
+  template <int p, int TX, int TY, int KIND>
+  __global__ void __launch_bounds__(kThreads)
+  k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const double *x,
+             const double *b, const double *r_in, double *y, double *y2)
+  {
+    constexpr int HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
+    constexpr int RO = (TX * TY) / kThreads;
+    __shared__ double fld[HY][HX];
+    __shared__ double yv[4][TY][HX];
+    __shared__ double red[3 * NM][kThreads / 64];
+
+    const int nt   = tiles_x * tiles_y;
+    const int nb8  = (nt + 7) & ~7;
+    const int bid  = blockIdx.x;
+    const int tile = (bid & 7) * (nb8 >> 3) + (bid >> 3);
+    if (tile >= nt)
+      return;
+    const int i0 = (tile % tiles_x) * TX, j0 = (tile / tiles_x) * TY;
+    const int P = G.P, tid = threadIdx.x;
+
+    double part[3 * NM];
+#pragma unroll
+    for (int k = 0; k < 3 * NM; ++k)
+      part[k] = 0.;
+
+    for (int v = 0; v < W.nv; ++v)
+      {
+        const bool active = KIND == 0 || W.state[v].status == 0;
+        if (!active) // uniform over the grid
+          continue;
+        const double   c  = W.vec[v].coef, ex = W.vec[v].e[0], ey = W.vec[v].e[1];
+        const int64_t  vb = int64_t(W.block[v]) * G.stride;
+        const double  *xv = x + vb;
+        const double   beta = (KIND == 2 && it > 1) ? W.state[v].beta : 0.;
+        // stage the operand of the tile (KIND 2: the new direction)
+        for (int k = tid; k < NH; k += kThreads)
+          {
+            const int kx = k % HX, ky = k / HX;
+            const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
+            double    val = 0.;
+            if (gi >= 0 && gj >= 0)
+              {
+                const int64_t idx = node_index(G, gi, gj);
+                if constexpr (KIND == 2)
+                  {
+                    const double z = r_in[vb + idx] / mass_diag(G, gi, gj);
+                    val            = it > 1 ? beta * xv[idx] - z : -z;
+                  }
+                else
+                  val = xv[idx];
+              }
+            fld[ky][kx] = val;
+          }
+        __syncthreads();
+        const bool cx = c != 0. && ex != 0., cy = c != 0. && ey != 0.;
+        for (int idx = tid; idx < TY * HX; idx += kThreads)
+          {
+            const int row = idx / HX, col = idx % HX;
+            const int gj  = j0 + row;
+            if (gj > P)
+              continue;
+            double My = 0., Ky = 0., Dy = 0., DTy = 0.;
+#pragma unroll
+            for (int o = -p; o <= p; ++o)
+              {
+                const double fv = fld[row + p + o][col];
+                My += band<p>(G.ay.M, gj, o) * fv;
+                if (cy)
+                  Ky += band<p>(G.ay.K, gj, o) * fv;
+                if (cx && cy)
+                  {
+                    Dy += band<p>(G.ay.D, gj, o) * fv;
+                    DTy += band<p>(G.ay.DT, gj, o) * fv;
+                  }
+              }
+            // A x = Mx (My x + c ey^2 Ky x) + Kx (c ex^2 My x) + Dx^T (c ex ey Dy x) + Dx (c ex ey Dy^T x)
+            yv[0][row][col] = My + c * ey * ey * Ky;
+            yv[1][row][col] = c * ex * ex * My;
+            yv[2][row][col] = c * ex * ey * Dy;
+            yv[3][row][col] = c * ex * ey * DTy;
+          }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+          {
+            const int ii = tid % TX, jj = tid / TX + r * (kThreads / TX);
+            const int gi = i0 + ii, gj = j0 + jj;
+            if (gi > P || gj > P)
+              continue;
+            double acc = 0.;
+#pragma unroll
+            for (int o = -p; o <= p; ++o)
+              {
+                const int cc = ii + p + o;
+                acc += band<p>(G.ax.M, gi, o) * yv[0][jj][cc];
+                if (cx)
+                  acc += band<p>(G.ax.K, gi, o) * yv[1][jj][cc];
+                if (cx && cy)
+                  acc += band<p>(G.ax.DT, gi, o) * yv[2][jj][cc] + band<p>(G.ax.D, gi, o) * yv[3][jj][cc];
+              }
+            const int64_t idx   = node_index(G, gi, gj);
+            const bool    slave = is_slave(G, gi, gj);
+            double        own   = fld[jj + p][ii + p];
+            if (slave) // constrained rows are the identity (MatrixFreeOperators::Base::vmult),
+              {        // on the slave's own entry (the staged tile holds its master's)
+                if constexpr (KIND == 2)
+                  own = it > 1 ? beta * xv[idx] - r_in[vb + idx] : -r_in[vb + idx];
+                else
+                  own = xv[idx];
+                acc = own;
+              }
+            if constexpr (KIND == 0)
+              y[vb + idx] = acc;
+            else if constexpr (KIND == 1)
+              {
+                const double bb = b[vb + idx];
+                const double rr = acc - bb;
+                y[vb + idx] = rr;
+                part[3 * v] += rr * rr;
+                part[3 * v + 1] += bb * bb;
+                part[3 * v + 2] += rr * (rr / mass_diag(G, gi, gj));
+              }
+            else
+              {
+                y[vb + idx]  = own; // p_new
+                y2[vb + idx] = acc; // q = A p_new
+                part[3 * v] += own * acc;
+              }
+          }
+        __syncthreads();
+      }
+    if constexpr (KIND == 0)
+      return;
+    else
+      {
+        block_sum<3 * NM>(part, red);
+        if (tid == 0)
+          for (int k = 0; k < 3 * NM; ++k)
+            W.partial[size_t(k) * W.max_blocks + tile] = part[k];
+        if (!last_block(W.counter, nt))
+          return;
+        __shared__ double tot[3 * NM];
+        for (int v = 0; v < W.nv; ++v)
+          {
+            if (KIND == 2 && W.state[v].status != 0)
+              continue;
+            const int nk = KIND == 1 ? 3 : 1;
+            for (int k = 0; k < nk; ++k)
+              {
+                const double s = sum_partials(W.partial + size_t(3 * v + k) * W.max_blocks, nt, red);
+                if (tid == 0)
+                  tot[3 * v + k] = s;
+              }
+          }
+        if (tid == 0)
+          {
+            for (int v = 0; v < W.nv; ++v)
+              {
+                CgState &st = W.state[v];
+                if constexpr (KIND == 1)
+                  {
+                    st.res      = sqrt(tot[3 * v]);
+                    st.tol      = W.tol_rel * sqrt(tot[3 * v + 1]);
+                    st.rho      = tot[3 * v + 2];
+                    st.rho_prev = 0.;
+                    st.its      = 0;
+                    // SolverControl::check(0, res): success first, then failure at the cap
+                    st.status = st.res <= st.tol ? 1 : (W.max_its <= 0 || isnan(st.res)) ? 2 : 0;
+                  }
+                else
+                  {
+                    if (st.status != 0)
+                      continue;
+                    st.alpha = st.rho / tot[3 * v];
+                  }
+              }
+            *W.counter = 0u;
+          }
+      }
+  }
+
+  // x += alpha p; r += alpha q; res = |r|, the stopping test; rho = r.D^-1 r; beta
+  __global__ void __launch_bounds__(kThreads)
+  k_cg_update(const Geo G, const CgWork W, const int it, double *x, double *r, const double *p, const double *q)
+  {
+    __shared__ double red[2 * NM][kThreads / 64];
+    const int64_t     n = int64_t(G.np) * G.np;
+    double            part[2 * NM];
+#pragma unroll
+    for (int k = 0; k < 2 * NM; ++k)
+      part[k] = 0.;
+    double alpha[NM];
+    bool   act[NM];
+#pragma unroll
+    for (int v = 0; v < NM; ++v)
+      {
+        act[v]   = v < W.nv && W.state[v].status == 0;
+        alpha[v] = act[v] ? W.state[v].alpha : 0.;
+      }
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int    i = int(k % G.np), j = int(k / G.np);
+        const double d = mass_diag(G, i, j);
+#pragma unroll
+        for (int v = 0; v < NM; ++v)
+          if (act[v])
+            {
+              const int64_t o  = int64_t(W.block[v]) * G.stride + k;
+              x[o]             = x[o] + alpha[v] * p[o];
+              const double rr  = r[o] + alpha[v] * q[o];
+              r[o]             = rr;
+              part[2 * v] += rr * rr;
+              part[2 * v + 1] += rr * (rr / d);
+            }
+      }
+    block_sum<2 * NM>(part, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 2 * NM; ++k)
+        W.partial[size_t(k) * W.max_blocks + blockIdx.x] = part[k];
+    if (!last_block(W.counter, gridDim.x))
+      return;
+    __shared__ double tot[2 * NM];
+    for (int v = 0; v < W.nv; ++v)
+      {
+        if (!act[v])
+          continue;
+        for (int k = 0; k < 2; ++k)
+          {
+            const double s = sum_partials(W.partial + size_t(2 * v + k) * W.max_blocks, gridDim.x, red);
+            if (threadIdx.x == 0)
+              tot[2 * v + k] = s;
+          }
+      }
+    if (threadIdx.x == 0)
+      {
+        for (int v = 0; v < W.nv; ++v)
+          {
+            if (!act[v])
+              continue;
+            CgState &st = W.state[v];
+            st.res      = sqrt(fabs(tot[2 * v]));
+            st.its      = it;
+            if (st.res <= st.tol)
+              st.status = 1;
+            else if (it >= W.max_its || isnan(st.res))
+              st.status = 2;
+            st.rho_prev = st.rho;
+            st.rho      = tot[2 * v + 1];
+            st.beta     = st.rho / st.rho_prev;
+          }
+        *W.counter = 0u;
+      }
+  }
+
+  // ------------------------------------------------------------------ diagnostics
+  __global__ void __launch_bounds__(kThreads)
+  k_diagnostics(const Geo G, const double *f, const double *u_exact, double *u_prev, double *sums, double *partial,
+                unsigned *counter, const int max_blocks)
+  {
+    __shared__ double red[6][kThreads / 64];
+    const int64_t     n = int64_t(G.np) * G.np;
+    double            s[6] = {0., 0., 0., 0., 0., 0.};
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        if (is_slave(G, i, j))
+          continue;
+        double v[Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          v[a] = f[a * G.stride + k];
+        double rho, ux, uy;
+        moments(v, rho, ux, uy);
+        const double w   = G.ax.ml[i] * G.ay.ml[j];
+        const double exu = u_exact ? u_exact[2 * k] : 0., exv = u_exact ? u_exact[2 * k + 1] : 0.;
+        s[0] += w * ((ux - exu) * (ux - exu) + (uy - exv) * (uy - exv));
+        s[1] += w * (exu * exu + exv * exv);
+        s[2] += w * (ux * ux + uy * uy);
+        s[3] += w * rho;
+        s[4] += w;
+        if (u_prev)
+          {
+            s[5] += w * hypot(ux - u_prev[2 * k], uy - u_prev[2 * k + 1]);
+            u_prev[2 * k]     = ux;
+            u_prev[2 * k + 1] = uy;
+          }
+      }
+    block_sum<6>(s, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 6; ++k)
+        partial[size_t(k) * max_blocks + blockIdx.x] = s[k];
+    if (!last_block(counter, gridDim.x))
+      return;
+    for (int k = 0; k < 6; ++k)
+      {
+        const double t = sum_partials(partial + size_t(k) * max_blocks, gridDim.x, red);
+        if (threadIdx.x == 0)
+          sums[k] = t;
+      }
+    if (threadIdx.x == 0)
+      *counter = 0u;
+  }
+
+  // ------------------------------------------------------------------ launchers
+  static int
+  nodal_blocks(const Geo &G)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    const int64_t b = (n + kThreads - 1) / kThreads;
+    return int(b < 4096 ? b : 4096);
+  }
+
+  hipError_t
+  collide_launch(const Geo &G, double *g, const double omega, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_collide, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, g, omega);
+    return hipGetLastError();
+  }
+  hipError_t
+  equilibrium_launch(const Geo &G, const double *f, double *feq, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_equilibrium, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, f, feq);
+    return hipGetLastError();
+  }
+  hipError_t
+  predictor_corrector_launch(const Geo &G, double *f, const double *feq, const double *incr, const double theta,
+                             hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_predictor_corrector, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, f, feq, incr, theta);
+    return hipGetLastError();
+  }
+  hipError_t
+  pc_from_f_launch(const Geo &G, double *f, const double *incr, const double theta, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_pc_from_f, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, f, incr, theta);
+    return hipGetLastError();
+  }
+  hipError_t
+  add_increments_launch(const Geo &G, double *f, const double *x, const unsigned mask, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_add_increments, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, f, x, mask);
+    return hipGetLastError();
+  }
+  hipError_t
+  extrapolate_launch(const Geo &G, double *x, double *x_prev, const unsigned mask, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_extrapolate, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, x_prev, mask);
+    return hipGetLastError();
+  }
+  hipError_t
+  lumped_solve_launch(const Geo &G, double *x, const double *r, const int nv, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_lumped_solve, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, r, nv);
+    return hipGetLastError();
+  }
+  hipError_t
+  richardson_update_launch(const Geo &G, double *x, const double *r, const double *Ax, const int nv, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_richardson_update, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, r, Ax, nv);
+    return hipGetLastError();
+  }
+
+  template <int KIND>
+  static hipError_t
+  operator_launch(const Geo &G, const CgWork &W, const int it, const double *x, const double *b, const double *r,
+                  double *y, double *y2, hipStream_t s)
+  {
+    constexpr int TX = 32, TY = 16;
+    const int     tx = (G.P + 1 + TX - 1) / TX, ty = (G.P + 1 + TY - 1) / TY;
+    const int     nt = tx * ty, nb8 = (nt + 7) & ~7;
+    if (KIND != 0 && nt > W.max_blocks)
+      return hipErrorInvalidValue;
+    switch (G.p)
+      {
+        case 1:
+          hipLaunchKernelGGL((k_operator<1, TX, TY, KIND>), dim3(nb8), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
+                             y, y2);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_operator<2, TX, TY, KIND>), dim3(nb8), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
+                             y, y2);
+          break;
+        case 3:
+          hipLaunchKernelGGL((k_operator<3, TX, TY, KIND>), dim3(nb8), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
+                             y, y2);
+          break;
+        default:
+          return hipErrorInvalidValue;
+      }
+    return hipGetLastError();
+  }
+
+  hipError_t
+  vmult_launch(const Geo &G, const CgWork &W, const double *x, double *y, hipStream_t s)
+  {
+    return operator_launch<0>(G, W, 0, x, nullptr, nullptr, y, nullptr, s);
+  }
+  hipError_t
+  cg_start_launch(const Geo &G, const CgWork &W, const double *x, const double *b, double *r, hipStream_t s)
+  {
+    return operator_launch<1>(G, W, 0, x, b, nullptr, r, nullptr, s);
+  }
+  hipError_t
+  cg_direction_launch(const Geo &G, const CgWork &W, const int it, const double *p_old, const double *r,
+                      double *p_new, double *q, hipStream_t s)
+  {
+    return operator_launch<2>(G, W, it, p_old, nullptr, r, p_new, q, s);
+  }
+  hipError_t
+  cg_update_launch(const Geo &G, const CgWork &W, const int it, double *x, double *r, const double *p, const double *q,
+                   hipStream_t s)
+  {
+    int nb = nodal_blocks(G);
+    if (nb > W.max_blocks)
+      nb = W.max_blocks;
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, G, W, it, x, r, p, q);
+    return hipGetLastError();
+  }
+  hipError_t
+  diagnostics_launch(const Geo &G, const double *f, const double *u_exact, double *u_prev, double *sums,
+                     double *partial, unsigned *counter, const int max_blocks, hipStream_t s)
+  {
+    int nb = nodal_blocks(G);
+    if (nb > max_blocks)
+      nb = max_blocks;
+    hipLaunchKernelGGL(k_diagnostics, dim3(nb), dim3(kThreads), 0, s, G, f, u_exact, u_prev, sums, partial, counter,
+                       max_blocks);
+    return hipGetLastError();
+  }
+} // namespace lbfem_amd

@@ -1,0 +1,218 @@
+"""Parity of the HIP path (through the C ABI of liblbfem_hip.so) with the CPU
+oracle, on the same meshes, flows and seeded inputs. FP64 throughout; the
+tolerances are those the north-star parity asks for: <= 1e-12 relative in the
+populations per step, <= 1e-10 after many steps of a stable configuration,
+CG iteration counts within +-1 per solve.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # collected on CPU hosts, skipped there
+    pytest.skip("no GPU", allow_module_level=True)
+
+import lbfem_amd as A  # noqa: E402
+import oracle as O  # noqa: E402
+
+CASES = {"tgv": O.TGV, "couette": O.COUETTE, "cavity": O.CAVITY}
+
+
+def pair(case, N, p, Re=100.0, mach=0.1, modes=(1, 1), stretch=1.2):
+    tc = A.test_cases.make(case, mach=mach, modes=modes, stretch=stretch)
+    tc.set_reynolds(Re)
+    gd = A.Discretization(tc, N, p)
+    od = O.Disc(N, p, CASES[case], mach=mach, modes=modes, stretch=stretch if case == "cavity" else 0.0,
+                reynolds=Re)
+    return tc, gd, od
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def run_both(case, N, p, scheme, mass, steps, cfl=0.25, streaming=A.TG2, cg_tol=1e-12, Re=100.0, richardson=2):
+    tc, gd, od = pair(case, N, p, Re=Re)
+    dt = od.time_step(cfl)
+    lam = od.relaxation_time
+    cls = A.Bardow if scheme == A.BARDOW else A.LeeLin
+    gs = cls(gd, streaming=streaming, mass=mass, cg_tolerance=cg_tol, richardson=richardson)
+    gs.set_time_step(dt, lam)
+    gs.set_initial_populations()
+    osch = O.Scheme(od, O.BARDOW if scheme == A.BARDOW else O.LEELIN, streaming, mass, richardson=richardson,
+                    cg_tol=cg_tol)
+    osch.set_time_step(dt, lam)
+    osch.set_initial_populations()
+    f0g, f0o = gs.f, osch.populations()
+    assert rel(f0g, f0o) < 1e-15, "initial populations"
+    errs = []
+    for k in range(steps):
+        gs.step(1)
+        osch.step(1)
+        errs.append(rel(gs.f, osch.populations()))
+    return gs, osch, errs, gd, od
+
+
+def test_disc_matches_oracle_geometry():
+    for case, p in (("tgv", 1), ("couette", 2), ("cavity", 3)):
+        tc, gd, od = pair(case, 6, p)
+        assert gd.n_nodes == od.n
+        assert np.allclose(gd.node, od.nodes(), atol=1e-15)
+        assert (gd.constrained() == od.constrained()).all()
+        assert np.allclose(gd.lumped_mass(), od.lumped_mass(), rtol=1e-14)
+        assert np.allclose(gd.mass_diagonal(), od.mass_diagonal(), rtol=1e-14)
+        assert abs(gd.h_min - od.h_min) < 1e-16
+        nodes, vel = od.walls()
+        assert (np.sort(gd.walls[0]) == np.sort(nodes)).all()
+
+
+@pytest.mark.parametrize("case", ["tgv", "couette", "cavity"])
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("scheme", [A.BARDOW, A.LEELIN])
+def test_fused_lumped_step(case, p, scheme):
+    """One kernel per step (collision, advection, M_L^-1, nodal update) vs the oracle."""
+    N = {1: 20, 2: 11, 3: 7}[p]
+    _, _, errs, _, _ = run_both(case, N, p, scheme, A.MASS_LUMPED, steps=5)
+    assert errs[0] < 1e-12, errs
+    assert max(errs) < 5e-12, errs
+
+
+@pytest.mark.parametrize("case,p", [("tgv", 1), ("cavity", 1), ("couette", 2), ("cavity", 2), ("tgv", 3)])
+@pytest.mark.parametrize("scheme", [A.BARDOW, A.LEELIN])
+def test_consistent_mass_step(case, p, scheme):
+    """Advection kernel + batched CG (tol 1e-12) + nodal update vs the oracle; CG iterations within +-1."""
+    N = {1: 16, 2: 9, 3: 6}[p]
+    steps = 4
+    gs, osch, errs, _, _ = run_both(case, N, p, scheme, A.MASS_CG, steps=steps)
+    assert errs[0] < 1e-12, errs
+    assert max(errs) < 1e-11, errs
+    assert abs(gs.cg_iterations - osch.cg_iterations) <= 8 * steps, (gs.cg_iterations, osch.cg_iterations)
+
+
+@pytest.mark.parametrize("streaming", [A.TG3, A.TG3_SPLIT])
+def test_tg3_streaming(streaming):
+    _, _, errs, _, _ = run_both("tgv", 12, 1, A.BARDOW, A.MASS_CG, steps=3, cfl=0.9, streaming=streaming)
+    assert max(errs) < 1e-11, errs
+
+
+def test_richardson_mass():
+    _, _, errs, _, _ = run_both("cavity", 12, 1, A.BARDOW, A.MASS_RICHARDSON, steps=3, richardson=2)
+    assert max(errs) < 5e-12, errs
+
+
+def test_many_steps_stable_case():
+    """<= 1e-10 after many steps (cavity, Lee-Lin lumped, CFL 0.5: a stable configuration)."""
+    tc, gd, od = pair("cavity", 24, 1, Re=400.0)
+    dt = od.time_step(0.5)
+    lam = od.relaxation_time
+    gs = A.LeeLin(gd, mass=A.MASS_LUMPED)
+    gs.set_time_step(dt, lam)
+    gs.set_initial_populations()
+    osch = O.Scheme(od, O.LEELIN, O.TG2, O.MASS_LUMPED)
+    osch.set_time_step(dt, lam)
+    osch.set_initial_populations()
+    gs.step(400)
+    osch.step(400)
+    assert rel(gs.f, osch.populations()) < 1e-10
+
+
+# ------------------------------------------------------------------ operators
+def _seeded_f(od, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    return O.W[:, None] * (1 + amp * rng.standard_normal((9, od.n)))
+
+
+@pytest.mark.parametrize("case,p", [("tgv", 1), ("couette", 2), ("cavity", 3), ("cavity", 1)])
+@pytest.mark.parametrize("kind", [A.BARDOW, A.LEELIN])
+def test_advection_operator(case, p, kind):
+    tc, gd, od = pair(case, 8, p)
+    f = _seeded_f(od, 4)
+    feq = od.compute_equilibrium(f)
+    dt, lam = 0.01, 0.003
+    rhs_o = od.advection(f, dt, lam, feq=feq if kind == A.LEELIN else None, leelin=kind == A.LEELIN,
+                         remove_mass_flux=kind == A.BARDOW)
+    fd, fed = gd.to_device(f, 9), gd.to_device(feq, 9)
+    rhs = gd.device_vector(8)
+    A.advection_apply(gd, kind, rhs, fd, fed, dt, lam, remove_mass_flux=kind == A.BARDOW)
+    torch.cuda.synchronize()
+    got = gd.to_host(rhs, 8)
+    scale = np.abs(rhs_o).max()
+    assert np.abs(got - rhs_o).max() < 1e-13 * scale
+
+
+@pytest.mark.parametrize("case,p", [("tgv", 2), ("cavity", 1), ("couette", 3)])
+def test_nodal_operators(case, p):
+    tc, gd, od = pair(case, 6, p, mach=0.3)
+    f = _seeded_f(od, 9)
+    x = 1e-3 * np.random.default_rng(1).standard_normal((8, od.n))
+    fd = gd.to_device(f, 9)
+    A.collide_bgk(gd, fd, 1.3)
+    torch.cuda.synchronize()
+    assert np.abs(gd.to_host(fd, 9) - od.collide_bgk(f, 1.3)).max() < 1e-15
+    fd = gd.to_device(f, 9)
+    feqd = gd.device_vector(9)
+    A.compute_equilibrium(gd, fd, feqd)
+    torch.cuda.synchronize()
+    feq = od.compute_equilibrium(f)
+    assert np.abs(gd.to_host(feqd, 9) - feq).max() < 1e-15
+    xd = gd.to_device(x, 8)
+    A.predictor_corrector(gd, fd, feqd, xd, 0.7)
+    torch.cuda.synchronize()
+    assert np.abs(gd.to_host(fd, 9) - od.predictor_corrector(f, feq, x, 0.7)).max() < 1e-15
+
+
+@pytest.mark.parametrize("case,p", [("tgv", 1), ("cavity", 2), ("couette", 3)])
+def test_mass_operator_and_solves(case, p):
+    tc, gd, od = pair(case, 7, p)
+    rng = np.random.default_rng(12)
+    r = rng.standard_normal(od.n)
+    r[od.constrained()] = 0
+    xd = gd.to_device(r, 1)
+    yd = gd.device_vector(1)
+    for c, e in ((0.0, (0.0, 0.0)), (0.02, (1.0, -1.0))):
+        A.streaming_vmult(gd, yd, xd, c, e)
+        torch.cuda.synchronize()
+        want = od.vmult(r, c, e)
+        assert np.abs(gd.to_host(yd, 1)[0] - want).max() < 1e-14 * np.abs(want).max()
+    for mass in (A.MASS_LUMPED, A.MASS_RICHARDSON, A.MASS_CG):
+        xo, its_o, _ = od.mass_solve(r, mass=mass, cg_tol=1e-12, richardson=3)
+        xg = gd.device_vector(1)
+        rd = gd.to_device(r, 1)
+        its = A.mass_solve(gd, xg, rd, mass=mass, cg_tolerance=1e-12, richardson=3)
+        got = gd.to_host(xg, 1)[0]
+        assert np.abs(got - xo).max() < 1e-11 * np.abs(xo).max(), mass
+        assert abs(its - its_o) <= 1
+
+
+def test_cg_non_convergence_is_reported():
+    tc, gd, od = pair("tgv", 8, 1)
+    r = np.random.default_rng(0).standard_normal(od.n)
+    r[od.constrained()] = 0
+    with pytest.raises(A.NoConvergence):
+        A.mass_solve(gd, gd.device_vector(1), gd.to_device(r, 1), cg_tolerance=1e-30, cg_max_iterations=5)
+
+
+def test_invalid_settings_rejected():
+    tc, gd, od = pair("tgv", 8, 1)
+    with pytest.raises(A.LbfemError):
+        A.LeeLin(gd, streaming=A.TG3)  # schemes.h:88
+    with pytest.raises(A.LbfemError):
+        A.Bardow(gd, streaming=A.TG3, mass=A.MASS_LUMPED)  # streaming.h:51-52
+
+
+def test_diagnostics_sums():
+    tc, gd, od = pair("tgv", 16, 1)
+    dt, lam = od.time_step(0.25), od.relaxation_time
+    gs = A.Bardow(gd, mass=A.MASS_LUMPED)
+    gs.set_time_step(dt, lam)
+    gs.set_initial_populations()
+    gs.step(10)
+    gs.synchronize()
+    xy = gd.node
+    _, ux, uy = tc.exact(xy[:, 0], xy[:, 1], 10 * dt)
+    ue = torch.from_numpy(np.stack([ux, uy], 1).reshape(-1).copy()).cuda()
+    up = torch.zeros(2 * gd.n_nodes, dtype=torch.float64, device="cuda")
+    s = A.diagnostics(gd, gs.device_populations(), ue, up)
+    so = od.diagnostics(gs.f, 10 * dt)
+    assert np.allclose(s[:5], so[:5], rtol=1e-12)
