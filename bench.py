@@ -157,7 +157,7 @@ def main():
                    "n_nodes": n, "mnups": round(value / 9, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_tile<1,32,16,kStepLeeLin>", "kernel_us": round(1e6 * t_kernel, 3),
+                     "kernel": "k_tile<1,32,8,kStepLeeLin>", "kernel_us": round(1e6 * t_kernel, 3),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "variants": variants,

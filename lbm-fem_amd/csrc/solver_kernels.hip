@@ -285,7 +285,7 @@ namespace lbfem_amd
 
     for (int v = 0; v < W.nv; ++v)
       {
-        const bool active = KIND == 0 || W.state[v].status == 0;
+        const bool active = KIND != 2 || W.state[v].status == 0; // the start sets every status
         if (!active) // uniform over the grid
           continue;
         const double   c  = W.vec[v].coef, ex = W.vec[v].e[0], ey = W.vec[v].e[1];

@@ -5,13 +5,16 @@
 // and the nodal work after it (f += M_L^-1 r, or the predictor-corrector).
 //
 // One workgroup owns a TX x TY block of nodes. It loads the block plus a halo
-// of p nodes (the reach of a Q_p row) for all nine populations, does the
-// nodal prologue in registers, then per population stages the field in LDS
-// and applies the operator as Kronecker products of assembled 1-D bands: a y
-// pass over the halo columns into LDS, then an x pass at the owned nodes.
-// Every output node is computed by exactly one thread from the same sequence
-// of operations, so the result is deterministic and translation invariant (no
-// scatter, no atomics); the periodic seam is a wrap of the halo index.
+// of p nodes (the reach of a Q_p row) for all nine populations once, does the
+// nodal prologue, and stages the results in LDS: g* (Bardow), or f and
+// h = a f + b feq (Lee-Lin). The operator of each population is then applied
+// at the owned nodes straight from LDS, as Kronecker products of the assembled
+// 1-D bands: each thread owns RO consecutive nodes of one column, sums the
+// y bands over a strip of RO + 2p rows of each of the 2p + 1 neighbouring
+// columns (shared between its rows), and contracts with its column's x bands.
+// The population loop has no barrier. Every output node is computed by exactly
+// one thread from the same operation sequence: deterministic, translation
+// invariant, no scatter, no atomics; the periodic seam is a wrap of the halo.
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -46,21 +49,22 @@ namespace lbfem_amd
     }
   } // namespace
 
+  // bands: 0 M, 1 D, 2 K, 3 D^T
   template <int p, int TX, int TY, int MODE>
   __global__ void __launch_bounds__(kThreads)
   k_tile(const Geo G, const TileParams S, const double *__restrict__ fin, const double *__restrict__ feq_in,
          double *__restrict__ out, double *__restrict__ rhs_out)
   {
-    constexpr int HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
-    constexpr int R  = (NH + kThreads - 1) / kThreads; // halo'd nodes per thread
-    constexpr int RO = (TX * TY) / kThreads;           // owned nodes per thread
-    static_assert((TX * TY) % kThreads == 0 && kThreads % TX == 0, "tile shape");
-    constexpr bool LL = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
+    constexpr int  W = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
+    constexpr int  TR = kThreads / TX; // thread rows
+    constexpr int  RO = TY / TR;       // owned nodes per thread, consecutive rows of one column
+    constexpr int  SR = RO + 2 * p;    // strip rows
+    static_assert(TY % TR == 0 && kThreads % TX == 0, "tile shape");
+    constexpr bool LL   = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
     constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
+    constexpr int  NF   = LL ? Q + NM : Q; // f (or g*) and, for Lee-Lin, h of the moving populations
 
-    __shared__ double fld[HY][HX];
-    __shared__ double hld[LL ? HY : 1][LL ? HX : 1];
-    __shared__ double yv[4][TY][HX];
+    __shared__ double F[NF][HY][HX];
 
     // XCD-aware tile order: consecutive tiles of a row band go to one XCD (one L2)
     const int nt   = S.tiles_x * S.tiles_y;
@@ -71,247 +75,280 @@ namespace lbfem_amd
       return;
     const int i0 = (tile % S.tiles_x) * TX, j0 = (tile / S.tiles_x) * TY;
     const int P = G.P, tid = threadIdx.x;
+    const double a_c = S.a, b_c = S.b, c_c = S.c, hdt2 = 0.5 * S.dt * S.dt;
 
     // ---------------------------------------------------------- prologue
-    // halo'd block: f (LL: also the moments the equilibria are built from) or g*
-    double v[R][Q];
-    double mom[LL ? R : 1][3];
+    // all loads of the halo'd block first (R rounds in flight), then the nodal work
+    constexpr int R = (NH + kThreads - 1) / kThreads;
+    double        v[R][Q];
+    int           gis[R], gjs[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
       {
         const int k  = tid + r * kThreads;
         const int kx = k % HX, ky = k / HX;
-        const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
-        const bool ok = k < NH && gi >= 0 && gj >= 0;
-        if (ok)
+        gis[r]       = k < NH ? wrap(i0 - p + kx, P, G.ax.periodic) : -1;
+        gjs[r]       = k < NH ? wrap(j0 - p + ky, P, G.ay.periodic) : -1;
+        if (gis[r] >= 0 && gjs[r] >= 0)
           {
-            const int64_t idx = node_index(G, gi, gj);
+            const int64_t idx = node_index(G, gis[r], gjs[r]);
 #pragma unroll
             for (int a = 0; a < Q; ++a)
               v[r][a] = fin[a * G.stride + idx];
-            double wx = 0., wy = 0.;
-            const bool wall = wall_of(G, gi, gj, wx, wy);
-            if constexpr (LL)
-              {
-                double rho, ux, uy;
-                if (feq_in == nullptr)
-                  {
-                    moments(v[r], rho, ux, uy);
-                    if (wall)
-                      {
-                        ux = wx;
-                        uy = wy;
-                      }
-                    mom[r][0] = rho;
-                    mom[r][1] = ux;
-                    mom[r][2] = uy;
-                  }
-              }
-            else if (S.collide)
-              collide_node(v[r], S.omega, wall, wx, wy);
-          }
-        else
-          {
-#pragma unroll
-            for (int a = 0; a < Q; ++a)
-              v[r][a] = 0.;
-            if constexpr (LL)
-              mom[r][0] = mom[r][1] = mom[r][2] = 0.;
-          }
-      }
-
-    // owned nodes: their own populations after the prologue (a slave reads its
-    // own entries, not its master's), and the accumulated increments
-    double own[RO][Q];
-    double xinc[RO][NM];
-    double mflux[RO];
-#pragma unroll
-    for (int r = 0; r < RO; ++r)
-      {
-        const int ii = tid % TX, jj = tid / TX + r * (kThreads / TX);
-        const int gi = i0 + ii, gj = j0 + jj;
-        mflux[r]     = 0.;
-#pragma unroll
-        for (int a = 0; a < NM; ++a)
-          xinc[r][a] = 0.;
-        if (gi <= P && gj <= P)
-          {
-            const int64_t idx = node_index(G, gi, gj);
-#pragma unroll
-            for (int a = 0; a < Q; ++a)
-              own[r][a] = fin[a * G.stride + idx];
-            if constexpr (!LL)
-              if (S.collide)
-                {
-                  double wx = 0., wy = 0.;
-                  const bool wall = wall_of(G, gi, gj, wx, wy);
-                  collide_node(own[r], S.omega, wall, wx, wy);
-                }
           }
         else
 #pragma unroll
           for (int a = 0; a < Q; ++a)
-            own[r][a] = 0.;
+            v[r][a] = 0.;
       }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      {
+        const int k = tid + r * kThreads;
+        if (k >= NH)
+          continue;
+        const int  kx = k % HX, ky = k / HX;
+        const bool ok = gis[r] >= 0 && gjs[r] >= 0;
+        double     wx = 0., wy = 0.;
+        const bool wall = ok && wall_of(G, gis[r], gjs[r], wx, wy);
+        if constexpr (LL)
+          {
+            double fe[Q];
+            if (!ok)
+#pragma unroll
+              for (int a = 0; a < Q; ++a)
+                fe[a] = 0.;
+            else if (feq_in == nullptr)
+              equilibrium_node(v[r], fe, wall, wx, wy);
+            else
+#pragma unroll
+              for (int a = 0; a < Q; ++a)
+                fe[a] = feq_in[a * G.stride + node_index(G, gis[r], gjs[r])];
+#pragma unroll
+            for (int a = 1; a < Q; ++a)
+              F[Q + a - 1][ky][kx] = a_c * v[r][a] + b_c * fe[a];
+          }
+        else if (S.collide && ok)
+          collide_node(v[r], S.omega, wall, wx, wy);
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          F[a][ky][kx] = v[r][a];
+      }
+    // this thread's column and rows; the x bands of the column and the y bands of
+    // the rows in registers (bands: 0 M, 1 D, 2 K, 3 D^T)
+    const int ii = tid % TX, jr = (tid / TX) * RO;
+    const int gi = i0 + ii;
+    double    xc[4][W], yc[RO][4][W];
+#pragma unroll
+    for (int o = 0; o < W; ++o)
+      {
+        const bool in = gi <= P;
+        xc[0][o]      = in ? band<p>(G.ax.M, gi, o - p) : 0.;
+        xc[1][o]      = in ? band<p>(G.ax.D, gi, o - p) : 0.;
+        xc[2][o]      = in ? band<p>(G.ax.K, gi, o - p) : 0.;
+        xc[3][o]      = in ? band<p>(G.ax.DT, gi, o - p) : 0.;
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+          {
+            const int  gj = j0 + jr + r;
+            const bool iy = gj <= P;
+            yc[r][0][o]   = iy ? band<p>(G.ay.M, gj, o - p) : 0.;
+            yc[r][1][o]   = iy ? band<p>(G.ay.D, gj, o - p) : 0.;
+            yc[r][2][o]   = iy ? band<p>(G.ay.K, gj, o - p) : 0.;
+            yc[r][3][o]   = iy ? band<p>(G.ay.DT, gj, o - p) : 0.;
+          }
+      }
+    __syncthreads();
 
-    const double a_c = S.a, b_c = S.b, c_c = S.c, hdt2 = 0.5 * S.dt * S.dt;
+    double xinc[RO][NM], mflux[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r)
+      {
+        mflux[r] = 0.;
+#pragma unroll
+        for (int a = 0; a < NM; ++a)
+          xinc[r][a] = 0.;
+      }
 
     // ---------------------------------------------------------- populations 1..8
 #pragma unroll
     for (int al = 1; al < Q; ++al)
       {
+        // directions as uniform run-time values: compile-time D2Q9 constants let the
+        // scheduler interleave the barrier-free populations and spill registers
         const double ex = S.e[al][0], ey = S.e[al][1];
         const bool   nx = ex != 0., ny = ey != 0.;
         if (!nx && !ny)
           continue; // at rest in this sweep: r = 0 (streaming.h:118-122)
-
-        // stage f_al (and h_al = a f + b feq for Lee-Lin) of the halo'd block
+        const double(*Ff)[HX] = F[al];
+        const double(*Fh)[HX] = F[LL ? Q + al - 1 : al];
+        double acc[RO];
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int r = 0; r < RO; ++r)
+          acc[r] = 0.;
+        // r = Dx (ex My h + c ex ey Dy^T f) + Kx (c ex^2 My f) + Mx (ey Dy h + c ey^2 Ky f) + Dx^T (c ex ey Dy f),
+        // h = f for Bardow (its a folded into the x weights); one branch-free body per direction type
+        const double sa = LL ? 1. : a_c;
+        if (nx && !ny)
           {
-            const int k = tid + r * kThreads;
-            if (k < NH)
+            const double wd = ex * sa, wk = c_c * ex * ex;
+#pragma unroll
+            for (int c = 0; c < W; ++c)
               {
-                const int kx = k % HX, ky = k / HX;
-                fld[ky][kx] = v[r][al];
-                if constexpr (LL)
+                const double kd = xc[1][c] * wd, kk = xc[2][c] * wk;
+#pragma unroll
+                for (int r = 0; r < RO; ++r)
                   {
-                    double fe;
-                    if (feq_in == nullptr)
+                    double My = 0., Myh = 0.;
+#pragma unroll
+                    for (int o = 0; o < W; ++o)
                       {
-                        const double rho = mom[r][0], ux = mom[r][1], uy = mom[r][2];
-                        fe               = feq_rt(al, rho, ux, uy, ux * ux + uy * uy);
+                        My += yc[r][0][o] * Ff[jr + r + o][ii + c];
+                        if constexpr (LL)
+                          Myh += yc[r][0][o] * Fh[jr + r + o][ii + c];
                       }
-                    else
-                      {
-                        const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
-                        fe = (gi >= 0 && gj >= 0) ? feq_in[al * G.stride + node_index(G, gi, gj)] : 0.;
-                      }
-                    hld[ky][kx] = a_c * v[r][al] + b_c * fe;
+                    if constexpr (!LL)
+                      Myh = My;
+                    acc[r] += kd * Myh + kk * My;
                   }
               }
           }
-        __syncthreads();
-
-        // y pass: owned rows, all halo columns
-        for (int idx = tid; idx < TY * HX; idx += kThreads)
+        else if (!nx && ny)
           {
-            const int row = idx / HX, col = idx % HX;
-            const int gj  = j0 + row;
-            if (gj > P)
-              continue;
-            double My = 0., Dy = 0., DTy = 0., Ky = 0., Myh = 0., Dyh = 0.;
+            const double wd = ey * sa, wk = c_c * ey * ey;
 #pragma unroll
-            for (int o = -p; o <= p; ++o)
+            for (int c = 0; c < W; ++c)
               {
-                const double fv = fld[row + p + o][col];
-                if (nx)
-                  My += band<p>(G.ay.M, gj, o) * fv;
-                if (ny)
+#pragma unroll
+                for (int r = 0; r < RO; ++r)
                   {
-                    Dy += band<p>(G.ay.D, gj, o) * fv;
-                    Ky += band<p>(G.ay.K, gj, o) * fv;
-                  }
-                if (nx && ny)
-                  DTy += band<p>(G.ay.DT, gj, o) * fv;
-                if constexpr (LL)
-                  {
-                    const double hv = hld[row + p + o][col];
-                    if (nx)
-                      Myh += band<p>(G.ay.M, gj, o) * hv;
-                    if (ny)
-                      Dyh += band<p>(G.ay.D, gj, o) * hv;
+                    double s = 0.;
+#pragma unroll
+                    for (int o = 0; o < W; ++o)
+                      {
+                        const double fv = Ff[jr + r + o][ii + c];
+                        const double hv = LL ? Fh[jr + r + o][ii + c] : fv;
+                        s += (yc[r][1][o] * wd) * hv + (yc[r][2][o] * wk) * fv;
+                      }
+                    acc[r] += xc[0][c] * s;
                   }
               }
-            if constexpr (!LL)
-              {
-                Myh = a_c * My;
-                Dyh = a_c * Dy;
-              }
-            // r = Dx U_D + Kx U_K + Mx U_M + Dx^T U_DT
-            yv[0][row][col] = ex * Myh + c_c * ex * ey * DTy; // U_D
-            yv[1][row][col] = c_c * ex * ex * My;               // U_K
-            yv[2][row][col] = ey * Dyh + c_c * ey * ey * Ky;   // U_M
-            yv[3][row][col] = c_c * ex * ey * Dy;               // U_DT
           }
-        __syncthreads();
-
-        // x pass at the owned nodes, plus the wall-face terms
+        else
+          {
+            const double wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
+#pragma unroll
+            for (int c = 0; c < W; ++c)
+              {
+#pragma unroll
+                for (int r = 0; r < RO; ++r)
+                  {
+                    double My = 0., Dy = 0., Ky = 0., DTy = 0., Myh = 0., Dyh = 0.;
+#pragma unroll
+                    for (int o = 0; o < W; ++o)
+                      {
+                        const double fv = Ff[jr + r + o][ii + c];
+                        My += yc[r][0][o] * fv;
+                        Dy += yc[r][1][o] * fv;
+                        Ky += yc[r][2][o] * fv;
+                        DTy += yc[r][3][o] * fv;
+                        if constexpr (LL)
+                          {
+                            const double hv = Fh[jr + r + o][ii + c];
+                            Myh += yc[r][0][o] * hv;
+                            Dyh += yc[r][1][o] * hv;
+                          }
+                      }
+                    if constexpr (!LL)
+                      {
+                        Myh = My;
+                        Dyh = Dy;
+                      }
+                    acc[r] += xc[1][c] * (wdx * Myh + wx * DTy) + xc[2][c] * (wk * My) +
+                              xc[0][c] * (wdy * Dyh + wky * Ky) + xc[3][c] * (wx * Dy);
+                  }
+              }
+          }
+        // surface term (dt^2/2) oint phi (n.e)(e.grad f) on walls (advection.h:177-199), as exact
+        // 1-D integrals along the wall: Dx of the tangential derivative, Mx of the normal one
 #pragma unroll
         for (int r = 0; r < RO; ++r)
           {
-            const int ii = tid % TX, jj = tid / TX + r * (kThreads / TX);
-            const int gi = i0 + ii, gj = j0 + jj;
-            if (gi > P || gj > P)
-              continue;
-            double acc = 0.;
-#pragma unroll
-            for (int o = -p; o <= p; ++o)
-              {
-                const int c = ii + p + o;
-                if (nx)
-                  acc += band<p>(G.ax.D, gi, o) * yv[0][jj][c] + band<p>(G.ax.K, gi, o) * yv[1][jj][c];
-                if (ny)
-                  acc += band<p>(G.ax.M, gi, o) * yv[2][jj][c];
-                if (nx && ny)
-                  acc += band<p>(G.ax.DT, gi, o) * yv[3][jj][c];
-              }
-            // surface term (dt^2/2) oint phi (n.e)(e.grad f) on walls (advection.h:177-199),
-            // as 1-D integrals along the wall: Dx for the tangential, Mx of the normal derivative
-            double face = 0.;
+            const int jj = jr + r, gj = j0 + jj;
             const int ty = jj + p, tx = ii + p;
-            if (!G.ay.periodic && ny && (gj == 0 || gj == P))
+            double    face = 0.;
+            if (gi <= P && gj <= P)
               {
-                const bool    lo = gj == 0;
-                const double *bn = lo ? G.ay.b0 : G.ay.b1;
-                const int     r0 = lo ? ty : ty - p; // first row of the boundary cell
-                double        t  = 0., nrm = 0.;
-#pragma unroll
-                for (int o = -p; o <= p; ++o)
+                if (!G.ay.periodic && ny && (gj == 0 || gj == P))
                   {
-                    double dn = 0.;
+                    const bool    lo = gj == 0;
+                    const double *bn = lo ? G.ay.b0 : G.ay.b1;
+                    const int     r0 = lo ? ty : ty - p;
+                    double        t = 0., nrm = 0.;
 #pragma unroll
-                    for (int l = 0; l <= p; ++l)
-                      dn += bn[l] * fld[r0 + l][tx + o];
-                    t += band<p>(G.ax.D, gi, o) * fld[ty][tx + o];
-                    nrm += band<p>(G.ax.M, gi, o) * dn;
+                    for (int o = 0; o < W; ++o)
+                      {
+                        double dn = 0.;
+#pragma unroll
+                        for (int l = 0; l <= p; ++l)
+                          dn += bn[l] * Ff[r0 + l][ii + o];
+                        t += xc[1][o] * Ff[ty][ii + o];
+                        nrm += xc[0][o] * dn;
+                      }
+                    face += hdt2 * (lo ? -ey : ey) * (ex * t + ey * nrm);
                   }
-                face += hdt2 * (lo ? -ey : ey) * (ex * t + ey * nrm);
-              }
-            if (!G.ax.periodic && nx && (gi == 0 || gi == P))
-              {
-                const bool    lo = gi == 0;
-                const double *bn = lo ? G.ax.b0 : G.ax.b1;
-                const int     c0 = lo ? tx : tx - p;
-                double        t  = 0., nrm = 0.;
-#pragma unroll
-                for (int o = -p; o <= p; ++o)
+                if (!G.ax.periodic && nx && (gi == 0 || gi == P))
                   {
-                    double dn = 0.;
+                    const bool    lo = gi == 0;
+                    const double *bn = lo ? G.ax.b0 : G.ax.b1;
+                    const int     c0 = lo ? tx : tx - p;
+                    double        t = 0., nrm = 0.;
 #pragma unroll
-                    for (int l = 0; l <= p; ++l)
-                      dn += bn[l] * fld[ty + o][c0 + l];
-                    t += band<p>(G.ay.D, gj, o) * fld[ty + o][tx];
-                    nrm += band<p>(G.ay.M, gj, o) * dn;
+                    for (int o = 0; o < W; ++o)
+                      {
+                        double dn = 0.;
+#pragma unroll
+                        for (int l = 0; l <= p; ++l)
+                          dn += bn[l] * Ff[jj + o][c0 + l];
+                        t += yc[r][1][o] * Ff[jj + o][tx];
+                        nrm += yc[r][0][o] * dn;
+                      }
+                    face += hdt2 * (lo ? -ex : ex) * (ex * nrm + ey * t);
                   }
-                face += hdt2 * (lo ? -ex : ex) * (ex * nrm + ey * t);
               }
             mflux[r] += face;
-            xinc[r][al - 1] = acc + face;
+            xinc[r][al - 1] = acc[r] + face;
           }
-        __syncthreads();
       }
 
     // ---------------------------------------------------------- epilogue
 #pragma unroll
     for (int r = 0; r < RO; ++r)
       {
-        const int ii = tid % TX, jj = tid / TX + r * (kThreads / TX);
-        const int gi = i0 + ii, gj = j0 + jj;
+        const int jj = jr + r, gj = j0 + jj;
         if (gi > P || gj > P)
           continue;
         const int64_t idx   = node_index(G, gi, gj);
         const bool    slave = (G.ax.periodic && gi == P) || (G.ay.periodic && gj == P);
+        double        own[Q];
+        if (!slave)
+#pragma unroll
+          for (int a = 0; a < Q; ++a)
+            own[a] = F[a][jj + p][ii + p];
+        else
+          {
+            // a slave's own entries (the staged tile holds its master's); only nodal work applies
+#pragma unroll
+            for (int a = 0; a < Q; ++a)
+              own[a] = fin[a * G.stride + idx];
+            if constexpr (!LL)
+              if (S.collide)
+                {
+                  double     wx = 0., wy = 0.;
+                  const bool wall = wall_of(G, gi, gj, wx, wy);
+                  collide_node(own, S.omega, wall, wx, wy);
+                }
+          }
         // mass flux of the wall term removed isotropically (advection.h:194-196)
         if (S.remove_mass_flux)
 #pragma unroll
@@ -325,19 +362,19 @@ namespace lbfem_amd
               xinc[r][a] *= mli;
             if constexpr (LL)
               {
-                double wx = 0., wy = 0.;
+                double     wx = 0., wy = 0.;
                 const bool wall = wall_of(G, gi, gj, wx, wy);
                 double     fe[Q];
-                equilibrium_node(own[r], fe, wall, wx, wy);
-                predictor_corrector_node(own[r], fe, xinc[r], S.theta, wall, wx, wy);
+                equilibrium_node(own, fe, wall, wx, wy);
+                predictor_corrector_node(own, fe, xinc[r], S.theta, wall, wx, wy);
               }
             else
 #pragma unroll
               for (int a = 1; a < Q; ++a)
-                own[r][a] += xinc[r][a - 1];
+                own[a] += xinc[r][a - 1];
 #pragma unroll
             for (int a = 0; a < Q; ++a)
-              out[a * G.stride + idx] = own[r][a];
+              out[a * G.stride + idx] = own[a];
           }
         else
           {
@@ -348,7 +385,7 @@ namespace lbfem_amd
             if (out != nullptr)
 #pragma unroll
               for (int a = 0; a < Q; ++a)
-                out[a * G.stride + idx] = own[r][a];
+                out[a * G.stride + idx] = own[a];
           }
       }
   }
@@ -359,27 +396,29 @@ namespace lbfem_amd
   launch_tile(const Geo &G, TileParams S, const double *fin, const double *feq, double *out, double *rhs,
               hipStream_t stream)
   {
-    S.tiles_x       = (G.P + 1 + TX - 1) / TX;
-    S.tiles_y       = (G.P + 1 + TY - 1) / TY;
-    const int nt    = S.tiles_x * S.tiles_y;
-    const int nb8   = (nt + 7) & ~7;
+    S.tiles_x     = (G.P + 1 + TX - 1) / TX;
+    S.tiles_y     = (G.P + 1 + TY - 1) / TY;
+    const int nt  = S.tiles_x * S.tiles_y;
+    const int nb8 = (nt + 7) & ~7;
     hipLaunchKernelGGL((k_tile<p, TX, TY, MODE>), dim3(nb8), dim3(kThreads), 0, stream, G, S, fin, feq, out, rhs);
     return hipGetLastError();
   }
 
+  // tile heights: 16 rows where the staged fields fit in about 55 KB of LDS, else 8
   template <int MODE>
   static hipError_t
   dispatch_degree(const Geo &G, const TileParams &S, const double *fin, const double *feq, double *out, double *rhs,
                   hipStream_t stream)
   {
+    constexpr bool LL = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
     switch (G.p)
       {
         case 1:
-          return launch_tile<1, 32, 16, MODE>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<1, 32, LL ? 8 : 16, MODE>(G, S, fin, feq, out, rhs, stream);
         case 2:
-          return launch_tile<2, 32, 16, MODE>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<2, 32, LL ? 8 : 16, MODE>(G, S, fin, feq, out, rhs, stream);
         case 3:
-          return launch_tile<3, 32, 16, MODE>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<3, 32, 8, MODE>(G, S, fin, feq, out, rhs, stream);
         default:
           return hipErrorInvalidValue;
       }
