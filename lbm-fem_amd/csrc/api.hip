@@ -103,13 +103,15 @@ struct lbfem_disc
   {
     const size_t        w = size_t(a.P + 1) * (2 * a.p + 1), m = size_t(a.P + 1);
     std::vector<double> host;
-    host.reserve(4 * w + 3 * m);
+    host.reserve(4 * w + 4 * m);
     for (const auto *v : {&a.M, &a.D, &a.DT, &a.K})
       host.insert(host.end(), v->begin(), v->end());
     for (size_t i = 0; i < m; ++i)
       host.push_back(1. / a.row_sum[i]);
     host.insert(host.end(), a.row_sum.begin(), a.row_sum.end());
     host.insert(host.end(), a.diagonal.begin(), a.diagonal.end());
+    for (size_t i = 0; i < m; ++i)
+      host.push_back(1. / a.diagonal[i]);
     buf.upload(host);
     dev.M        = buf.p;
     dev.D        = buf.p + w;
@@ -118,6 +120,7 @@ struct lbfem_disc
     dev.ml_inv   = buf.p + 4 * w;
     dev.ml       = buf.p + 4 * w + m;
     dev.diag     = buf.p + 4 * w + 2 * m;
+    dev.diag_inv = buf.p + 4 * w + 3 * m;
     dev.periodic = a.periodic;
     for (int l = 0; l < 4; ++l)
       {

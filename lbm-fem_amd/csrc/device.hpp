@@ -40,6 +40,7 @@ namespace lbfem_amd
     const double *ml_inv;  // 1 / row sums of M, P + 1
     const double *ml;      // row sums of M (lumped mass), P + 1
     const double *diag;    // diagonal of M, P + 1
+    const double *diag_inv; // 1 / diagonal of M, P + 1
     double        b0[4], b1[4];
     int           periodic;
   };

@@ -12,6 +12,8 @@
 // Reductions: wavefront shuffles and LDS inside a block, one partial per block,
 // and the last block to finish sums the partials in block order (a fixed
 // order, so results do not depend on scheduling) and updates the scalars.
+#include <algorithm>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -20,6 +22,7 @@ namespace lbfem_amd
   namespace
   {
     constexpr int kThreads = 256;
+    constexpr int kPart    = 3 * NM; // partial sums per block, contiguous
 
     template <int p>
     __device__ __forceinline__ double
@@ -42,11 +45,12 @@ namespace lbfem_amd
       return (G.ax.periodic && i == G.P) || (G.ay.periodic && j == G.P);
     }
 
-    // the Jacobi diagonal diag(M) = diag(Mx) diag(My), 1 on constrained rows
+    // the Jacobi preconditioner: the inverse of diag(M) = diag(Mx) diag(My), 1 on
+    // constrained rows, applied as a product (DiagonalMatrix::vmult of the inverse entries)
     __device__ __forceinline__ double
-    mass_diag(const Geo &G, const int i, const int j)
+    mass_diag_inv(const Geo &G, const int i, const int j)
     {
-      return is_slave(G, i, j) ? 1. : G.ax.diag[i] * G.ay.diag[j];
+      return is_slave(G, i, j) ? 1. : G.ax.diag_inv[i] * G.ay.diag_inv[j];
     }
 
     // sum over the block of up to K values per thread; result valid in thread 0
@@ -79,34 +83,6 @@ namespace lbfem_amd
       __syncthreads();
     }
 
-    // Last-block-done: returns true in the one block that sees all partials.
-    __device__ __forceinline__ bool
-    last_block(unsigned *counter, const unsigned nblocks)
-    {
-      __shared__ bool last;
-      __threadfence();
-      __syncthreads();
-      if (threadIdx.x == 0)
-        {
-          const unsigned t = atomicAdd(counter, 1u);
-          last             = (t == nblocks - 1);
-        }
-      __syncthreads();
-      if (last)
-        __threadfence();
-      return last;
-    }
-
-    // fixed-order sum of partial[0..n) by the whole block
-    __device__ __forceinline__ double
-    sum_partials(const double *partial, const int n, double (*lds)[kThreads / 64])
-    {
-      double s[1] = {0.};
-      for (int b = threadIdx.x; b < n; b += kThreads)
-        s[0] += partial[b];
-      block_sum<1>(s, lds);
-      return s[0];
-    }
   } // namespace
 
   // ------------------------------------------------------------------ nodal kernels
@@ -258,16 +234,19 @@ namespace lbfem_amd
   // ------------------------------------------------------------------ operator tile
   // KIND 0: y = A x.  KIND 1 (CG start): r = A x - b, partials |r|^2, |b|^2, r.D^-1 r.
   // KIND 2 (CG direction): p_new = beta p_old - D^-1 r, q = A p_new, partial p_new.q.
+  // A = M + c K_e = Mx (My + c ey^2 Ky) + Kx (c ex^2 My) + Dx^T (c ex ey Dy) + Dx (c ex ey Dy^T).
+  // The operands of all active vectors are staged in LDS at once (one barrier), then
+  // each thread applies the bands at its RO owned nodes of one column.
   template <int p, int TX, int TY, int KIND>
   __global__ void __launch_bounds__(kThreads)
   k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const double *x,
              const double *b, const double *r_in, double *y, double *y2)
   {
-    constexpr int HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
-    constexpr int RO = (TX * TY) / kThreads;
-    __shared__ double fld[HY][HX];
-    __shared__ double yv[4][TY][HX];
-    __shared__ double red[3 * NM][kThreads / 64];
+    constexpr int W_ = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
+    constexpr int TR = kThreads / TX, RO = TY / TR;
+    static_assert(TY % TR == 0, "tile shape");
+    __shared__ double X[1][HY][HX];
+    __shared__ double red[3][kThreads / 64];
 
     const int nt   = tiles_x * tiles_y;
     const int nb8  = (nt + 7) & ~7;
@@ -278,6 +257,79 @@ namespace lbfem_amd
     const int i0 = (tile % tiles_x) * TX, j0 = (tile / tiles_x) * TY;
     const int P = G.P, tid = threadIdx.x;
 
+    // active vectors (uniform): the start and plain products take every vector
+    // this block's vector (blockIdx.y); a converged one is skipped (uniform)
+    const int vsel = blockIdx.y;
+    if (KIND == 2 && W.state[vsel].status != 0)
+      {
+        if (tid == 0)
+          W.partial[size_t(tile) * kPart + 3 * vsel] = 0.;
+        return;
+      }
+    const unsigned act = 1u << vsel;
+    double         beta[NM];
+    for (int v = 0; v < NM; ++v)
+      beta[v] = (v == vsel && KIND == 2 && it > 1) ? W.state[v].beta : 0.;
+
+    // stage the operands (KIND 2: the new direction) of the halo'd block
+    for (int k = tid; k < NH; k += kThreads)
+      {
+        const int kx = k % HX, ky = k / HX;
+        const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
+        const bool    ok  = gi >= 0 && gj >= 0;
+        const int64_t idx = ok ? node_index(G, gi, gj) : 0;
+        const double  d   = ok ? mass_diag_inv(G, gi, gj) : 1.;
+        double xv[NM], rv[NM];
+#pragma unroll
+        for (int v = 0; v < NM; ++v)
+          {
+            xv[v] = rv[v] = 0.;
+            if (ok && (act & (1u << v)))
+              {
+                const int64_t vb = int64_t(W.block[v]) * G.stride;
+                if (KIND != 2 || it > 1)
+                  xv[v] = x[vb + idx];
+                if constexpr (KIND == 2)
+                  rv[v] = r_in[vb + idx];
+              }
+          }
+#pragma unroll
+        for (int v = 0; v < NM; ++v)
+          if (act & (1u << v))
+            {
+              double val = xv[v];
+              if constexpr (KIND == 2)
+                {
+                  const double z = d * rv[v];
+                  val            = it > 1 ? beta[v] * xv[v] - z : -z;
+                }
+              X[0][ky][kx] = val;
+            }
+      }
+    const int ii = tid % TX, jr = (tid / TX) * RO;
+    const int gi = i0 + ii;
+    double    xc[4][W_], yc[RO][4][W_];
+#pragma unroll
+    for (int o = 0; o < W_; ++o)
+      {
+        const bool in = gi <= P;
+        xc[0][o]      = in ? band<p>(G.ax.M, gi, o - p) : 0.;
+        xc[1][o]      = in ? band<p>(G.ax.D, gi, o - p) : 0.;
+        xc[2][o]      = in ? band<p>(G.ax.K, gi, o - p) : 0.;
+        xc[3][o]      = in ? band<p>(G.ax.DT, gi, o - p) : 0.;
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+          {
+            const int  gj = j0 + jr + r;
+            const bool iy = gj <= P;
+            yc[r][0][o]   = iy ? band<p>(G.ay.M, gj, o - p) : 0.;
+            yc[r][1][o]   = iy ? band<p>(G.ay.D, gj, o - p) : 0.;
+            yc[r][2][o]   = iy ? band<p>(G.ay.K, gj, o - p) : 0.;
+            yc[r][3][o]   = iy ? band<p>(G.ay.DT, gj, o - p) : 0.;
+          }
+      }
+    __syncthreads();
+
     double part[3 * NM];
 #pragma unroll
     for (int k = 0; k < 3 * NM; ++k)
@@ -285,221 +337,157 @@ namespace lbfem_amd
 
     for (int v = 0; v < W.nv; ++v)
       {
-        const bool active = KIND != 2 || W.state[v].status == 0; // the start sets every status
-        if (!active) // uniform over the grid
+        if (!(act & (1u << v)))
           continue;
-        const double   c  = W.vec[v].coef, ex = W.vec[v].e[0], ey = W.vec[v].e[1];
-        const int64_t  vb = int64_t(W.block[v]) * G.stride;
-        const double  *xv = x + vb;
-        const double   beta = (KIND == 2 && it > 1) ? W.state[v].beta : 0.;
-        // stage the operand of the tile (KIND 2: the new direction)
-        for (int k = tid; k < NH; k += kThreads)
-          {
-            const int kx = k % HX, ky = k / HX;
-            const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
-            double    val = 0.;
-            if (gi >= 0 && gj >= 0)
-              {
-                const int64_t idx = node_index(G, gi, gj);
-                if constexpr (KIND == 2)
-                  {
-                    const double z = r_in[vb + idx] / mass_diag(G, gi, gj);
-                    val            = it > 1 ? beta * xv[idx] - z : -z;
-                  }
-                else
-                  val = xv[idx];
-              }
-            fld[ky][kx] = val;
-          }
-        __syncthreads();
-        const bool cx = c != 0. && ex != 0., cy = c != 0. && ey != 0.;
-        for (int idx = tid; idx < TY * HX; idx += kThreads)
-          {
-            const int row = idx / HX, col = idx % HX;
-            const int gj  = j0 + row;
-            if (gj > P)
-              continue;
-            double My = 0., Ky = 0., Dy = 0., DTy = 0.;
+        const double  c = W.vec[v].coef, ex = W.vec[v].e[0], ey = W.vec[v].e[1];
+        const int64_t vb = int64_t(W.block[v]) * G.stride;
+        const double(*Xv)[HX] = X[0];
+        double acc[RO];
 #pragma unroll
-            for (int o = -p; o <= p; ++o)
-              {
-                const double fv = fld[row + p + o][col];
-                My += band<p>(G.ay.M, gj, o) * fv;
-                if (cy)
-                  Ky += band<p>(G.ay.K, gj, o) * fv;
-                if (cx && cy)
-                  {
-                    Dy += band<p>(G.ay.D, gj, o) * fv;
-                    DTy += band<p>(G.ay.DT, gj, o) * fv;
-                  }
-              }
-            // A x = Mx (My x + c ey^2 Ky x) + Kx (c ex^2 My x) + Dx^T (c ex ey Dy x) + Dx (c ex ey Dy^T x)
-            yv[0][row][col] = My + c * ey * ey * Ky;
-            yv[1][row][col] = c * ex * ex * My;
-            yv[2][row][col] = c * ex * ey * Dy;
-            yv[3][row][col] = c * ex * ey * DTy;
+        for (int r = 0; r < RO; ++r)
+          acc[r] = 0.;
+        if (c == 0. || (ex == 0. && ey == 0.)) // the mass matrix: Mx (x) My
+          {
+#pragma unroll
+            for (int cc = 0; cc < W_; ++cc)
+#pragma unroll
+              for (int r = 0; r < RO; ++r)
+                {
+                  double s = 0.;
+#pragma unroll
+                  for (int o = 0; o < W_; ++o)
+                    s += yc[r][0][o] * Xv[jr + r + o][ii + cc];
+                  acc[r] += xc[0][cc] * s;
+                }
           }
-        __syncthreads();
+        else
+          {
+            const double kyy = c * ey * ey, kxx = c * ex * ex, kxy = c * ex * ey;
+#pragma unroll
+            for (int cc = 0; cc < W_; ++cc)
+#pragma unroll
+              for (int r = 0; r < RO; ++r)
+                {
+                  double My = 0., Ky = 0., Dy = 0., DTy = 0.;
+#pragma unroll
+                  for (int o = 0; o < W_; ++o)
+                    {
+                      const double xv = Xv[jr + r + o][ii + cc];
+                      My += yc[r][0][o] * xv;
+                      Ky += yc[r][2][o] * xv;
+                      Dy += yc[r][1][o] * xv;
+                      DTy += yc[r][3][o] * xv;
+                    }
+                  acc[r] += xc[0][cc] * (My + kyy * Ky) + xc[2][cc] * (kxx * My) + xc[3][cc] * (kxy * Dy) +
+                            xc[1][cc] * (kxy * DTy);
+                }
+          }
 #pragma unroll
         for (int r = 0; r < RO; ++r)
           {
-            const int ii = tid % TX, jj = tid / TX + r * (kThreads / TX);
-            const int gi = i0 + ii, gj = j0 + jj;
+            const int gj = j0 + jr + r;
             if (gi > P || gj > P)
               continue;
-            double acc = 0.;
-#pragma unroll
-            for (int o = -p; o <= p; ++o)
-              {
-                const int cc = ii + p + o;
-                acc += band<p>(G.ax.M, gi, o) * yv[0][jj][cc];
-                if (cx)
-                  acc += band<p>(G.ax.K, gi, o) * yv[1][jj][cc];
-                if (cx && cy)
-                  acc += band<p>(G.ax.DT, gi, o) * yv[2][jj][cc] + band<p>(G.ax.D, gi, o) * yv[3][jj][cc];
-              }
             const int64_t idx   = node_index(G, gi, gj);
             const bool    slave = is_slave(G, gi, gj);
-            double        own   = fld[jj + p][ii + p];
+            double        own   = Xv[jr + r + p][ii + p];
+            double        a     = acc[r];
             if (slave) // constrained rows are the identity (MatrixFreeOperators::Base::vmult),
               {        // on the slave's own entry (the staged tile holds its master's)
                 if constexpr (KIND == 2)
-                  own = it > 1 ? beta * xv[idx] - r_in[vb + idx] : -r_in[vb + idx];
+                  own = it > 1 ? beta[v] * x[vb + idx] - r_in[vb + idx] : -r_in[vb + idx]; // D^-1 = 1
                 else
-                  own = xv[idx];
-                acc = own;
+                  own = x[vb + idx];
+                a = own;
               }
             if constexpr (KIND == 0)
-              y[vb + idx] = acc;
+              y[vb + idx] = a;
             else if constexpr (KIND == 1)
               {
                 const double bb = b[vb + idx];
-                const double rr = acc - bb;
-                y[vb + idx] = rr;
+                const double rr = a - bb;
+                y[vb + idx]     = rr;
                 part[3 * v] += rr * rr;
                 part[3 * v + 1] += bb * bb;
-                part[3 * v + 2] += rr * (rr / mass_diag(G, gi, gj));
+                part[3 * v + 2] += rr * (mass_diag_inv(G, gi, gj) * rr);
               }
             else
               {
                 y[vb + idx]  = own; // p_new
-                y2[vb + idx] = acc; // q = A p_new
-                part[3 * v] += own * acc;
+                y2[vb + idx] = a;   // q = A p_new
+                part[3 * v] += own * a;
               }
           }
-        __syncthreads();
       }
-    if constexpr (KIND == 0)
-      return;
-    else
+    if constexpr (KIND != 0)
       {
-        block_sum<3 * NM>(part, red);
+        double pv[3] = {part[3 * vsel], part[3 * vsel + 1], part[3 * vsel + 2]};
+        block_sum<3>(pv, red);
         if (tid == 0)
-          for (int k = 0; k < 3 * NM; ++k)
-            W.partial[size_t(k) * W.max_blocks + tile] = part[k];
-        if (!last_block(W.counter, nt))
-          return;
-        __shared__ double tot[3 * NM];
-        for (int v = 0; v < W.nv; ++v)
-          {
-            if (KIND == 2 && W.state[v].status != 0)
-              continue;
-            const int nk = KIND == 1 ? 3 : 1;
-            for (int k = 0; k < nk; ++k)
-              {
-                const double s = sum_partials(W.partial + size_t(3 * v + k) * W.max_blocks, nt, red);
-                if (tid == 0)
-                  tot[3 * v + k] = s;
-              }
-          }
-        if (tid == 0)
-          {
-            for (int v = 0; v < W.nv; ++v)
-              {
-                CgState &st = W.state[v];
-                if constexpr (KIND == 1)
-                  {
-                    st.res      = sqrt(tot[3 * v]);
-                    st.tol      = W.tol_rel * sqrt(tot[3 * v + 1]);
-                    st.rho      = tot[3 * v + 2];
-                    st.rho_prev = 0.;
-                    st.its      = 0;
-                    // SolverControl::check(0, res): success first, then failure at the cap
-                    st.status = st.res <= st.tol ? 1 : (W.max_its <= 0 || isnan(st.res)) ? 2 : 0;
-                  }
-                else
-                  {
-                    if (st.status != 0)
-                      continue;
-                    st.alpha = st.rho / tot[3 * v];
-                  }
-              }
-            *W.counter = 0u;
-          }
+          for (int k = 0; k < (KIND == 1 ? 3 : 1); ++k)
+            W.partial[size_t(tile) * kPart + 3 * vsel + k] = pv[k];
       }
   }
 
-  // x += alpha p; r += alpha q; res = |r|, the stopping test; rho = r.D^-1 r; beta
+  // The scalars of one CG stage from the per-block partials, in one block and a
+  // fixed summation order (a kernel boundary, not a device-wide fence, orders it
+  // after the partials). STAGE 1: start (|r|, |b|, r.D^-1 r); 2: direction
+  // (alpha = rho / p.q); 3: update (|r|, stopping test, rho, beta).
+  template <int STAGE>
   __global__ void __launch_bounds__(kThreads)
-  k_cg_update(const Geo G, const CgWork W, const int it, double *x, double *r, const double *p, const double *q)
+  k_cg_finalize(const CgWork W, const int nparts, const int it)
   {
-    __shared__ double red[2 * NM][kThreads / 64];
-    const int64_t     n = int64_t(G.np) * G.np;
-    double            part[2 * NM];
+    constexpr int stride = STAGE == 3 ? 2 : 3, nk = STAGE == 1 ? 3 : STAGE == 2 ? 1 : 2;
+    constexpr int K      = stride * NM;
+    __shared__ double red[K][kThreads / 64];
+    __shared__ double tot[K];
+    __shared__ int    status0[NM];
+    if (threadIdx.x < NM)
+      status0[threadIdx.x] = int(threadIdx.x) < W.nv ? W.state[threadIdx.x].status : 1;
+    __syncthreads();
+    // every sum in one pass: thread t takes partials t, t + 256, ... (a fixed order)
+    double acc[K];
 #pragma unroll
-    for (int k = 0; k < 2 * NM; ++k)
-      part[k] = 0.;
-    double alpha[NM];
-    bool   act[NM];
+    for (int k = 0; k < K; ++k)
+      acc[k] = 0.;
+    for (int bb = threadIdx.x; bb < nparts; bb += kThreads)
 #pragma unroll
-    for (int v = 0; v < NM; ++v)
-      {
-        act[v]   = v < W.nv && W.state[v].status == 0;
-        alpha[v] = act[v] ? W.state[v].alpha : 0.;
-      }
-    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
-      {
-        const int    i = int(k % G.np), j = int(k / G.np);
-        const double d = mass_diag(G, i, j);
+      for (int v = 0; v < NM; ++v)
+        if (v < W.nv && (STAGE == 1 || status0[v] == 0))
 #pragma unroll
-        for (int v = 0; v < NM; ++v)
-          if (act[v])
-            {
-              const int64_t o  = int64_t(W.block[v]) * G.stride + k;
-              x[o]             = x[o] + alpha[v] * p[o];
-              const double rr  = r[o] + alpha[v] * q[o];
-              r[o]             = rr;
-              part[2 * v] += rr * rr;
-              part[2 * v + 1] += rr * (rr / d);
-            }
-      }
-    block_sum<2 * NM>(part, red);
+          for (int k = 0; k < nk; ++k)
+            acc[stride * v + k] += W.partial[size_t(bb) * kPart + stride * v + k];
+    block_sum<K>(acc, red);
     if (threadIdx.x == 0)
-      for (int k = 0; k < 2 * NM; ++k)
-        W.partial[size_t(k) * W.max_blocks + blockIdx.x] = part[k];
-    if (!last_block(W.counter, gridDim.x))
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        tot[k] = acc[k];
+    if (threadIdx.x != 0)
       return;
-    __shared__ double tot[2 * NM];
     for (int v = 0; v < W.nv; ++v)
       {
-        if (!act[v])
-          continue;
-        for (int k = 0; k < 2; ++k)
+        CgState &st = W.state[v];
+        if constexpr (STAGE == 1)
           {
-            const double s = sum_partials(W.partial + size_t(2 * v + k) * W.max_blocks, gridDim.x, red);
-            if (threadIdx.x == 0)
-              tot[2 * v + k] = s;
+            st.res      = sqrt(tot[3 * v]);
+            st.tol      = W.tol_rel * sqrt(tot[3 * v + 1]);
+            st.rho      = tot[3 * v + 2];
+            st.rho_prev = 0.;
+            st.its      = 0;
+            // SolverControl::check(0, res): success first, then failure at the cap
+            st.status = st.res <= st.tol ? 1 : (W.max_its <= 0 || isnan(st.res)) ? 2 : 0;
           }
-      }
-    if (threadIdx.x == 0)
-      {
-        for (int v = 0; v < W.nv; ++v)
+        else if constexpr (STAGE == 2)
           {
-            if (!act[v])
+            if (status0[v] == 0)
+              st.alpha = st.rho / tot[3 * v];
+          }
+        else
+          {
+            if (status0[v] != 0)
               continue;
-            CgState &st = W.state[v];
-            st.res      = sqrt(fabs(tot[2 * v]));
-            st.its      = it;
+            st.res = sqrt(fabs(tot[2 * v]));
+            st.its = it;
             if (st.res <= st.tol)
               st.status = 1;
             else if (it >= W.max_its || isnan(st.res))
@@ -508,14 +496,66 @@ namespace lbfem_amd
             st.rho      = tot[2 * v + 1];
             st.beta     = st.rho / st.rho_prev;
           }
-        *W.counter = 0u;
       }
+  }
+
+  // x += alpha p; r += alpha q, with the partials of |r|^2 and r.D^-1 r
+  // (the stopping test and beta follow in k_cg_finalize<3>)
+  __global__ void __launch_bounds__(kThreads)
+  k_cg_update(const Geo G, const CgWork W, double *__restrict__ x, double *__restrict__ r,
+              const double *__restrict__ p, const double *__restrict__ q)
+  {
+    __shared__ double red[2 * NM][kThreads / 64];
+    const int64_t     n = int64_t(G.np) * G.np;
+    double            part[2 * NM];
+#pragma unroll
+    for (int k = 0; k < 2 * NM; ++k)
+      part[k] = 0.;
+    double  alpha[NM];
+    int64_t off[NM];
+    bool    act[NM];
+#pragma unroll
+    for (int v = 0; v < NM; ++v)
+      {
+        act[v]   = v < W.nv && W.state[v].status == 0;
+        alpha[v] = act[v] ? W.state[v].alpha : 0.;
+        off[v]   = v < W.nv ? int64_t(W.block[v]) * G.stride : 0;
+      }
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int    i = int(k % G.np), j = int(k / G.np);
+        const double d = mass_diag_inv(G, i, j);
+        double       xv[NM], pv[NM], qv[NM], rv[NM];
+#pragma unroll
+        for (int v = 0; v < NM; ++v)
+          if (act[v])
+            {
+              xv[v] = x[off[v] + k];
+              pv[v] = p[off[v] + k];
+              qv[v] = q[off[v] + k];
+              rv[v] = r[off[v] + k];
+            }
+#pragma unroll
+        for (int v = 0; v < NM; ++v)
+          if (act[v])
+            {
+              x[off[v] + k]   = xv[v] + alpha[v] * pv[v];
+              const double rr = rv[v] + alpha[v] * qv[v];
+              r[off[v] + k]   = rr;
+              part[2 * v] += rr * rr;
+              part[2 * v + 1] += rr * (d * rr);
+            }
+      }
+    block_sum<2 * NM>(part, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 2 * NM; ++k)
+        W.partial[size_t(blockIdx.x) * kPart + k] = part[k];
   }
 
   // ------------------------------------------------------------------ diagnostics
   __global__ void __launch_bounds__(kThreads)
-  k_diagnostics(const Geo G, const double *f, const double *u_exact, double *u_prev, double *sums, double *partial,
-                unsigned *counter, const int max_blocks)
+  k_diagnostics(const Geo G, const double *f, const double *u_exact, double *u_prev, double *partial,
+                const int max_blocks)
   {
     __shared__ double red[6][kThreads / 64];
     const int64_t     n = int64_t(G.np) * G.np;
@@ -548,17 +588,23 @@ namespace lbfem_amd
     block_sum<6>(s, red);
     if (threadIdx.x == 0)
       for (int k = 0; k < 6; ++k)
-        partial[size_t(k) * max_blocks + blockIdx.x] = s[k];
-    if (!last_block(counter, gridDim.x))
-      return;
-    for (int k = 0; k < 6; ++k)
-      {
-        const double t = sum_partials(partial + size_t(k) * max_blocks, gridDim.x, red);
-        if (threadIdx.x == 0)
-          sums[k] = t;
-      }
+        partial[size_t(blockIdx.x) * kPart + k] = s[k];
+  }
+
+  __global__ void __launch_bounds__(kThreads)
+  k_diagnostics_finalize(const double *partial, const int nparts, const int max_blocks, double *sums)
+  {
+    __shared__ double red[6][kThreads / 64];
+    (void)max_blocks;
+    double acc[6] = {0., 0., 0., 0., 0., 0.};
+    for (int bb = threadIdx.x; bb < nparts; bb += kThreads)
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] += partial[size_t(bb) * kPart + k];
+    block_sum<6>(acc, red);
     if (threadIdx.x == 0)
-      *counter = 0u;
+      for (int k = 0; k < 6; ++k)
+        sums[k] = acc[k];
   }
 
   // ------------------------------------------------------------------ launchers
@@ -633,20 +679,22 @@ namespace lbfem_amd
     switch (G.p)
       {
         case 1:
-          hipLaunchKernelGGL((k_operator<1, TX, TY, KIND>), dim3(nb8), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
+          hipLaunchKernelGGL((k_operator<1, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
                              y, y2);
           break;
         case 2:
-          hipLaunchKernelGGL((k_operator<2, TX, TY, KIND>), dim3(nb8), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
+          hipLaunchKernelGGL((k_operator<2, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
                              y, y2);
           break;
         case 3:
-          hipLaunchKernelGGL((k_operator<3, TX, TY, KIND>), dim3(nb8), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
+          hipLaunchKernelGGL((k_operator<3, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
                              y, y2);
           break;
         default:
           return hipErrorInvalidValue;
       }
+    if (KIND != 0)
+      hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, nt, it);
     return hipGetLastError();
   }
 
@@ -671,20 +719,19 @@ namespace lbfem_amd
                    hipStream_t s)
   {
     int nb = nodal_blocks(G);
-    if (nb > W.max_blocks)
-      nb = W.max_blocks;
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, G, W, it, x, r, p, q);
+    nb     = std::min(nb, std::min(W.max_blocks, 1024));
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, G, W, x, r, p, q);
+    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kThreads), 0, s, W, nb, it);
     return hipGetLastError();
   }
   hipError_t
   diagnostics_launch(const Geo &G, const double *f, const double *u_exact, double *u_prev, double *sums,
                      double *partial, unsigned *counter, const int max_blocks, hipStream_t s)
   {
-    int nb = nodal_blocks(G);
-    if (nb > max_blocks)
-      nb = max_blocks;
-    hipLaunchKernelGGL(k_diagnostics, dim3(nb), dim3(kThreads), 0, s, G, f, u_exact, u_prev, sums, partial, counter,
-                       max_blocks);
+    (void)counter;
+    int nb = std::min(nodal_blocks(G), std::min(max_blocks, 1024));
+    hipLaunchKernelGGL(k_diagnostics, dim3(nb), dim3(kThreads), 0, s, G, f, u_exact, u_prev, partial, max_blocks);
+    hipLaunchKernelGGL(k_diagnostics_finalize, dim3(1), dim3(kThreads), 0, s, partial, nb, max_blocks, sums);
     return hipGetLastError();
   }
 } // namespace lbfem_amd

@@ -24,13 +24,6 @@ namespace lbfem_amd
   {
     constexpr int kThreads = 256;
 
-    __device__ __forceinline__ double
-    feq_rt(const int a, const double rho, const double ux, const double uy, const double uu)
-    {
-      const double eu = double(ex_of(a)) * ux + double(ey_of(a)) * uy;
-      return w_of(a) * rho * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * uu);
-    }
-
     // Band coefficient o in [-p, p] of row i.
     template <int p>
     __device__ __forceinline__ double
@@ -58,7 +51,6 @@ namespace lbfem_amd
     constexpr int  W = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
     constexpr int  TR = kThreads / TX; // thread rows
     constexpr int  RO = TY / TR;       // owned nodes per thread, consecutive rows of one column
-    constexpr int  SR = RO + 2 * p;    // strip rows
     static_assert(TY % TR == 0 && kThreads % TX == 0, "tile shape");
     constexpr bool LL   = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
     constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);

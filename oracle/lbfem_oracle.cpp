@@ -898,8 +898,8 @@ namespace
       {
         ++it;
         rho_prev = rho;
-        for (int i = 0; i < n; ++i)
-          v[i] = r[i] / d.diag[i];
+        for (int i = 0; i < n; ++i) // DiagonalMatrix::vmult with the inverse entries 1/diag
+          v[i] = (1. / d.diag[i]) * r[i];
         rho = dot(r.data(), v.data(), n);
         if (it > 1)
           {
