@@ -28,12 +28,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=400.0, cfl=0.5):
+def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=400.0, cfl=0.5, mixed=False):
     tc = A.test_cases.make("cavity", mach=0.1 / (1 / 3) ** 0.5, stretch=1.2)  # U0 = Ma c_s = 0.1
     tc.set_reynolds(Re)
     disc = A.Discretization(tc, N, p)
     cls = A.LeeLin if leelin else A.Bardow
-    s = cls(disc, mass=A.MASS_LUMPED if mass is None else mass, cg_tolerance=cg_tol)
+    s = cls(disc, mass=A.MASS_LUMPED if mass is None else mass, cg_tolerance=cg_tol,
+            precision=A.MIXED if mixed else A.FP64)
     dt = disc.time_step(cfl)
     s.set_time_step(dt, tc.relaxation_time())
     s.set_initial_populations(tc)
@@ -94,9 +95,10 @@ def main():
     variants = {}
     # ---- consistent mass (CG tol 1e-10), and Q2 128^2
     for name, N, p, mass in (("q1_256_leelin_cg", 256, 1, A.MASS_CG), ("q2_128_leelin_lumped", 128, 2, A.MASS_LUMPED),
-                             ("q2_128_leelin_cg", 128, 2, A.MASS_CG), ("q1_256_bardow_lumped", 256, 1, None)):
-        leelin = not name.endswith("bardow_lumped")
-        d2, s2 = make_scheme(A, N, p, leelin=leelin, mass=mass)
+                             ("q2_128_leelin_cg", 128, 2, A.MASS_CG), ("q1_256_bardow_lumped", 256, 1, None),
+                             ("q1_256_leelin_lumped_mixed", 256, 1, None), ("q2_128_leelin_lumped_mixed", 128, 2, None)):
+        leelin = "bardow" not in name
+        d2, s2 = make_scheme(A, N, p, leelin=leelin, mass=mass, mixed=name.endswith("mixed"))
         steps = max(10, K // 4) if mass == A.MASS_CG else K
         its0 = s2.cg_iterations
         sec2 = timed(torch, s2, steps, W)
@@ -109,13 +111,14 @@ def main():
     # ---- HBM-resident sweep (working set > 256 MB Infinity Cache)
     sweep = []
     if not args.no_sweep:
-        for N, steps in ((2048, 40), (4096, 12)):
-            d3, s3 = make_scheme(A, N, 1)
+        for N, steps, mixed in ((2048, 40, False), (4096, 12, False), (2048, 40, True), (4096, 12, True)):
+            d3, s3 = make_scheme(A, N, 1, mixed=mixed)
             sec3 = timed(torch, s3, steps, 3)
             kt3 = kernel_time(s3, steps)
             tk = kt3["advection"] / steps
-            gbs = 18 * 8 * d3.n_nodes / tk / 1e9
-            sweep.append({"cells": f"{N}x{N}", "degree": 1, "n_nodes": d3.n_nodes, "ms_per_step": 1e3 * sec3 / steps,
+            gbs = 18 * (4 if mixed else 8) * d3.n_nodes / tk / 1e9
+            sweep.append({"cells": f"{N}x{N}", "degree": 1, "form": "mixed" if mixed else "fp64",
+                          "n_nodes": d3.n_nodes, "ms_per_step": 1e3 * sec3 / steps,
                           "MLUPS": 9 * d3.n_nodes * steps / sec3 / 1e6, "kernel_us": 1e6 * tk,
                           "achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)})
             log("sweep", sweep[-1])

@@ -148,7 +148,9 @@ struct lbfem_scheme
   double                dt = 0., lambda = 0.;
   hipStream_t           stream = nullptr;
   DevBuf<double>        f[2];
-  int                   cur = 0;
+  DevBuf<float>         g32[2]; // the mixed form: FP32 deviations f - w rho0
+  bool                  mixed = false;
+  int                   cur   = 0;
   DevBuf<double>        rhs, x[2], r, pb[2], q, ax_tmp;
   DevBuf<CgState>       state;
   DevBuf<double>        partial;
@@ -385,8 +387,12 @@ namespace
         TileParams S = tile_params(s, s.sweeps[0], ll);
         S.collide    = 1;
         s.mark(-1);
-        HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, G, S, s.f[s.cur].p, nullptr, s.f[1 - s.cur].p, nullptr,
-                              s.stream));
+        if (s.mixed)
+          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, G, S, s.g32[s.cur].p, (const float *)nullptr,
+                                s.g32[1 - s.cur].p, (float *)nullptr, s.stream));
+        else
+          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, G, S, s.f[s.cur].p, nullptr, s.f[1 - s.cur].p, nullptr,
+                                s.stream));
         s.mark(1);
         s.cur = 1 - s.cur;
         return LBFEM_OK;
@@ -795,17 +801,28 @@ extern "C"
         return fail(LBFEM_EINVAL, "TG3 streaming needs the consistent mass (Mass::cg)");
       if (set->scheme == LBFEM_LEELIN && set->streaming != LBFEM_TG2) // schemes.h:88
         return fail(LBFEM_EINVAL, "LeeLin streams with TG2 only");
-      if (set->precision != LBFEM_FP64)
-        return fail(LBFEM_EINVAL, "only the FP64 form is available");
+      if (set->precision != LBFEM_FP64 && set->precision != LBFEM_MIXED)
+        return fail(LBFEM_EINVAL, "unknown precision");
+      if (set->precision == LBFEM_MIXED && !(m.type == LBFEM_MASS_LUMPED && set->streaming == LBFEM_TG2))
+        return fail(LBFEM_EINVAL, "the mixed-precision form is available for TG2 with the lumped mass");
       auto s = std::make_unique<lbfem_scheme>();
       s->d   = d;
       s->set = *set;
       HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
       const size_t st = size_t(d->stride);
-      s->f[0].alloc(Q * st);
-      s->f[1].alloc(Q * st);
-      HIP_CHECK(hipMemset(s->f[0].p, 0, Q * st * sizeof(double)));
-      HIP_CHECK(hipMemset(s->f[1].p, 0, Q * st * sizeof(double)));
+      s->mixed        = set->precision == LBFEM_MIXED;
+      if (s->mixed)
+        for (auto &b : s->g32)
+          {
+            b.alloc(Q * st);
+            HIP_CHECK(hipMemset(b.p, 0, Q * st * sizeof(float)));
+          }
+      else
+        for (auto &b : s->f)
+          {
+            b.alloc(Q * st);
+            HIP_CHECK(hipMemset(b.p, 0, Q * st * sizeof(double)));
+          }
       std::array<double, 2 * Q> all{};
       for (int a = 0; a < Q; ++a)
         {
@@ -883,6 +900,18 @@ extern "C"
       if (!s || !f)
         return fail(LBFEM_EINVAL, "null argument");
       const int64_t n = s->d->n;
+      if (s->mixed) // deviations from the rest state, rounded to FP32
+        {
+          std::vector<float> dv(static_cast<size_t>(n));
+          for (int a = 0; a < Q; ++a)
+            {
+              for (int64_t i = 0; i < n; ++i)
+                dv[i] = float(f[a * n + i] - w_of(a));
+              HIP_CHECK(hipMemcpy(s->g32[s->cur].p + a * s->d->stride, dv.data(), n * sizeof(float),
+                                  hipMemcpyHostToDevice));
+            }
+          return LBFEM_OK;
+        }
       for (int a = 0; a < Q; ++a)
         HIP_CHECK(hipMemcpyAsync(s->f[s->cur].p + a * s->d->stride, f + a * n, n * sizeof(double),
                                  hipMemcpyHostToDevice, s->stream));
@@ -936,6 +965,19 @@ extern "C"
       if (!s || !f)
         return fail(LBFEM_EINVAL, "null argument");
       const int64_t n = s->d->n;
+      if (s->mixed)
+        {
+          HIP_CHECK(hipStreamSynchronize(s->stream));
+          std::vector<float> dv(static_cast<size_t>(n));
+          for (int a = 0; a < Q; ++a)
+            {
+              HIP_CHECK(hipMemcpy(dv.data(), s->g32[s->cur].p + a * s->d->stride, n * sizeof(float),
+                                  hipMemcpyDeviceToHost));
+              for (int64_t i = 0; i < n; ++i)
+                f[a * n + i] = w_of(a) + double(dv[i]);
+            }
+          return LBFEM_OK;
+        }
       for (int a = 0; a < Q; ++a)
         HIP_CHECK(hipMemcpyAsync(f + a * n, s->f[s->cur].p + a * s->d->stride, n * sizeof(double),
                                  hipMemcpyDeviceToHost, s->stream));
@@ -949,6 +991,8 @@ extern "C"
   {
     if (!s || !f)
       return fail(LBFEM_EINVAL, "null argument");
+    if (s->mixed)
+      return fail(LBFEM_EINVAL, "the mixed form keeps FP32 deviations; use lbfem_scheme_get_populations");
     *f = s->f[s->cur].p;
     return LBFEM_OK;
   }

@@ -29,6 +29,9 @@ namespace lbfem_amd
 
   hipError_t tile_launch(int mode, const Geo &G, const TileParams &S, const double *fin, const double *feq,
                          double *out, double *rhs, hipStream_t stream);
+  // the mixed form: FP32 deviations from the rest state
+  hipError_t tile_launch(int mode, const Geo &G, const TileParams &S, const float *fin, const float *feq, float *out,
+                         float *rhs, hipStream_t stream);
 
   // nodal operations over every node (collision.h:78-168)
   hipError_t collide_launch(const Geo &G, double *g, double omega, hipStream_t stream);

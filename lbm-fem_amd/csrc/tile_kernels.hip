@@ -42,11 +42,13 @@ namespace lbfem_amd
     }
   } // namespace
 
-  // bands: 0 M, 1 D, 2 K, 3 D^T
-  template <int p, int TX, int TY, int MODE>
+  // T: the storage and arithmetic type of the populations: double (the FP64
+  // form) or float (the mixed form, deviations from the rest state; moment sums
+  // in FP64). bands: 0 M, 1 D, 2 K, 3 D^T
+  template <typename T, int p, int TX, int TY, int MODE>
   __global__ void __launch_bounds__(kThreads)
-  k_tile(const Geo G, const TileParams S, const double *__restrict__ fin, const double *__restrict__ feq_in,
-         double *__restrict__ out, double *__restrict__ rhs_out)
+  k_tile(const Geo G, const TileParams S, const T *__restrict__ fin, const T *__restrict__ feq_in,
+         T *__restrict__ out, T *__restrict__ rhs_out)
   {
     constexpr int  W = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
     constexpr int  TR = kThreads / TX; // thread rows
@@ -56,7 +58,7 @@ namespace lbfem_amd
     constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
     constexpr int  NF   = LL ? Q + NM : Q; // f (or g*) and, for Lee-Lin, h of the moving populations
 
-    __shared__ double F[NF][HY][HX];
+    __shared__ T F[NF][HY][HX];
 
     // XCD-aware tile order: consecutive tiles of a row band go to one XCD (one L2)
     const int nt   = S.tiles_x * S.tiles_y;
@@ -67,12 +69,12 @@ namespace lbfem_amd
       return;
     const int i0 = (tile % S.tiles_x) * TX, j0 = (tile / S.tiles_x) * TY;
     const int P = G.P, tid = threadIdx.x;
-    const double a_c = S.a, b_c = S.b, c_c = S.c, hdt2 = 0.5 * S.dt * S.dt;
+    const T a_c = T(S.a), b_c = T(S.b), c_c = T(S.c), hdt2 = T(0.5 * S.dt * S.dt);
 
     // ---------------------------------------------------------- prologue
     // all loads of the halo'd block first (R rounds in flight), then the nodal work
     constexpr int R = (NH + kThreads - 1) / kThreads;
-    double        v[R][Q];
+    T             v[R][Q];
     int           gis[R], gjs[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -105,11 +107,11 @@ namespace lbfem_amd
         const bool wall = ok && wall_of(G, gis[r], gjs[r], wx, wy);
         if constexpr (LL)
           {
-            double fe[Q];
+            T fe[Q];
             if (!ok)
 #pragma unroll
               for (int a = 0; a < Q; ++a)
-                fe[a] = 0.;
+                fe[a] = T(0);
             else if (feq_in == nullptr)
               equilibrium_node(v[r], fe, wall, wx, wy);
             else
@@ -130,29 +132,29 @@ namespace lbfem_amd
     // the rows in registers (bands: 0 M, 1 D, 2 K, 3 D^T)
     const int ii = tid % TX, jr = (tid / TX) * RO;
     const int gi = i0 + ii;
-    double    xc[4][W], yc[RO][4][W];
+    T         xc[4][W], yc[RO][4][W];
 #pragma unroll
     for (int o = 0; o < W; ++o)
       {
         const bool in = gi <= P;
-        xc[0][o]      = in ? band<p>(G.ax.M, gi, o - p) : 0.;
-        xc[1][o]      = in ? band<p>(G.ax.D, gi, o - p) : 0.;
-        xc[2][o]      = in ? band<p>(G.ax.K, gi, o - p) : 0.;
-        xc[3][o]      = in ? band<p>(G.ax.DT, gi, o - p) : 0.;
+        xc[0][o]      = in ? T(band<p>(G.ax.M, gi, o - p)) : T(0);
+        xc[1][o]      = in ? T(band<p>(G.ax.D, gi, o - p)) : T(0);
+        xc[2][o]      = in ? T(band<p>(G.ax.K, gi, o - p)) : T(0);
+        xc[3][o]      = in ? T(band<p>(G.ax.DT, gi, o - p)) : T(0);
 #pragma unroll
         for (int r = 0; r < RO; ++r)
           {
             const int  gj = j0 + jr + r;
             const bool iy = gj <= P;
-            yc[r][0][o]   = iy ? band<p>(G.ay.M, gj, o - p) : 0.;
-            yc[r][1][o]   = iy ? band<p>(G.ay.D, gj, o - p) : 0.;
-            yc[r][2][o]   = iy ? band<p>(G.ay.K, gj, o - p) : 0.;
-            yc[r][3][o]   = iy ? band<p>(G.ay.DT, gj, o - p) : 0.;
+            yc[r][0][o]   = iy ? T(band<p>(G.ay.M, gj, o - p)) : T(0);
+            yc[r][1][o]   = iy ? T(band<p>(G.ay.D, gj, o - p)) : T(0);
+            yc[r][2][o]   = iy ? T(band<p>(G.ay.K, gj, o - p)) : T(0);
+            yc[r][3][o]   = iy ? T(band<p>(G.ay.DT, gj, o - p)) : T(0);
           }
       }
     __syncthreads();
 
-    double xinc[RO][NM], mflux[RO];
+    T xinc[RO][NM], mflux[RO];
 #pragma unroll
     for (int r = 0; r < RO; ++r)
       {
@@ -168,36 +170,47 @@ namespace lbfem_amd
       {
         // directions as uniform run-time values: compile-time D2Q9 constants let the
         // scheduler interleave the barrier-free populations and spill registers
-        const double ex = S.e[al][0], ey = S.e[al][1];
-        const bool   nx = ex != 0., ny = ey != 0.;
+        const T ex = T(S.e[al][0]), ey = T(S.e[al][1]);
+        const bool nx = ex != T(0), ny = ey != T(0);
         if (!nx && !ny)
           continue; // at rest in this sweep: r = 0 (streaming.h:118-122)
-        const double(*Ff)[HX] = F[al];
-        const double(*Fh)[HX] = F[LL ? Q + al - 1 : al];
-        double acc[RO];
+        const T(*Ff)[HX] = F[al];
+        const T(*Fh)[HX] = F[LL ? Q + al - 1 : al];
+        T acc[RO];
 #pragma unroll
         for (int r = 0; r < RO; ++r)
           acc[r] = 0.;
         // r = Dx (ex My h + c ex ey Dy^T f) + Kx (c ex^2 My f) + Mx (ey Dy h + c ey^2 Ky f) + Dx^T (c ex ey Dy f),
         // h = f for Bardow (its a folded into the x weights); one branch-free body per direction type
-        const double sa = LL ? 1. : a_c;
+        const T sa = LL ? T(1) : a_c;
+        // the mixed form applies the operator to f - f(node): every term annihilates
+        // constants (its bands' row or column sums vanish), and FP32 then carries the
+        // small differences instead of cancelling large values
+        constexpr bool CEN = sizeof(T) == 4;
+        T              cf[RO], ch[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+          {
+            cf[r] = CEN ? Ff[jr + r + p][ii + p] : T(0);
+            ch[r] = (CEN && LL) ? Fh[jr + r + p][ii + p] : T(0);
+          }
         if (nx && !ny)
           {
-            const double wd = ex * sa, wk = c_c * ex * ex;
+            const T wd = ex * sa, wk = c_c * ex * ex;
 #pragma unroll
             for (int c = 0; c < W; ++c)
               {
-                const double kd = xc[1][c] * wd, kk = xc[2][c] * wk;
+                const T kd = xc[1][c] * wd, kk = xc[2][c] * wk;
 #pragma unroll
                 for (int r = 0; r < RO; ++r)
                   {
-                    double My = 0., Myh = 0.;
+                    T My = 0, Myh = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        My += yc[r][0][o] * Ff[jr + r + o][ii + c];
+                        My += yc[r][0][o] * (Ff[jr + r + o][ii + c] - cf[r]);
                         if constexpr (LL)
-                          Myh += yc[r][0][o] * Fh[jr + r + o][ii + c];
+                          Myh += yc[r][0][o] * (Fh[jr + r + o][ii + c] - ch[r]);
                       }
                     if constexpr (!LL)
                       Myh = My;
@@ -207,19 +220,19 @@ namespace lbfem_amd
           }
         else if (!nx && ny)
           {
-            const double wd = ey * sa, wk = c_c * ey * ey;
+            const T wd = ey * sa, wk = c_c * ey * ey;
 #pragma unroll
             for (int c = 0; c < W; ++c)
               {
 #pragma unroll
                 for (int r = 0; r < RO; ++r)
                   {
-                    double s = 0.;
+                    T s = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        const double fv = Ff[jr + r + o][ii + c];
-                        const double hv = LL ? Fh[jr + r + o][ii + c] : fv;
+                        const T fv = Ff[jr + r + o][ii + c] - cf[r];
+                        const T hv = LL ? Fh[jr + r + o][ii + c] - ch[r] : fv;
                         s += (yc[r][1][o] * wd) * hv + (yc[r][2][o] * wk) * fv;
                       }
                     acc[r] += xc[0][c] * s;
@@ -228,25 +241,25 @@ namespace lbfem_amd
           }
         else
           {
-            const double wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
+            const T wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
 #pragma unroll
             for (int c = 0; c < W; ++c)
               {
 #pragma unroll
                 for (int r = 0; r < RO; ++r)
                   {
-                    double My = 0., Dy = 0., Ky = 0., DTy = 0., Myh = 0., Dyh = 0.;
+                    T My = 0, Dy = 0, Ky = 0, DTy = 0, Myh = 0, Dyh = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        const double fv = Ff[jr + r + o][ii + c];
+                        const T fv = Ff[jr + r + o][ii + c] - cf[r];
                         My += yc[r][0][o] * fv;
                         Dy += yc[r][1][o] * fv;
                         Ky += yc[r][2][o] * fv;
                         DTy += yc[r][3][o] * fv;
                         if constexpr (LL)
                           {
-                            const double hv = Fh[jr + r + o][ii + c];
+                            const T hv = Fh[jr + r + o][ii + c] - ch[r];
                             Myh += yc[r][0][o] * hv;
                             Dyh += yc[r][1][o] * hv;
                           }
@@ -268,7 +281,7 @@ namespace lbfem_amd
           {
             const int jj = jr + r, gj = j0 + jj;
             const int ty = jj + p, tx = ii + p;
-            double    face = 0.;
+            T         face = 0;
             if (gi <= P && gj <= P)
               {
                 if (!G.ay.periodic && ny && (gj == 0 || gj == P))
@@ -276,14 +289,14 @@ namespace lbfem_amd
                     const bool    lo = gj == 0;
                     const double *bn = lo ? G.ay.b0 : G.ay.b1;
                     const int     r0 = lo ? ty : ty - p;
-                    double        t = 0., nrm = 0.;
+                    T             t = 0, nrm = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        double dn = 0.;
+                        T dn = 0;
 #pragma unroll
                         for (int l = 0; l <= p; ++l)
-                          dn += bn[l] * Ff[r0 + l][ii + o];
+                          dn += T(bn[l]) * Ff[r0 + l][ii + o];
                         t += xc[1][o] * Ff[ty][ii + o];
                         nrm += xc[0][o] * dn;
                       }
@@ -294,14 +307,14 @@ namespace lbfem_amd
                     const bool    lo = gi == 0;
                     const double *bn = lo ? G.ax.b0 : G.ax.b1;
                     const int     c0 = lo ? tx : tx - p;
-                    double        t = 0., nrm = 0.;
+                    T             t = 0, nrm = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        double dn = 0.;
+                        T dn = 0;
 #pragma unroll
                         for (int l = 0; l <= p; ++l)
-                          dn += bn[l] * Ff[jj + o][c0 + l];
+                          dn += T(bn[l]) * Ff[jj + o][c0 + l];
                         t += yc[r][1][o] * Ff[jj + o][tx];
                         nrm += yc[r][0][o] * dn;
                       }
@@ -322,7 +335,7 @@ namespace lbfem_amd
           continue;
         const int64_t idx   = node_index(G, gi, gj);
         const bool    slave = (G.ax.periodic && gi == P) || (G.ay.periodic && gj == P);
-        double        own[Q];
+        T             own[Q];
         if (!slave)
 #pragma unroll
           for (int a = 0; a < Q; ++a)
@@ -345,10 +358,10 @@ namespace lbfem_amd
         if (S.remove_mass_flux)
 #pragma unroll
           for (int a = 1; a < Q; ++a)
-            xinc[r][a - 1] -= (w_of(a) / (1. - w_of(0))) * mflux[r];
+            xinc[r][a - 1] -= T(w_of(a) / (1. - w_of(0))) * mflux[r];
         if constexpr (STEP)
           {
-            const double mli = slave ? 0. : G.ax.ml_inv[gi] * G.ay.ml_inv[gj];
+            const T mli = slave ? T(0) : T(G.ax.ml_inv[gi] * G.ay.ml_inv[gj]);
 #pragma unroll
             for (int a = 0; a < NM; ++a)
               xinc[r][a] *= mli;
@@ -356,7 +369,7 @@ namespace lbfem_amd
               {
                 double     wx = 0., wy = 0.;
                 const bool wall = wall_of(G, gi, gj, wx, wy);
-                double     fe[Q];
+                T          fe[Q];
                 equilibrium_node(own, fe, wall, wx, wy);
                 predictor_corrector_node(own, fe, xinc[r], S.theta, wall, wx, wy);
               }
@@ -373,7 +386,7 @@ namespace lbfem_amd
             // right-hand side (constrained entries get nothing) and, fused, g*
 #pragma unroll
             for (int a = 0; a < NM; ++a)
-              rhs_out[a * G.stride + idx] = slave ? 0. : xinc[r][a];
+              rhs_out[a * G.stride + idx] = slave ? T(0) : xinc[r][a];
             if (out != nullptr)
 #pragma unroll
               for (int a = 0; a < Q; ++a)
@@ -383,34 +396,54 @@ namespace lbfem_amd
   }
 
   // ------------------------------------------------------------------ launch
-  template <int p, int TX, int TY, int MODE>
+  template <typename T, int p, int TX, int TY, int MODE>
   static hipError_t
-  launch_tile(const Geo &G, TileParams S, const double *fin, const double *feq, double *out, double *rhs,
-              hipStream_t stream)
+  launch_tile(const Geo &G, TileParams S, const T *fin, const T *feq, T *out, T *rhs, hipStream_t stream)
   {
     S.tiles_x     = (G.P + 1 + TX - 1) / TX;
     S.tiles_y     = (G.P + 1 + TY - 1) / TY;
     const int nt  = S.tiles_x * S.tiles_y;
     const int nb8 = (nt + 7) & ~7;
-    hipLaunchKernelGGL((k_tile<p, TX, TY, MODE>), dim3(nb8), dim3(kThreads), 0, stream, G, S, fin, feq, out, rhs);
+    hipLaunchKernelGGL((k_tile<T, p, TX, TY, MODE>), dim3(nb8), dim3(kThreads), 0, stream, G, S, fin, feq, out, rhs);
     return hipGetLastError();
   }
 
   // tile heights: 16 rows where the staged fields fit in about 55 KB of LDS, else 8
-  template <int MODE>
+  template <typename T, int MODE>
   static hipError_t
-  dispatch_degree(const Geo &G, const TileParams &S, const double *fin, const double *feq, double *out, double *rhs,
-                  hipStream_t stream)
+  dispatch_degree(const Geo &G, const TileParams &S, const T *fin, const T *feq, T *out, T *rhs, hipStream_t stream)
   {
-    constexpr bool LL = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
+    constexpr bool LL   = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
+    constexpr bool F32  = sizeof(T) == 4;
+    constexpr int  TY12 = (LL && !F32) ? 8 : 16;
     switch (G.p)
       {
         case 1:
-          return launch_tile<1, 32, LL ? 8 : 16, MODE>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<T, 1, 32, TY12, MODE>(G, S, fin, feq, out, rhs, stream);
         case 2:
-          return launch_tile<2, 32, LL ? 8 : 16, MODE>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<T, 2, 32, TY12, MODE>(G, S, fin, feq, out, rhs, stream);
         case 3:
-          return launch_tile<3, 32, 8, MODE>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<T, 3, 32, 8, MODE>(G, S, fin, feq, out, rhs, stream);
+        default:
+          return hipErrorInvalidValue;
+      }
+  }
+
+  template <typename T>
+  static hipError_t
+  tile_dispatch(const int mode, const Geo &G, const TileParams &S, const T *fin, const T *feq, T *out, T *rhs,
+                hipStream_t stream)
+  {
+    switch (mode)
+      {
+        case kStepBardow:
+          return dispatch_degree<T, kStepBardow>(G, S, fin, feq, out, rhs, stream);
+        case kStepLeeLin:
+          return dispatch_degree<T, kStepLeeLin>(G, S, fin, feq, out, rhs, stream);
+        case kAdvectBardow:
+          return dispatch_degree<T, kAdvectBardow>(G, S, fin, feq, out, rhs, stream);
+        case kAdvectLeeLin:
+          return dispatch_degree<T, kAdvectLeeLin>(G, S, fin, feq, out, rhs, stream);
         default:
           return hipErrorInvalidValue;
       }
@@ -420,18 +453,13 @@ namespace lbfem_amd
   tile_launch(const int mode, const Geo &G, const TileParams &S, const double *fin, const double *feq, double *out,
               double *rhs, hipStream_t stream)
   {
-    switch (mode)
-      {
-        case kStepBardow:
-          return dispatch_degree<kStepBardow>(G, S, fin, feq, out, rhs, stream);
-        case kStepLeeLin:
-          return dispatch_degree<kStepLeeLin>(G, S, fin, feq, out, rhs, stream);
-        case kAdvectBardow:
-          return dispatch_degree<kAdvectBardow>(G, S, fin, feq, out, rhs, stream);
-        case kAdvectLeeLin:
-          return dispatch_degree<kAdvectLeeLin>(G, S, fin, feq, out, rhs, stream);
-        default:
-          return hipErrorInvalidValue;
-      }
+    return tile_dispatch<double>(mode, G, S, fin, feq, out, rhs, stream);
+  }
+
+  hipError_t
+  tile_launch(const int mode, const Geo &G, const TileParams &S, const float *fin, const float *feq, float *out,
+              float *rhs, hipStream_t stream)
+  {
+    return tile_dispatch<float>(mode, G, S, fin, feq, out, rhs, stream);
   }
 } // namespace lbfem_amd

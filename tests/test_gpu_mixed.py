@@ -1,0 +1,102 @@
+"""The mixed-precision form (FP32 deviations f - w rho0, FP64 moment sums)
+against the FP64 form and the oracle. Tolerance: relative L2 velocity error
+<= 1e-5 against FP64 after the run, as the north-star accuracy bar states."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import lbfem_amd as A  # noqa: E402
+import oracle as O  # noqa: E402
+
+
+def velocity(f):
+    rho, ux, uy = A.moments(f)
+    return rho, ux, uy
+
+
+def l2_rel(u, v, ref_u, ref_v, w):
+    return np.sqrt((w * ((u - ref_u) ** 2 + (v - ref_v) ** 2)).sum() / (w * (ref_u ** 2 + ref_v ** 2)).sum())
+
+
+def run(case, N, p, scheme, precision, steps, Re=100.0, mach=0.1, cfl=0.25, modes=(1, 1)):
+    tc = A.test_cases.make(case, mach=mach, modes=modes, stretch=1.2)
+    tc.set_reynolds(Re)
+    d = A.Discretization(tc, N, p)
+    s = (A.Bardow if scheme == A.BARDOW else A.LeeLin)(d, mass=A.MASS_LUMPED, precision=precision)
+    dt = d.time_step(cfl)
+    s.set_time_step(dt, tc.relaxation_time())
+    s.set_initial_populations(tc)
+    s.step(steps)
+    return d, s, dt, tc
+
+
+@pytest.mark.parametrize("case,p", [("tgv", 1), ("couette", 2), ("cavity", 1), ("cavity", 2), ("tgv", 3)])
+@pytest.mark.parametrize("scheme", [A.BARDOW, A.LEELIN])
+def test_mixed_one_step_vs_oracle(case, p, scheme):
+    N = {1: 24, 2: 12, 3: 8}[p]
+    tc = A.test_cases.make(case, mach=0.1, modes=(1, 1), stretch=1.2)
+    tc.set_reynolds(100.0)
+    d = A.Discretization(tc, N, p)
+    od = O.Disc(N, p, {"tgv": O.TGV, "couette": O.COUETTE, "cavity": O.CAVITY}[case], mach=0.1, modes=(1, 1),
+                stretch=1.2 if case == "cavity" else 0.0, reynolds=100.0)
+    dt, lam = od.time_step(0.25), od.relaxation_time
+    s = (A.Bardow if scheme == A.BARDOW else A.LeeLin)(d, mass=A.MASS_LUMPED, precision=A.MIXED)
+    s.set_time_step(dt, lam)
+    s.set_initial_populations()
+    o = O.Scheme(od, O.BARDOW if scheme == A.BARDOW else O.LEELIN, O.TG2, O.MASS_LUMPED)
+    o.set_time_step(dt, lam)
+    o.set_initial_populations()
+    f0 = o.populations()
+    s.step(1)
+    o.step(1)
+    f1 = o.populations()
+    # error relative to the change the step makes and to the deviations FP32 stores
+    dev = np.abs(f0 - O.W[:, None]).max()
+    err = np.abs(s.f - f1).max()
+    assert err < 2e-6 * max(dev, np.abs(f1 - f0).max()), (err, dev)
+
+
+@pytest.mark.parametrize("case,N,p,scheme,steps", [
+    ("tgv", 64, 1, A.BARDOW, 1000),
+    ("tgv", 64, 1, A.LEELIN, 2000),
+    ("cavity", 64, 1, A.LEELIN, 2000),
+    ("cavity", 32, 2, A.BARDOW, 2000),
+])
+def test_mixed_velocity_error_vs_fp64(case, N, p, scheme, steps):
+    d, s64, dt, tc = run(case, N, p, scheme, A.FP64, steps)
+    _, s32, _, _ = run(case, N, p, scheme, A.MIXED, steps)
+    _, u64, v64 = velocity(s64.f)
+    _, u32, v32 = velocity(s32.f)
+    indep = ~d.constrained()
+    w = d.lumped_mass()[indep]
+    err = l2_rel(u32[indep], v32[indep], u64[indep], v64[indep], w)
+    print(f"{case} N={N} p={p} steps={steps}: relative L2 velocity error mixed vs FP64 = {err:.2e}")
+    assert err < 1e-5, err
+
+
+def test_mixed_decay_matches_analytic():
+    """TGV modes (1,1): the mixed form follows exp(-nu (k1^2 + k2^2) t) as closely as FP64."""
+    d, s32, dt, tc = run("tgv", 64, 1, A.BARDOW, A.MIXED, 1000)
+    _, s64, _, _ = run("tgv", 64, 1, A.BARDOW, A.FP64, 1000)
+    xy = d.node
+    indep = ~d.constrained()
+    w = d.lumped_mass()[indep]
+    _, ue, ve = tc.exact(xy[:, 0], xy[:, 1], 1000 * dt)
+    _, u32, v32 = velocity(s32.f)
+    _, u64, v64 = velocity(s64.f)
+    e32 = l2_rel(u32[indep], v32[indep], ue[indep], ve[indep], w)
+    e64 = l2_rel(u64[indep], v64[indep], ue[indep], ve[indep], w)
+    assert abs(e32 - e64) < 1e-5, (e32, e64)
+
+
+def test_mixed_rejected_outside_lumped_tg2():
+    tc = A.test_cases.make("tgv", mach=0.1)
+    tc.set_reynolds(100.0)
+    d = A.Discretization(tc, 8, 1)
+    with pytest.raises(A.LbfemError):
+        A.Bardow(d, mass=A.MASS_CG, precision=A.MIXED)
