@@ -4,10 +4,13 @@ Workload (the plain-method GPU configuration): lid-driven cavity on the unit
 square, U0 = 0.1 (Ma = 0.1732), tanh clustering 1.2, Q1 on 256 x 256 cells,
 Lee-Lin predictor-corrector, Re 400, CFL 0.5, FP64. The headline value is the
 lumped-mass step (one fused kernel per step); the consistent-mass (CG, tol
-1e-10) step, the Q2 128 x 128 mesh and an HBM-resident Q1 sweep (2048^2,
-4096^2 cells, beyond the 256 MB Infinity Cache) are reported beside it.
+1e-10) step, the Q2 128 x 128 mesh, the mixed-precision form and, with
+--sweep, an HBM-resident Q1 sweep (2048^2, 4096^2 cells, beyond the 256 MB
+Infinity Cache) are reported beside it. A STREAM-like copy measures the
+achievable HBM bandwidth of the device.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--sweep]
+       python bench.py --profile-headline   # the headline steps and the copy only (profiler passes)
 Prints one JSON line.
 """
 import argparse
@@ -22,13 +25,17 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 Q = 9
+# kernel names of the tile variants (Cfg<T, p, TX, TY, RO, PS, mode>), fused Lee-Lin step, FP64, Q1
+TILE_KERNEL = {0: "k_tile<Cfg<double, 1, 32, 8, 1, 1, 1> >", 1: "k_tile<Cfg<double, 1, 32, 8, 1, 2, 1> >",
+               2: "k_tile<Cfg<double, 1, 32, 8, 1, 4, 1> >", 3: "k_tile<Cfg<double, 1, 16, 8, 1, 4, 1> >"}
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_tile_cavity_q1_256.txt")
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=400.0, cfl=0.5, mixed=False):
+def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=400.0, cfl=0.5, mixed=False, tune=True):
     tc = A.test_cases.make("cavity", mach=0.1 / (1 / 3) ** 0.5, stretch=1.2)  # U0 = Ma c_s = 0.1
     tc.set_reynolds(Re)
     disc = A.Discretization(tc, N, p)
@@ -38,6 +45,8 @@ def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=400.0, cfl=0.5
     dt = disc.time_step(cfl)
     s.set_time_step(dt, tc.relaxation_time())
     s.set_initial_populations(tc)
+    if tune:  # the tile shape is measured once per kernel and mesh, before any timing
+        s.autotune()
     return disc, s
 
 
@@ -52,13 +61,50 @@ def timed(torch, s, steps, warmup):
     return time.perf_counter() - t0
 
 
-def kernel_time(s, steps):
-    """Average duration of the step's advection-stage kernel(s), from HIP events on the scheme's stream."""
-    s.enable_stage_timers(True)
+def kernel_time(torch, s, steps):
+    """Average time per step from two HIP events on the scheme's own stream around `steps` queued steps.
+    A lumped TG2 step is one launch of the fused tile kernel, so this is that kernel's average launch
+    duration (back-to-back launches on one stream)."""
+    st = torch.cuda.ExternalStream(s.stream_handle)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.synchronize()
+    e0.record(st)
     s.step(steps)
-    t = s.stage_times()
-    s.enable_stage_timers(False)
-    return t
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) * 1e-3 / steps
+
+
+def hbm_probe(torch, A, nbytes=1 << 30, reps=10):
+    """Copy bandwidth (read + write bytes / time) of lbfem_hbm_copy over nbytes, HIP events on torch's stream."""
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    st = torch.cuda.current_stream()
+    A.hbm_copy(dst, src, st.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        A.hbm_copy(dst, src, st.cuda_stream)
+    e1.record(st)
+    e1.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    ok = bool(torch.equal(dst[:1024], src[:1024]))
+    del src, dst
+    return gbs, ok
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of the headline kernel (variant) from the committed profiler summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy kernel's reading of its known size), or None."""
+    key = f"traffic_bytes_per_launch[{kernel}]:"
+    try:
+        with open(PMC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith(key):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -67,7 +113,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep", action="store_true")
+    ap.add_argument("--profile-headline", action="store_true")
+    ap.add_argument("--profile-cells", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -80,13 +128,27 @@ def main():
     assert torch.cuda.is_available(), "bench needs a GPU"
     K, W = args.steps, args.warmup
 
+    if args.profile_headline:
+        # profiler passes: the headline steps, then the copy probe over 256 MiB (known size)
+        N = args.profile_cells
+        disc, s = make_scheme(A, N, 1)
+        log(f"profile-headline: tile variant {s.tile_variant} ({TILE_KERNEL[s.tile_variant]})")
+        timed(torch, s, K, W)
+        gbs, ok = hbm_probe(torch, A, nbytes=256 << 20, reps=3)
+        log(f"profile-headline: {K + W} steps of cavity Q1 {N}^2, copy probe 3 x 256 MiB at {gbs:.0f} GB/s (ok={ok})")
+        return
+
+    hbm_gbs, hbm_ok = hbm_probe(torch, A)
+    log(f"copy probe: {hbm_gbs:.0f} GB/s (read + write), checked={hbm_ok}")
+
     # ---- headline: cavity Q1 256^2, Lee-Lin, lumped (fused kernel)
-    disc, s = make_scheme(A, 256, 1)
+    disc, s = make_scheme(A, 256, 1, tune=False)
+    variant, tune_s = s.autotune(force=True)
+    log("autotune (s per launch by variant):", [round(1e6 * t, 2) for t in tune_s], "us -> variant", variant)
     n = disc.n_nodes
     sec = timed(torch, s, K, W)
     value = 9 * n * K / sec / 1e6
-    kt = kernel_time(s, K)
-    t_kernel = kt["advection"] / K
+    t_kernel = kernel_time(torch, s, K)
     alg_bytes = 18 * 8 * n
     achieved = alg_bytes / t_kernel / 1e9
     log(f"cavity Q1 256^2 leelin lumped: {1e3 * sec / K:.4f} ms/step, {value:.1f} MLUPS, kernel {1e6 * t_kernel:.2f} us, "
@@ -104,18 +166,17 @@ def main():
         sec2 = timed(torch, s2, steps, W)
         its = (s2.cg_iterations - its0) / (8 * (steps + W)) if mass == A.MASS_CG else 0.0
         variants[name] = {"ms_per_step": 1e3 * sec2 / steps, "MLUPS": 9 * d2.n_nodes * steps / sec2 / 1e6,
-                          "cg_its_per_solve": round(its, 2)}
+                          "cg_its_per_solve": round(its, 2), "tile_variant": s2.tile_variant}
         log(name, variants[name])
         del s2, d2
 
     # ---- HBM-resident sweep (working set > 256 MB Infinity Cache)
     sweep = []
-    if not args.no_sweep:
+    if args.sweep:
         for N, steps, mixed in ((2048, 40, False), (4096, 12, False), (2048, 40, True), (4096, 12, True)):
             d3, s3 = make_scheme(A, N, 1, mixed=mixed)
             sec3 = timed(torch, s3, steps, 3)
-            kt3 = kernel_time(s3, steps)
-            tk = kt3["advection"] / steps
+            tk = kernel_time(torch, s3, steps)
             gbs = 18 * (4 if mixed else 8) * d3.n_nodes / tk / 1e9
             sweep.append({"cells": f"{N}x{N}", "degree": 1, "form": "mixed" if mixed else "fp64",
                           "n_nodes": d3.n_nodes, "ms_per_step": 1e3 * sec3 / steps,
@@ -159,10 +220,11 @@ def main():
                                "Re 400, CFL 0.5, U0 0.1, stretch 1.2",
                    "n_nodes": n, "mnups": round(value / 9, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_tile<1,32,8,kStepLeeLin>", "kernel_us": round(1e6 * t_kernel, 3),
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(TILE_KERNEL[variant]),
+                     "kernel": TILE_KERNEL[variant], "kernel_us": round(1e6 * t_kernel, 3),
+                     "algorithmic_bytes_per_launch": alg_bytes, "measured_copy_GBs": round(hbm_gbs, 1)},
         "cpu_baseline": cpu,
+        "tile_variant": variant, "autotune_us": [round(1e6 * t, 3) for t in tune_s],
         "variants": variants,
         "hbm_sweep": sweep,
     }

@@ -162,6 +162,9 @@ int lbfem_scheme_device_populations(lbfem_scheme *scheme, double **f);
  * stream; returns after the work is queued unless CG needs host checks. */
 int lbfem_scheme_step(lbfem_scheme *scheme, int n_steps);
 int lbfem_scheme_synchronize(lbfem_scheme *scheme);
+/* The scheme's HIP stream (hipStream_t as void*), on which step() queues its
+ * kernels: for callers that order their own work or events after a step. */
+int lbfem_scheme_get_stream(const lbfem_scheme *scheme, void **stream);
 /* MassSolver::cg_iterations (mass.h:215), summed over all solves. */
 int lbfem_scheme_cg_iterations(const lbfem_scheme *scheme, int64_t *iterations);
 /* StageTimers (include/lbfem/timers.h:17-23): seconds in collision,
@@ -169,12 +172,33 @@ int lbfem_scheme_cg_iterations(const lbfem_scheme *scheme, int64_t *iterations);
 int lbfem_scheme_enable_stage_timers(lbfem_scheme *scheme, int enable);
 int lbfem_scheme_stage_times(const lbfem_scheme *scheme, double *seconds4);
 
+/* No counterpart in the reference (its deal.II kernels have no launch shape):
+ * the shape of the scheme's tile kernel, a TileVariant (0 wide 32 x 8/16
+ * tiles, 1 and 2: 32 x 8 tiles with a node's populations split over 2 or 4
+ * threads, 3: 16 x 8 tiles split over 4; -1 restores the size-based default).
+ * Every variant gives bitwise identical populations. */
+int lbfem_scheme_set_tile_variant(lbfem_scheme *scheme, int variant);
+int lbfem_scheme_get_tile_variant(const lbfem_scheme *scheme, int *variant);
+/* Times every variant of the scheme's tile kernel on the scheme's own buffers
+ * (its state is not changed) and keeps the fastest; the choice is cached per
+ * (kernel mode, degree, mesh size, periodicity) for the process and reused
+ * unless force != 0. seconds (optional): per-variant seconds per launch (-1
+ * where not timed). Synchronizes the scheme's stream. */
+int lbfem_scheme_autotune(lbfem_scheme *scheme, int force, int *variant, double *seconds);
+
 /* The diagnostics sums of cgdbe (apps/cgdbe/cgdbe.cc:353-368) over the
  * non-slave nodes with lumped-mass weights, from device populations:
  * sums[0..5] = |u-u_ex|^2, |u_ex|^2, |u|^2, rho, area, |u-u_prev|;
  * u_exact and u_prev: device arrays of 2 doubles per node (u_prev updated). */
 int lbfem_diagnostics(const lbfem_disc *disc, const double *f, const double *u_exact, double *u_prev, double *sums,
                       void *stream);
+
+/* ---------------------------------------------------------- measurement
+ * No counterpart in the reference: a STREAM-like device copy of `bytes`
+ * (a multiple of 16, 16-byte aligned pointers) on the stream, the measured
+ * HBM bandwidth a step is compared with and the known-size kernel that
+ * calibrates the profiler's byte counters. */
+int lbfem_hbm_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

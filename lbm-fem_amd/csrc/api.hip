@@ -6,8 +6,11 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -161,6 +164,7 @@ struct lbfem_scheme
   double                tg3         = 0.;
   bool                  extrapolate = false;
   int                   last_its    = 1;
+  int                   tile_variant = -1; // TileVariant of the tile kernel; -1: tile_default_variant
   // stage timers (include/lbfem/timers.h): collision, advection, mass, total
   bool                                   timing = false;
   std::vector<std::pair<hipEvent_t, int>> marks;
@@ -174,6 +178,12 @@ struct lbfem_scheme
       (void)hipHostFree(host_state);
     if (stream)
       (void)hipStreamDestroy(stream);
+  }
+
+  int
+  variant() const
+  {
+    return tile_variant >= 0 ? tile_variant : tile_default_variant(d->G);
   }
 
   void
@@ -317,7 +327,8 @@ namespace
     if (collide)
       gstar = s.f[1 - s.cur].p;
     s.mark(-1);
-    HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, G, S, fin, nullptr, gstar, s.rhs.p, s.stream));
+    HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, s.variant(), G, S, fin, (const double *)nullptr, gstar,
+                          s.rhs.p, s.stream));
     s.mark(1);
     if (collide)
       s.cur = 1 - s.cur;
@@ -375,6 +386,60 @@ namespace
     return LBFEM_OK;
   }
 
+  // One launch of the tile kernel the scheme's step uses, with variant v, on the
+  // scheme's buffers without changing its state: the fused step writes the
+  // spare population buffer, the advection the right-hand side (and, for
+  // Bardow, g* into the spare buffer).
+  int
+  tile_trial(lbfem_scheme &s, const int v)
+  {
+    const Geo &G  = s.d->G;
+    const bool ll = s.set.scheme == LBFEM_LEELIN;
+    TileParams S  = tile_params(s, s.sweeps[0], ll);
+    if (s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2)
+      {
+        S.collide = 1;
+        if (s.mixed)
+          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, v, G, S, s.g32[s.cur].p, (const float *)nullptr,
+                                s.g32[1 - s.cur].p, (float *)nullptr, s.stream));
+        else
+          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, v, G, S, s.f[s.cur].p, (const double *)nullptr,
+                                s.f[1 - s.cur].p, (double *)nullptr, s.stream));
+      }
+    else
+      {
+        S.collide = ll ? 0 : 1;
+        HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, v, G, S, s.f[s.cur].p, (const double *)nullptr,
+                              ll ? nullptr : s.f[1 - s.cur].p, s.rhs.p, s.stream));
+      }
+    return LBFEM_OK;
+  }
+
+  struct TuneKey
+  {
+    int  mode, p, P, px, py;
+    bool operator<(const TuneKey &o) const
+    {
+      return std::tie(mode, p, P, px, py) < std::tie(o.mode, o.p, o.P, o.px, o.py);
+    }
+  };
+
+  std::map<TuneKey, int> &
+  tune_cache()
+  {
+    static std::map<TuneKey, int> cache;
+    return cache;
+  }
+
+  TuneKey
+  tune_key(const lbfem_scheme &s)
+  {
+    const Geo &G     = s.d->G;
+    const bool fused = s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2;
+    const int  mode  = (fused ? 0 : 2) + (s.set.scheme == LBFEM_LEELIN ? 1 : 0) + (s.mixed ? 4 : 0);
+    return TuneKey{mode, G.p, G.P, G.ax.periodic, G.ay.periodic};
+  }
+
   int
   step_once(lbfem_scheme &s)
   {
@@ -388,11 +453,11 @@ namespace
         S.collide    = 1;
         s.mark(-1);
         if (s.mixed)
-          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, G, S, s.g32[s.cur].p, (const float *)nullptr,
-                                s.g32[1 - s.cur].p, (float *)nullptr, s.stream));
+          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, s.variant(), G, S, s.g32[s.cur].p,
+                                (const float *)nullptr, s.g32[1 - s.cur].p, (float *)nullptr, s.stream));
         else
-          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, G, S, s.f[s.cur].p, nullptr, s.f[1 - s.cur].p, nullptr,
-                                s.stream));
+          HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, s.variant(), G, S, s.f[s.cur].p,
+                                (const double *)nullptr, s.f[1 - s.cur].p, (double *)nullptr, s.stream));
         s.mark(1);
         s.cur = 1 - s.cur;
         return LBFEM_OK;
@@ -662,8 +727,8 @@ extern "C"
       TileParams S       = tile_params(s, ee, kind == LBFEM_LEELIN);
       S.collide          = 0;
       S.remove_mass_flux = remove_mass_flux ? 1 : 0;
-      HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, d->G, S, f,
-                            kind == LBFEM_LEELIN ? feq : nullptr, nullptr, rhs, as_stream(stream)));
+      HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, tile_default_variant(d->G), d->G, S,
+                            f, kind == LBFEM_LEELIN ? feq : nullptr, (double *)nullptr, rhs, as_stream(stream)));
       return LBFEM_OK;
     });
   }
@@ -1078,6 +1143,118 @@ extern "C"
       HIP_CHECK(diagnostics_launch(d->G, f, u_exact, u_prev, dsum.p, d->partial.p, d->counter.p, kMaxCgBlocks, st));
       HIP_CHECK(hipMemcpyAsync(sums, dsum.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipStreamSynchronize(st));
+      return LBFEM_OK;
+    });
+  }
+  int
+  lbfem_scheme_get_stream(const lbfem_scheme *s, void **stream)
+  {
+    return guarded([&]() -> int {
+      if (!s || !stream)
+        return fail(LBFEM_EINVAL, "null argument");
+      *stream = static_cast<void *>(s->stream);
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_set_tile_variant(lbfem_scheme *s, const int variant)
+  {
+    return guarded([&]() -> int {
+      if (!s)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (variant < -1 || variant >= kTileVariants)
+        return fail(LBFEM_EINVAL, "tile variant out of range");
+      s->tile_variant = variant;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_get_tile_variant(const lbfem_scheme *s, int *variant)
+  {
+    return guarded([&]() -> int {
+      if (!s || !variant)
+        return fail(LBFEM_EINVAL, "null argument");
+      *variant = s->variant();
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_scheme_autotune(lbfem_scheme *s, const int force, int *variant, double *seconds)
+  {
+    return guarded([&]() -> int {
+      if (!s)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (!(s->dt > 0.))
+        return fail(LBFEM_EINVAL, "set the time step before tuning");
+      const TuneKey key = tune_key(*s);
+      auto         &cache = tune_cache();
+      auto          hit   = cache.find(key);
+      if (seconds)
+        for (int v = 0; v < kTileVariants; ++v)
+          seconds[v] = -1.;
+      if (hit != cache.end() && !force)
+        s->tile_variant = hit->second;
+      else
+        {
+          hipEvent_t e0, e1;
+          HIP_CHECK(hipEventCreate(&e0));
+          HIP_CHECK(hipEventCreate(&e1));
+          double best = 1e300;
+          int    pick = tile_default_variant(s->d->G);
+          for (int v = 0; v < kTileVariants; ++v)
+            {
+              // two launches to load the code, three to size the sample, then about 2 ms of launches
+              for (int k = 0; k < 2; ++k)
+                if (const int st = tile_trial(*s, v); st != LBFEM_OK)
+                  return st;
+              float ms = 0.f;
+              HIP_CHECK(hipEventRecord(e0, s->stream));
+              for (int k = 0; k < 3; ++k)
+                if (const int st = tile_trial(*s, v); st != LBFEM_OK)
+                  return st;
+              HIP_CHECK(hipEventRecord(e1, s->stream));
+              HIP_CHECK(hipEventSynchronize(e1));
+              HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+              const int n = std::max(5, std::min(200, int(2.0 / std::max(1e-4, double(ms) / 3))));
+              HIP_CHECK(hipEventRecord(e0, s->stream));
+              for (int k = 0; k < n; ++k)
+                if (const int st = tile_trial(*s, v); st != LBFEM_OK)
+                  return st;
+              HIP_CHECK(hipEventRecord(e1, s->stream));
+              HIP_CHECK(hipEventSynchronize(e1));
+              HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+              const double t = 1e-3 * ms / n;
+              if (seconds)
+                seconds[v] = t;
+              if (t < best)
+                {
+                  best = t;
+                  pick = v;
+                }
+            }
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          cache[key]      = pick;
+          s->tile_variant = pick;
+        }
+      if (variant)
+        *variant = s->tile_variant;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_hbm_copy(void *dst, const void *src, size_t bytes, void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!dst || !src)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (bytes % 16 != 0 || reinterpret_cast<uintptr_t>(dst) % 16 != 0 || reinterpret_cast<uintptr_t>(src) % 16 != 0)
+        return fail(LBFEM_EINVAL, "lbfem_hbm_copy: size and addresses must be multiples of 16 bytes");
+      HIP_CHECK(stream_copy_launch(dst, src, bytes, as_stream(stream)));
       return LBFEM_OK;
     });
   }

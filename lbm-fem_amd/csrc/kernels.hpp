@@ -27,11 +27,61 @@ namespace lbfem_amd
     int    tiles_x, tiles_y;  // set by the launcher
   };
 
-  hipError_t tile_launch(int mode, const Geo &G, const TileParams &S, const double *fin, const double *feq,
-                         double *out, double *rhs, hipStream_t stream);
-  // the mixed form: FP32 deviations from the rest state
-  hipError_t tile_launch(int mode, const Geo &G, const TileParams &S, const float *fin, const float *feq, float *out,
-                         float *rhs, hipStream_t stream);
+  // shapes of the tile kernel (all give bitwise identical results)
+  enum TileVariant : int
+  {
+    kTileWide     = 0, // 32 x 8 or 32 x 16 nodes, 256 threads, all populations of a node in one thread
+    kTileSplit2   = 1, // 32 x 8 nodes, 512 threads, the populations of a node split over 2 threads
+    kTileSplit4   = 2, // 32 x 8 nodes, 1024 threads, split over 4
+    kTileSmall4   = 3, // 16 x 8 nodes, 512 threads, split over 4
+    kTileVariants = 4
+  };
+
+  template <typename T, int p>
+  hipError_t tile_launch_degree(int mode, int variant, const Geo &G, const TileParams &S, const T *fin, const T *feq,
+                                T *out, T *rhs, hipStream_t stream);
+  extern template hipError_t tile_launch_degree<double, 1>(int, int, const Geo &, const TileParams &, const double *,
+                                                           const double *, double *, double *, hipStream_t);
+  extern template hipError_t tile_launch_degree<double, 2>(int, int, const Geo &, const TileParams &, const double *,
+                                                           const double *, double *, double *, hipStream_t);
+  extern template hipError_t tile_launch_degree<double, 3>(int, int, const Geo &, const TileParams &, const double *,
+                                                           const double *, double *, double *, hipStream_t);
+  extern template hipError_t tile_launch_degree<float, 1>(int, int, const Geo &, const TileParams &, const float *,
+                                                          const float *, float *, float *, hipStream_t);
+  extern template hipError_t tile_launch_degree<float, 2>(int, int, const Geo &, const TileParams &, const float *,
+                                                          const float *, float *, float *, hipStream_t);
+  extern template hipError_t tile_launch_degree<float, 3>(int, int, const Geo &, const TileParams &, const float *,
+                                                          const float *, float *, float *, hipStream_t);
+
+  // One launch of the tile kernel over the mesh. double: every mode; float (the
+  // mixed form's FP32 deviations from the rest state): the fused-step modes.
+  template <typename T>
+  inline hipError_t
+  tile_launch(const int mode, const int variant, const Geo &G, const TileParams &S, const T *fin, const T *feq,
+              T *out, T *rhs, hipStream_t stream)
+  {
+    switch (G.p)
+      {
+        case 1:
+          return tile_launch_degree<T, 1>(mode, variant, G, S, fin, feq, out, rhs, stream);
+        case 2:
+          return tile_launch_degree<T, 2>(mode, variant, G, S, fin, feq, out, rhs, stream);
+        case 3:
+          return tile_launch_degree<T, 3>(mode, variant, G, S, fin, feq, out, rhs, stream);
+        default:
+          return hipErrorInvalidValue;
+      }
+  }
+
+  // The variant used until a measurement picks one: below about four wide-tile
+  // wavefronts per SIMD of the device (256 CUs x 4 SIMDs) a node's populations
+  // are split over four threads, so that the device fills.
+  inline int
+  tile_default_variant(const Geo &G)
+  {
+    const long long nodes = (long long)(G.P + 1) * (G.P + 1);
+    return nodes < 64LL * 4 * 1024 ? kTileSplit4 : kTileWide;
+  }
 
   // nodal operations over every node (collision.h:78-168)
   hipError_t collide_launch(const Geo &G, double *g, double omega, hipStream_t stream);
@@ -89,6 +139,7 @@ namespace lbfem_amd
                                       hipStream_t stream);
 
   // diagnostics sums over non-slave nodes (cgdbe.cc:353-368)
+  hipError_t stream_copy_launch(void *dst, const void *src, size_t bytes, hipStream_t s);
   hipError_t diagnostics_launch(const Geo &G, const double *f, const double *u_exact, double *u_prev, double *sums,
                                 double *partial, unsigned *counter, int max_blocks, hipStream_t stream);
 } // namespace lbfem_amd

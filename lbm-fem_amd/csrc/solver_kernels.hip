@@ -734,4 +734,39 @@ namespace lbfem_amd
     hipLaunchKernelGGL(k_diagnostics_finalize, dim3(1), dim3(kThreads), 0, s, partial, nb, max_blocks, sums);
     return hipGetLastError();
   }
+  // ---------------------------------------------------------------- HBM probe
+  // A plain copy, four 16 B loads in flight per lane per pass over a
+  // grid-strided range: the achievable-bandwidth reference and the known-size
+  // kernel that scales the byte counters of the profiler.
+  __global__ void __launch_bounds__(kThreads)
+  k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, const size_t n)
+  {
+    constexpr int U      = 4;
+    const size_t  stride = size_t(gridDim.x) * kThreads;
+    size_t        i      = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    for (; i + (U - 1) * stride < n; i += U * stride)
+      {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          dst[i + u * stride] = v[u];
+      }
+    for (; i < n; i += stride)
+      dst[i] = src[i];
+  }
+
+  hipError_t
+  stream_copy_launch(void *dst, const void *src, const size_t bytes, hipStream_t s)
+  {
+    const size_t n  = bytes / sizeof(double2);
+    const size_t nb = std::min<size_t>((n + kThreads - 1) / kThreads, 8192);
+    if (nb == 0)
+      return hipSuccess;
+    hipLaunchKernelGGL(k_stream_copy, dim3(unsigned(nb)), dim3(kThreads), 0, s, static_cast<const double2 *>(src),
+                       static_cast<double2 *>(dst), n);
+    return hipGetLastError();
+  }
 } // namespace lbfem_amd

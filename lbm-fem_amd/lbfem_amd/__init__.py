@@ -20,6 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "liblbfem_hip.so")
 
 OK, EINVAL, ENOCONV, EHIP = 0, 1, 2, 3
+TILE_WIDE, TILE_SPLIT2, TILE_SPLIT4, TILE_SMALL4 = 0, 1, 2, 3
+TILE_VARIANTS = 4
 LEELIN, BARDOW = 0, 1
 TG2, TG3, TG3_SPLIT = 0, 1, 2
 MASS_CG, MASS_LUMPED, MASS_RICHARDSON = 0, 1, 2
@@ -102,6 +104,11 @@ def lib():
             "lbfem_scheme_enable_stage_timers": (i, [_vp, i]),
             "lbfem_scheme_stage_times": (i, [_vp, p]),
             "lbfem_diagnostics": (i, [_vp, _vp, _vp, _vp, p, _vp]),
+            "lbfem_hbm_copy": (i, [_vp, _vp, C.c_size_t, _vp]),
+            "lbfem_scheme_set_tile_variant": (i, [_vp, i]),
+            "lbfem_scheme_get_stream": (i, [_vp, C.POINTER(_vp)]),
+            "lbfem_scheme_get_tile_variant": (i, [_vp, C.POINTER(C.c_int)]),
+            "lbfem_scheme_autotune": (i, [_vp, i, C.POINTER(C.c_int), p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -328,6 +335,30 @@ class Scheme:
         check(lib().lbfem_scheme_stage_times(self.h, _p(t)))
         return {"collision": t[0], "advection": t[1], "mass": t[2], "total": t[3]}
 
+    @property
+    def stream_handle(self):
+        """The scheme's hipStream_t as an int (torch.cuda.ExternalStream takes it)."""
+        h = _vp()
+        check(lib().lbfem_scheme_get_stream(self.h, C.byref(h)))
+        return h.value or 0
+
+    @property
+    def tile_variant(self):
+        v = C.c_int(0)
+        check(lib().lbfem_scheme_get_tile_variant(self.h, C.byref(v)))
+        return v.value
+
+    @tile_variant.setter
+    def tile_variant(self, v):
+        check(lib().lbfem_scheme_set_tile_variant(self.h, int(v)))
+
+    def autotune(self, force=False):
+        """Times the tile kernel's variants on this scheme, keeps the fastest; returns (variant, seconds)."""
+        v = C.c_int(0)
+        t = np.zeros(TILE_VARIANTS)
+        check(lib().lbfem_scheme_autotune(self.h, int(force), C.byref(v), _p(t)))
+        return v.value, t
+
 
 class Bardow(Scheme):
     """include/lbfem/schemes.h:104-142."""
@@ -345,3 +376,11 @@ def diagnostics(disc, f_dev_ptr, u_exact=None, u_prev=None, stream=None):
     check(lib().lbfem_diagnostics(disc.h, C.c_void_p(f_dev_ptr), _ptr(u_exact), _ptr(u_prev), _p(s),
                                   _stream(stream)))
     return s
+
+
+def hbm_copy(dst, src, stream=None):
+    """lbfem_hbm_copy: device copy of src into dst (torch tensors of equal byte size, on the GPU)."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nbytes:
+        raise LbfemError(EINVAL, "hbm_copy: tensors differ in size")
+    check(lib().lbfem_hbm_copy(C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()), nbytes, _stream(stream)))
