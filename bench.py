@@ -26,8 +26,8 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 Q = 9
 # kernel names of the tile variants (Cfg<T, p, TX, TY, RO, PS, mode>), fused Lee-Lin step, FP64, Q1
-TILE_KERNEL = {0: "k_tile<Cfg<double, 1, 32, 8, 1, 1, 1> >", 1: "k_tile<Cfg<double, 1, 32, 8, 1, 2, 1> >",
-               2: "k_tile<Cfg<double, 1, 32, 8, 1, 4, 1> >", 3: "k_tile<Cfg<double, 1, 16, 8, 1, 4, 1> >"}
+TILE_KERNEL = {v: f"k_tile<Cfg<double, 1, {tx}, {ty}, {ps}, 1> >"
+               for v, (tx, ty, ps) in enumerate(((32, 8, 1), (32, 16, 1), (32, 8, 2), (32, 8, 4), (16, 8, 4)))}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_tile_cavity_q1_256.txt")
 
 

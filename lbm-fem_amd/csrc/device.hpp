@@ -94,8 +94,9 @@ namespace lbfem_amd
     rho           = ((((((((f[0] + f[1]) + f[2]) + f[3]) + f[4]) + f[5]) + f[6]) + f[7]) + f[8]);
     const double mx = ((((f[1] + f[2]) - f[4]) - f[5]) - f[6]) + f[8];
     const double my = ((((f[2] + f[3]) + f[4]) - f[6]) - f[7]) - f[8];
-    ux            = mx / rho;
-    uy            = my / rho;
+    const double inv = 1. / rho;
+    ux               = mx * inv;
+    uy               = my * inv;
   }
 
   // e_a . u for the D2Q9 directions
@@ -204,9 +205,10 @@ namespace lbfem_amd
             double(d[8]));
     const double mx = ((((double(d[1]) + double(d[2])) - double(d[4])) - double(d[5])) - double(d[6])) + double(d[8]);
     const double my = ((((double(d[2]) + double(d[3])) + double(d[4])) - double(d[6])) - double(d[7])) - double(d[8]);
-    rho             = 1. + drho;
-    ux              = mx / rho;
-    uy              = my / rho;
+    rho              = 1. + drho;
+    const double inv = 1. / rho;
+    ux               = mx * inv;
+    uy               = my * inv;
   }
 
   // The deviation of the equilibrium, w (rho - rho0) + w rho (3 e.u + 4.5 (e.u)^2 - 1.5 u^2),
@@ -299,9 +301,9 @@ namespace lbfem_amd
     double drho = ((((((((fh[0] + fh[1]) + fh[2]) + fh[3]) + fh[4]) + fh[5]) + fh[6]) + fh[7]) + fh[8]);
     double ux   = (((((fh[1] + fh[2]) - fh[4]) - fh[5]) - fh[6]) + fh[8]);
     double uy   = (((((fh[2] + fh[3]) + fh[4]) - fh[6]) - fh[7]) - fh[8]);
-    const double rho = 1. + drho;
-    ux /= rho;
-    uy /= rho;
+    const double rho = 1. + drho, ir = 1. / rho;
+    ux *= ir;
+    uy *= ir;
     if (wall)
       {
         ux = wx;

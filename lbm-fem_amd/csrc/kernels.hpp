@@ -30,11 +30,12 @@ namespace lbfem_amd
   // shapes of the tile kernel (all give bitwise identical results)
   enum TileVariant : int
   {
-    kTileWide     = 0, // 32 x 8 or 32 x 16 nodes, 256 threads, all populations of a node in one thread
-    kTileSplit2   = 1, // 32 x 8 nodes, 512 threads, the populations of a node split over 2 threads
-    kTileSplit4   = 2, // 32 x 8 nodes, 1024 threads, split over 4
-    kTileSmall4   = 3, // 16 x 8 nodes, 512 threads, split over 4
-    kTileVariants = 4
+    kTileWide     = 0, // 32 x 8 nodes, 256 threads, all populations of a node in one thread
+    kTileTall     = 1, // 32 x 16 nodes, 512 threads (less halo per node)
+    kTileSplit2   = 2, // 32 x 8 nodes, 512 threads, the populations of a node split over 2 threads
+    kTileSplit4   = 3, // 32 x 8 nodes, 1024 threads, split over 4
+    kTileSmall4   = 4, // 16 x 8 nodes, 512 threads, split over 4
+    kTileVariants = 5
   };
 
   template <typename T, int p>

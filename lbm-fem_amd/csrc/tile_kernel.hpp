@@ -8,10 +8,11 @@
 // of p nodes (the reach of a Q_p row) for all nine populations once, does the
 // nodal prologue, and stages the results in LDS: g* (Bardow), or f and
 // h = a f + b feq (Lee-Lin). The operator of each population is then applied
-// at the owned nodes straight from LDS, as Kronecker products of the assembled
-// 1-D bands: a thread owns RO consecutive nodes of one column, sums the y bands
-// over a strip of RO + 2p rows of each of the 2p + 1 neighbouring columns
-// (shared between its rows), and contracts with its column's x bands.
+// at the owned nodes straight from LDS. On Q1 meshes a thread first forms the
+// 3 x 3 weights of its node for the five parts of the operator (Kronecker
+// products of its assembled 1-D bands, the scheme coefficients folded in) and
+// applies them to each population; on Q2 and Q3 it contracts the y bands over
+// the 2p + 1 neighbouring columns and then the x bands of its column.
 //
 // With PS > 1 the eight populations of a node are split over PS threads (PS
 // wavefronts of the block hold the same nodes, each wavefront a fixed set of
@@ -19,11 +20,12 @@
 // the nodal epilogue. Small meshes then fill the device with PS times as many
 // wavefronts, each with a PS times shorter chain of dependent operations.
 //
-// Every variant computes every output value by the same operation sequence
-// (the wall mass flux is summed by pairs of populations, then over the pairs,
-// in every variant), so that all variants give bitwise identical results:
-// deterministic, translation invariant, no scatter, no atomics; the periodic
-// seam is a wrap of the halo.
+// The prologue loads a thread's own node first, so that the Lee-Lin epilogue
+// reuses that node's equilibria. Every variant computes every output value by
+// the same operation sequence (the wall mass flux is summed by pairs of
+// populations, then over the pairs), so that all variants give bitwise
+// identical results: deterministic, translation invariant, no scatter, no
+// atomics; the periodic seam is a wrap of the halo.
 #pragma once
 
 #include "device.hpp"
@@ -41,34 +43,38 @@ namespace lbfem_amd
       return B[i * (2 * p + 1) + o + p];
     }
 
-    // global node line for a tile position, with the periodic wrap; -1 outside
+    // global node line of a tile position, -1 outside a non-periodic axis. A
+    // periodic position lies in [-p, P + TX + p): positions past P + p feed no
+    // owned node and map to any line inside.
     __device__ __forceinline__ int
     wrap(const int i, const int P, const int periodic)
     {
       if (periodic)
-        return ((i % P) + P) % P;
+        {
+          const int w = i < 0 ? i + P : (i >= P ? i - P : i);
+          return w < P ? w : P - 1;
+        }
       return (i < 0 || i > P) ? -1 : i;
     }
 
     // A tile variant: T the storage and arithmetic type of the populations
     // (double for the FP64 form, float for the mixed form's deviations), Q_p,
-    // tile TX x TY, RO owned rows per thread, PS threads per node.
-    template <typename T_, int p_, int TX_, int TY_, int RO_, int PS_, int MODE_>
+    // tile TX x TY (one node per thread and part), PS threads per node.
+    template <typename T_, int p_, int TX_, int TY_, int PS_, int MODE_>
     struct Cfg
     {
       using T                    = T_;
-      static constexpr int  p    = p_, TX = TX_, TY = TY_, RO = RO_, PS = PS_, MODE = MODE_;
+      static constexpr int  p    = p_, TX = TX_, TY = TY_, PS = PS_, MODE = MODE_;
       static constexpr int  W    = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
-      static constexpr int  TPP  = TX * TY / RO; // threads per population part
-      static constexpr int  NT   = TPP * PS;     // threads of the block
-      static constexpr int  PPP  = NM / PS;      // populations per part
+      static constexpr int  TPP  = TX * TY; // threads per population part
+      static constexpr int  NT   = TPP * PS; // threads of the block
+      static constexpr int  PPP  = NM / PS;  // populations per part
       static constexpr bool LL   = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
       static constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
-      static constexpr int  NF   = LL ? Q + NM : Q; // f (or g*) and, for Lee-Lin, h of the moving populations
-      static constexpr int  R    = (NH + NT - 1) / NT; // prologue rounds
-      static_assert(TY % RO == 0 && TPP % TX == 0, "tile shape");
+      static constexpr int  NF   = LL ? Q + NM : Q;      // f (or g*) and, for Lee-Lin, h of the moving populations
+      static constexpr int  R    = (NH + NT - 1) / NT;   // prologue rounds
+      static constexpr bool WGT  = p == 1;               // the operator as 3 x 3 node weights
       static_assert(TPP % 64 == 0 && NT <= 1024, "whole wavefronts, at most 1024 threads");
-      static_assert(PS == 1 || RO == 1, "population split with one row per thread");
       static_assert(NM % PS == 0 && PPP % 2 == 0, "parts of whole population pairs");
     };
 
@@ -80,6 +86,38 @@ namespace lbfem_amd
       typename C::T MP[C::PS > 1 ? NM / 2 : 1][C::TY][C::TX]; // wall mass flux of population pairs
     };
 
+    // Halo-tile position of prologue slot k: the TX x TY owned nodes first (slot
+    // k is the node of thread k of part 0), then the ring of width p.
+    template <class C>
+    __device__ __forceinline__ void
+    slot_position(const int k, int &kx, int &ky)
+    {
+      constexpr int p = C::p, TX = C::TX, TY = C::TY, HX = C::HX, HY = C::HY;
+      if (k < TX * TY)
+        {
+          kx = p + k % TX;
+          ky = p + k / TX;
+          return;
+        }
+      const int m = k - TX * TY;
+      if (m < p * HX)
+        {
+          kx = m % HX;
+          ky = m / HX;
+        }
+      else if (m < 2 * p * HX)
+        {
+          kx = (m - p * HX) % HX;
+          ky = HY - p + (m - p * HX) / HX;
+        }
+      else
+        {
+          const int m3 = m - 2 * p * HX, c = m3 % (2 * p);
+          kx           = c < p ? c : TX + c;
+          ky           = p + m3 / (2 * p);
+        }
+    }
+
     // bands: 0 M, 1 D, 2 K, 3 D^T
     template <class C>
     __device__ __forceinline__ void
@@ -88,25 +126,30 @@ namespace lbfem_amd
          typename C::T *__restrict__ rhs_out, Smem<C> &sm)
     {
       using T                   = typename C::T;
-      constexpr int  p          = C::p, TX = C::TX, TY = C::TY, RO = C::RO, PS = C::PS;
+      constexpr int  p          = C::p, TX = C::TX, PS = C::PS;
       constexpr int  W          = C::W, HX = C::HX, HY = C::HY, NH = C::NH, NT = C::NT, R = C::R, NF = C::NF;
-      constexpr bool LL         = C::LL, STEP = C::STEP;
+      constexpr bool LL         = C::LL, STEP = C::STEP, WGT = C::WGT;
+      // the mixed form applies the operator to f - f(node): every term annihilates
+      // constants (its bands' row or column sums vanish), and FP32 then carries the
+      // small differences instead of cancelling large values
+      constexpr bool CEN        = sizeof(T) == 4;
       T(&F)[NF][HY][HX]         = sm.F;
-      const int i0 = (tile % S.tiles_x) * TX, j0 = (tile / S.tiles_x) * TY;
+      const int i0 = (tile % S.tiles_x) * TX, j0 = (tile / S.tiles_x) * C::TY;
       const int P = G.P, tid = threadIdx.x;
       const T   a_c = T(S.a), b_c = T(S.b), c_c = T(S.c), hdt2 = T(0.5 * S.dt * S.dt);
+      const T   sa = LL ? T(1) : a_c; // Bardow's a in the first-order weights, Lee-Lin's in h
 
       // ---------------------------------------------------------- prologue
       // all loads of the halo'd block first (R rounds in flight), then the nodal work
-      T             v[R][Q];
-      int           gis[R], gjs[R];
+      T   v[R][Q];
+      int gis[R], gjs[R], kxs[R], kys[R];
 #pragma unroll
       for (int r = 0; r < R; ++r)
         {
-          const int k  = tid + r * NT;
-          const int kx = k % HX, ky = k / HX;
-          gis[r]       = k < NH ? wrap(i0 - p + kx, P, G.ax.periodic) : -1;
-          gjs[r]       = k < NH ? wrap(j0 - p + ky, P, G.ay.periodic) : -1;
+          const int k = tid + r * NT;
+          slot_position<C>(k, kxs[r], kys[r]);
+          gis[r] = k < NH ? wrap(i0 - p + kxs[r], P, G.ax.periodic) : -1;
+          gjs[r] = k < NH ? wrap(j0 - p + kys[r], P, G.ay.periodic) : -1;
           if (gis[r] >= 0 && gjs[r] >= 0)
             {
               const int64_t idx = node_index(G, gis[r], gjs[r]);
@@ -117,15 +160,16 @@ namespace lbfem_amd
           else
 #pragma unroll
             for (int a = 0; a < Q; ++a)
-              v[r][a] = 0.;
+              v[r][a] = T(0);
         }
+      T fe_own[Q]; // Lee-Lin: the equilibria of slot tid, the own node of part 0 threads
 #pragma unroll
       for (int r = 0; r < R; ++r)
         {
           const int k = tid + r * NT;
           if (k >= NH)
             continue;
-          const int  kx = k % HX, ky = k / HX;
+          const int  kx = kxs[r], ky = kys[r];
           const bool ok = gis[r] >= 0 && gjs[r] >= 0;
           double     wx = 0., wy = 0.;
           const bool wall = ok && wall_of(G, gis[r], gjs[r], wx, wy);
@@ -145,6 +189,10 @@ namespace lbfem_amd
 #pragma unroll
               for (int a = 1; a < Q; ++a)
                 F[Q + a - 1][ky][kx] = a_c * v[r][a] + b_c * fe[a];
+              if (r == 0)
+#pragma unroll
+                for (int a = 0; a < Q; ++a)
+                  fe_own[a] = fe[a];
             }
           else if (S.collide && ok)
             collide_node(v[r], S.omega, wall, wx, wy);
@@ -152,138 +200,176 @@ namespace lbfem_amd
           for (int a = 0; a < Q; ++a)
             F[a][ky][kx] = v[r][a];
         }
-      // this thread's column and rows; the x bands of the column and the y bands of
-      // the rows in registers (bands: 0 M, 1 D, 2 K, 3 D^T)
+      // this thread's node; the x bands of its column and the y bands of its row
+      // (bands: 0 M, 1 D, 2 K, 3 D^T)
       // population part of this thread, uniform over its wavefront
       const int q  = PS > 1 ? __builtin_amdgcn_readfirstlane(tid / C::TPP) : 0;
       const int lt = PS > 1 ? tid % C::TPP : tid;
-      const int ii = lt % TX, jr = (lt / TX) * RO;
-      const int gi = i0 + ii;
-      T         xc[4][W], yc[RO][4][W];
+      const int ii = lt % TX, jj = lt / TX;
+      const int gi = i0 + ii, gj = j0 + jj;
+      T         xc[4][W], yc[4][W];
 #pragma unroll
       for (int o = 0; o < W; ++o)
         {
-          const bool in = gi <= P;
+          const bool in = gi <= P, iy = gj <= P;
           xc[0][o]      = in ? T(band<p>(G.ax.M, gi, o - p)) : T(0);
           xc[1][o]      = in ? T(band<p>(G.ax.D, gi, o - p)) : T(0);
           xc[2][o]      = in ? T(band<p>(G.ax.K, gi, o - p)) : T(0);
           xc[3][o]      = in ? T(band<p>(G.ax.DT, gi, o - p)) : T(0);
+          yc[0][o]      = iy ? T(band<p>(G.ay.M, gj, o - p)) : T(0);
+          yc[1][o]      = iy ? T(band<p>(G.ay.D, gj, o - p)) : T(0);
+          yc[2][o]      = iy ? T(band<p>(G.ay.K, gj, o - p)) : T(0);
+          yc[3][o]      = iy ? T(band<p>(G.ay.DT, gj, o - p)) : T(0);
+        }
+      // Q1: the node's weights of the first-order terms (sa D_x (x) M_y, sa M_x (x) D_y) and
+      // of the second-order terms (c K_x (x) M_y, c M_x (x) K_y, c (D_x (x) D_y^T + D_x^T (x) D_y)),
+      // [x offset][y offset], in two phases so that at most three sets are live: phase A
+      // the x-class populations whole and the diagonal ones' x and cross parts, phase B
+      // the y-class populations and the diagonal ones' y parts
+      constexpr int WW = WGT ? W : 1;
+      T             wgx[WW][WW], wkx[WW][WW], wbb[WW][WW], xm[WW], yd[WW], yk[WW];
+      if constexpr (WGT)
+        {
 #pragma unroll
-          for (int r = 0; r < RO; ++r)
+          for (int c = 0; c < W; ++c)
             {
-              const int  gj = j0 + jr + r;
-              const bool iy = gj <= P;
-              yc[r][0][o]   = iy ? T(band<p>(G.ay.M, gj, o - p)) : T(0);
-              yc[r][1][o]   = iy ? T(band<p>(G.ay.D, gj, o - p)) : T(0);
-              yc[r][2][o]   = iy ? T(band<p>(G.ay.K, gj, o - p)) : T(0);
-              yc[r][3][o]   = iy ? T(band<p>(G.ay.DT, gj, o - p)) : T(0);
+              const T xd = sa * xc[1][c], xk = c_c * xc[2][c], xcd = c_c * xc[1][c], xct = c_c * xc[3][c];
+#pragma unroll
+              for (int o = 0; o < W; ++o)
+                {
+                  wgx[c][o] = xd * yc[0][o];
+                  wkx[c][o] = xk * yc[0][o];
+                  wbb[c][o] = xcd * yc[3][o] + xct * yc[1][o];
+                }
+              xm[c] = xc[0][c];
+              yd[c] = sa * yc[1][c];
+              yk[c] = c_c * yc[2][c];
             }
         }
       __syncthreads();
 
       // ---------------------------------------------------------- populations 1..8
-      // res: the operator's value at the owned rows, fc: the wall surface term
-      auto pop = [&](const int al, T(&res)[RO], T(&fc)[RO]) {
-        // directions as uniform run-time values: compile-time D2Q9 constants let the
-        // scheduler interleave the barrier-free populations and spill registers
-        const T ex = T(S.e[al][0]), ey = T(S.e[al][1]);
+      // r = ex sa Dx My h + ey sa Mx Dy h + c (ex^2 Kx My + ey^2 Mx Ky + ex ey (Dx Dy^T + Dx^T Dy)) f,
+      // h = f for Bardow; one branch-free body per direction class. Directions are uniform
+      // run-time values: compile-time D2Q9 constants let the scheduler interleave the
+      // barrier-free populations and spill registers. A population at rest in the sweep
+      // gets r = 0 (streaming.h:118-122).
+      auto popA = [&](const int al, T &res) {
+        const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
         const bool nx = ex != T(0), ny = ey != T(0);
-        if (!nx && !ny)
-          {
-            // at rest in this sweep: r = 0 (streaming.h:118-122)
-#pragma unroll
-            for (int r = 0; r < RO; ++r)
-              res[r] = fc[r] = T(0);
-            return;
-          }
+        res           = T(0);
         const T(*Ff)[HX] = F[al];
         const T(*Fh)[HX] = F[LL ? Q + al - 1 : al];
-        T acc[RO];
-#pragma unroll
-        for (int r = 0; r < RO; ++r)
-          acc[r] = 0.;
-        // r = Dx (ex My h + c ex ey Dy^T f) + Kx (c ex^2 My f) + Mx (ey Dy h + c ey^2 Ky f) + Dx^T (c ex ey Dy f),
-        // h = f for Bardow (its a folded into the x weights); one branch-free body per direction type
-        const T sa = LL ? T(1) : a_c;
-        // the mixed form applies the operator to f - f(node): every term annihilates
-        // constants (its bands' row or column sums vanish), and FP32 then carries the
-        // small differences instead of cancelling large values
-        constexpr bool CEN = sizeof(T) == 4;
-        T              cf[RO], ch[RO];
-#pragma unroll
-        for (int r = 0; r < RO; ++r)
+        const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
+        const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
+        if constexpr (WGT)
           {
-            cf[r] = CEN ? Ff[jr + r + p][ii + p] : T(0);
-            ch[r] = (CEN && LL) ? Fh[jr + r + p][ii + p] : T(0);
-          }
-        if (nx && !ny)
-          {
-            const T wd = ex * sa, wk = c_c * ex * ex;
-#pragma unroll
-            for (int c = 0; c < W; ++c)
+            if (!nx)
+              return; // y class: phase B
+            T hx = 0, fx = 0, fb = 0;
+            if (!ny)
               {
-                const T kd = xc[1][c] * wd, kk = xc[2][c] * wk;
 #pragma unroll
-                for (int r = 0; r < RO; ++r)
+                for (int c = 0; c < W; ++c)
+#pragma unroll
+                  for (int o = 0; o < W; ++o)
+                    {
+                      if (CEN && c == p && o == p)
+                        continue; // f - f(node) vanishes there
+                      const T fv = Ff[jj + o][ii + c] - cf;
+                      const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                      hx += wgx[c][o] * hv;
+                      fx += wkx[c][o] * fv;
+                    }
+                res = ex * hx + (ex * ex) * fx;
+              }
+            else
+              {
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+#pragma unroll
+                  for (int o = 0; o < W; ++o)
+                    {
+                      if (CEN && c == p && o == p)
+                        continue;
+                      const T fv = Ff[jj + o][ii + c] - cf;
+                      const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                      hx += wgx[c][o] * hv;
+                      fx += wkx[c][o] * fv;
+                      fb += wbb[c][o] * fv;
+                    }
+                res = (ex * hx + (ex * ex) * fx) + (ex * ey) * fb;
+              }
+          }
+        else
+          {
+            if (!nx && !ny)
+              return;
+            // Q2, Q3: y bands over the columns' strips, then the column's x bands
+            T acc = 0;
+            if (nx && !ny)
+              {
+                const T wd = ex * sa, wk = c_c * ex * ex;
+#pragma unroll
+                for (int c = 0; c < W; ++c)
                   {
                     T My = 0, Myh = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        My += yc[r][0][o] * (Ff[jr + r + o][ii + c] - cf[r]);
+                        My += yc[0][o] * (Ff[jj + o][ii + c] - cf);
                         if constexpr (LL)
-                          Myh += yc[r][0][o] * (Fh[jr + r + o][ii + c] - ch[r]);
+                          Myh += yc[0][o] * (Fh[jj + o][ii + c] - ch);
                       }
                     if constexpr (!LL)
                       Myh = My;
-                    acc[r] += kd * Myh + kk * My;
+                    acc += (xc[1][c] * wd) * Myh + (xc[2][c] * wk) * My;
                   }
               }
-          }
-        else if (!nx && ny)
-          {
-            const T wd = ey * sa, wk = c_c * ey * ey;
-#pragma unroll
-            for (int c = 0; c < W; ++c)
+            else if (!nx && ny)
               {
+                const T wd = ey * sa, wk = c_c * ey * ey;
+                T       y1[W], y2[W];
 #pragma unroll
-                for (int r = 0; r < RO; ++r)
+                for (int o = 0; o < W; ++o)
+                  {
+                    y1[o] = yc[1][o] * wd;
+                    y2[o] = yc[2][o] * wk;
+                  }
+#pragma unroll
+                for (int c = 0; c < W; ++c)
                   {
                     T s = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        const T fv = Ff[jr + r + o][ii + c] - cf[r];
-                        const T hv = LL ? Fh[jr + r + o][ii + c] - ch[r] : fv;
-                        s += (yc[r][1][o] * wd) * hv + (yc[r][2][o] * wk) * fv;
+                        const T fv = Ff[jj + o][ii + c] - cf;
+                        const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                        s += y1[o] * hv + y2[o] * fv;
                       }
-                    acc[r] += xc[0][c] * s;
+                    acc += xc[0][c] * s;
                   }
               }
-          }
-        else
-          {
-            const T wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
-#pragma unroll
-            for (int c = 0; c < W; ++c)
+            else
               {
+                const T wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
 #pragma unroll
-                for (int r = 0; r < RO; ++r)
+                for (int c = 0; c < W; ++c)
                   {
                     T My = 0, Dy = 0, Ky = 0, DTy = 0, Myh = 0, Dyh = 0;
 #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        const T fv = Ff[jr + r + o][ii + c] - cf[r];
-                        My += yc[r][0][o] * fv;
-                        Dy += yc[r][1][o] * fv;
-                        Ky += yc[r][2][o] * fv;
-                        DTy += yc[r][3][o] * fv;
+                        const T fv = Ff[jj + o][ii + c] - cf;
+                        My += yc[0][o] * fv;
+                        Dy += yc[1][o] * fv;
+                        Ky += yc[2][o] * fv;
+                        DTy += yc[3][o] * fv;
                         if constexpr (LL)
                           {
-                            const T hv = Fh[jr + r + o][ii + c] - ch[r];
-                            Myh += yc[r][0][o] * hv;
-                            Dyh += yc[r][1][o] * hv;
+                            const T hv = Fh[jj + o][ii + c] - ch;
+                            Myh += yc[0][o] * hv;
+                            Dyh += yc[1][o] * hv;
                           }
                       }
                     if constexpr (!LL)
@@ -291,98 +377,157 @@ namespace lbfem_amd
                         Myh = My;
                         Dyh = Dy;
                       }
-                    acc[r] += xc[1][c] * (wdx * Myh + wx * DTy) + xc[2][c] * (wk * My) +
-                              xc[0][c] * (wdy * Dyh + wky * Ky) + xc[3][c] * (wx * Dy);
+                    acc += xc[1][c] * (wdx * Myh + wx * DTy) + xc[2][c] * (wk * My) +
+                           xc[0][c] * (wdy * Dyh + wky * Ky) + xc[3][c] * (wx * Dy);
                   }
               }
-          }
-        // surface term (dt^2/2) oint phi (n.e)(e.grad f) on walls (advection.h:177-199), as exact
-        // 1-D integrals along the wall: Dx of the tangential derivative, Mx of the normal one
-#pragma unroll
-        for (int r = 0; r < RO; ++r)
-          {
-            const int jj = jr + r, gj = j0 + jj;
-            const int ty = jj + p, tx = ii + p;
-            T         face = 0;
-            if (gi <= P && gj <= P)
-              {
-                if (!G.ay.periodic && ny && (gj == 0 || gj == P))
-                  {
-                    const bool    lo = gj == 0;
-                    const double *bn = lo ? G.ay.b0 : G.ay.b1;
-                    const int     r0 = lo ? ty : ty - p;
-                    T             t = 0, nrm = 0;
-#pragma unroll
-                    for (int o = 0; o < W; ++o)
-                      {
-                        T dn = 0;
-#pragma unroll
-                        for (int l = 0; l <= p; ++l)
-                          dn += T(bn[l]) * Ff[r0 + l][ii + o];
-                        t += xc[1][o] * Ff[ty][ii + o];
-                        nrm += xc[0][o] * dn;
-                      }
-                    face += hdt2 * (lo ? -ey : ey) * (ex * t + ey * nrm);
-                  }
-                if (!G.ax.periodic && nx && (gi == 0 || gi == P))
-                  {
-                    const bool    lo = gi == 0;
-                    const double *bn = lo ? G.ax.b0 : G.ax.b1;
-                    const int     c0 = lo ? tx : tx - p;
-                    T             t = 0, nrm = 0;
-#pragma unroll
-                    for (int o = 0; o < W; ++o)
-                      {
-                        T dn = 0;
-#pragma unroll
-                        for (int l = 0; l <= p; ++l)
-                          dn += T(bn[l]) * Ff[jj + o][c0 + l];
-                        t += yc[r][1][o] * Ff[jj + o][tx];
-                        nrm += yc[r][0][o] * dn;
-                      }
-                    face += hdt2 * (lo ? -ex : ex) * (ex * nrm + ey * t);
-                  }
-              }
-            res[r] = acc[r];
-            fc[r]  = face;
+            res = acc;
           }
       };
+      // phase B (Q1 only): adds the y parts with the weights sa M_x (x) D_y, c M_x (x) K_y;
+      // res holds phase A's value
+      T wgy[WW][WW], wky[WW][WW];
+      auto popB = [&](const int al, T &res) {
+        if constexpr (WGT)
+          {
+            const T ey = T(S.e[al][1]);
+            if (ey == T(0))
+              return;
+            const T(*Ff)[HX] = F[al];
+            const T(*Fh)[HX] = F[LL ? Q + al - 1 : al];
+            const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
+            const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
+            T       hy = 0, fy = 0;
+#pragma unroll
+            for (int c = 0; c < W; ++c)
+#pragma unroll
+              for (int o = 0; o < W; ++o)
+                {
+                  if (CEN && c == p && o == p)
+                    continue;
+                  const T fv = Ff[jj + o][ii + c] - cf;
+                  const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                  hy += wgy[c][o] * hv;
+                  fy += wky[c][o] * fv;
+                }
+            res = res + (ey * hy + (ey * ey) * fy);
+          }
+      };
+      auto weights_b = [&]() {
+        if constexpr (WGT)
+#pragma unroll
+          for (int c = 0; c < W; ++c)
+#pragma unroll
+            for (int o = 0; o < W; ++o)
+              {
+                wgy[c][o] = xm[c] * yd[o];
+                wky[c][o] = xm[c] * yk[o];
+              }
+      };
+      // the wall surface term of population al at the owned node
+      auto face = [&](const int al) -> T {
+        const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
+        const bool nx = ex != T(0), ny = ey != T(0);
+        const T(*Ff)[HX] = F[al];
+        T fc             = T(0);
+        // surface term (dt^2/2) oint phi (n.e)(e.grad f) on walls (advection.h:177-199), as exact
+        // 1-D integrals along the wall: Dx of the tangential derivative, Mx of the normal one.
+        // On Q1 the bands are read again here (wall nodes only), so that they need no
+        // registers next to the weights.
+        auto xb = [&](const int b, const int o) -> T {
+          if constexpr (WGT)
+            return T(band<p>(b == 0 ? G.ax.M : G.ax.D, gi, o - p));
+          else
+            return xc[b][o];
+        };
+        auto yb = [&](const int b, const int o) -> T {
+          if constexpr (WGT)
+            return T(band<p>(b == 0 ? G.ay.M : G.ay.D, gj, o - p));
+          else
+            return yc[b][o];
+        };
+        const int ty = jj + p, tx = ii + p;
+        if (gi <= P && gj <= P)
+          {
+            if (!G.ay.periodic && ny && (gj == 0 || gj == P))
+              {
+                const bool    lo = gj == 0;
+                const double *bn = lo ? G.ay.b0 : G.ay.b1;
+                const int     r0 = lo ? ty : ty - p;
+                T             t = 0, nrm = 0;
+#pragma unroll
+                for (int o = 0; o < W; ++o)
+                  {
+                    T dn = 0;
+#pragma unroll
+                    for (int l = 0; l <= p; ++l)
+                      dn += T(bn[l]) * Ff[r0 + l][ii + o];
+                    t += xb(1, o) * Ff[ty][ii + o];
+                    nrm += xb(0, o) * dn;
+                  }
+                fc += hdt2 * (lo ? -ey : ey) * (ex * t + ey * nrm);
+              }
+            if (!G.ax.periodic && nx && (gi == 0 || gi == P))
+              {
+                const bool    lo = gi == 0;
+                const double *bn = lo ? G.ax.b0 : G.ax.b1;
+                const int     c0 = lo ? tx : tx - p;
+                T             t = 0, nrm = 0;
+#pragma unroll
+                for (int o = 0; o < W; ++o)
+                  {
+                    T dn = 0;
+#pragma unroll
+                    for (int l = 0; l <= p; ++l)
+                      dn += T(bn[l]) * Ff[jj + o][c0 + l];
+                    t += yb(1, o) * Ff[jj + o][tx];
+                    nrm += yb(0, o) * dn;
+                  }
+                fc += hdt2 * (lo ? -ex : ex) * (ex * nrm + ey * t);
+              }
+          }
+        return fc;
+      };
 
-      T xinc[RO][NM], mp[RO][NM / 2];
+      T xinc[NM], mp[NM / 2];
       if constexpr (PS == 1)
         {
 #pragma unroll
           for (int al = 1; al < Q; ++al)
-            {
-              T res[RO], fc[RO];
-              pop(al, res, fc);
+            popA(al, xinc[al - 1]);
+          weights_b();
 #pragma unroll
-              for (int r = 0; r < RO; ++r)
-                {
-                  xinc[r][al - 1] = res[r] + fc[r];
-                  if (al & 1)
-                    mp[r][(al - 1) >> 1] = fc[r];
-                  else
-                    mp[r][(al - 1) >> 1] += fc[r];
-                }
+          for (int al = 1; al < Q; ++al)
+            {
+              popB(al, xinc[al - 1]);
+              const T fc   = face(al);
+              xinc[al - 1] = xinc[al - 1] + fc;
+              if (al & 1)
+                mp[(al - 1) >> 1] = fc;
+              else
+                mp[(al - 1) >> 1] += fc;
             }
         }
       else
         {
-          T pair = T(0);
+          T res[C::PPP], pair = T(0);
+#pragma unroll
+          for (int k = 0; k < C::PPP; ++k)
+            popA(q * C::PPP + k + 1, res[k]); // uniform over the wavefront
+          weights_b();
 #pragma unroll
           for (int k = 0; k < C::PPP; ++k)
             {
-              const int al = q * C::PPP + k + 1; // uniform over the wavefront
-              T         res[1], fc[1];
-              pop(al, res, fc);
-              sm.X[al - 1][jr][ii] = res[0] + fc[0];
+              const int al = q * C::PPP + k + 1;
+              popB(al, res[k]);
+              const T fc           = face(al);
+              sm.X[al - 1][jj][ii] = res[k] + fc;
               if ((k & 1) == 0)
-                pair = fc[0];
+                pair = fc;
               else
                 {
-                  pair += fc[0];
-                  sm.MP[(al - 1) >> 1][jr][ii] = pair;
+                  pair += fc;
+                  sm.MP[(al - 1) >> 1][jj][ii] = pair;
                 }
             }
           __syncthreads();
@@ -390,82 +535,71 @@ namespace lbfem_amd
             return;
 #pragma unroll
           for (int a = 0; a < NM; ++a)
-            xinc[0][a] = sm.X[a][jr][ii];
+            xinc[a] = sm.X[a][jj][ii];
 #pragma unroll
           for (int k = 0; k < NM / 2; ++k)
-            mp[0][k] = sm.MP[k][jr][ii];
+            mp[k] = sm.MP[k][jj][ii];
         }
-      T mflux[RO];
-#pragma unroll
-      for (int r = 0; r < RO; ++r)
-        mflux[r] = ((mp[r][0] + mp[r][1]) + mp[r][2]) + mp[r][3];
+      const T mflux = ((mp[0] + mp[1]) + mp[2]) + mp[3];
 
       // ---------------------------------------------------------- epilogue
+      if (gi > P || gj > P)
+        return;
+      const int64_t idx   = node_index(G, gi, gj);
+      const bool    slave = (G.ax.periodic && gi == P) || (G.ay.periodic && gj == P);
+      double        wx = 0., wy = 0.;
+      const bool    wall = wall_of(G, gi, gj, wx, wy);
+      T             own[Q];
+      if (!slave)
 #pragma unroll
-      for (int r = 0; r < RO; ++r)
+        for (int a = 0; a < Q; ++a)
+          own[a] = F[a][jj + p][ii + p];
+      else
         {
-          const int jj = jr + r, gj = j0 + jj;
-          if (gi > P || gj > P)
-            continue;
-          const int64_t idx   = node_index(G, gi, gj);
-          const bool    slave = (G.ax.periodic && gi == P) || (G.ay.periodic && gj == P);
-          T             own[Q];
-          if (!slave)
+          // a slave's own entries (the staged tile holds its master's); only nodal work applies
 #pragma unroll
-            for (int a = 0; a < Q; ++a)
-              own[a] = F[a][jj + p][ii + p];
-          else
+          for (int a = 0; a < Q; ++a)
+            own[a] = fin[a * G.stride + idx];
+          if constexpr (!LL)
+            if (S.collide)
+              collide_node(own, S.omega, wall, wx, wy);
+        }
+      // mass flux of the wall term removed isotropically (advection.h:194-196)
+      if (S.remove_mass_flux)
+#pragma unroll
+        for (int a = 1; a < Q; ++a)
+          xinc[a - 1] -= T(w_of(a) / (1. - w_of(0))) * mflux;
+      if constexpr (STEP)
+        {
+          const T mli = slave ? T(0) : T(G.ax.ml_inv[gi] * G.ay.ml_inv[gj]);
+#pragma unroll
+          for (int a = 0; a < NM; ++a)
+            xinc[a] *= mli;
+          if constexpr (LL)
             {
-              // a slave's own entries (the staged tile holds its master's); only nodal work applies
-#pragma unroll
-              for (int a = 0; a < Q; ++a)
-                own[a] = fin[a * G.stride + idx];
-              if constexpr (!LL)
-                if (S.collide)
-                  {
-                    double     wx = 0., wy = 0.;
-                    const bool wall = wall_of(G, gi, gj, wx, wy);
-                    collide_node(own, S.omega, wall, wx, wy);
-                  }
+              // the own node's equilibria from the prologue (a slave's from its own entries)
+              if (slave)
+                equilibrium_node(own, fe_own, wall, wx, wy);
+              predictor_corrector_node(own, fe_own, xinc, S.theta, wall, wx, wy);
             }
-          // mass flux of the wall term removed isotropically (advection.h:194-196)
-          if (S.remove_mass_flux)
+          else
 #pragma unroll
             for (int a = 1; a < Q; ++a)
-              xinc[r][a - 1] -= T(w_of(a) / (1. - w_of(0))) * mflux[r];
-          if constexpr (STEP)
-            {
-              const T mli = slave ? T(0) : T(G.ax.ml_inv[gi] * G.ay.ml_inv[gj]);
+              own[a] += xinc[a - 1];
 #pragma unroll
-              for (int a = 0; a < NM; ++a)
-                xinc[r][a] *= mli;
-              if constexpr (LL)
-                {
-                  double     wx = 0., wy = 0.;
-                  const bool wall = wall_of(G, gi, gj, wx, wy);
-                  T          fe[Q];
-                  equilibrium_node(own, fe, wall, wx, wy);
-                  predictor_corrector_node(own, fe, xinc[r], S.theta, wall, wx, wy);
-                }
-              else
+          for (int a = 0; a < Q; ++a)
+            out[a * G.stride + idx] = own[a];
+        }
+      else
+        {
+          // right-hand side (constrained entries get nothing) and, fused, g*
 #pragma unroll
-                for (int a = 1; a < Q; ++a)
-                  own[a] += xinc[r][a - 1];
+          for (int a = 0; a < NM; ++a)
+            rhs_out[a * G.stride + idx] = slave ? T(0) : xinc[a];
+          if (out != nullptr)
 #pragma unroll
-              for (int a = 0; a < Q; ++a)
-                out[a * G.stride + idx] = own[a];
-            }
-          else
-            {
-              // right-hand side (constrained entries get nothing) and, fused, g*
-#pragma unroll
-              for (int a = 0; a < NM; ++a)
-                rhs_out[a * G.stride + idx] = slave ? T(0) : xinc[r][a];
-              if (out != nullptr)
-#pragma unroll
-                for (int a = 0; a < Q; ++a)
-                  out[a * G.stride + idx] = own[a];
-            }
+            for (int a = 0; a < Q; ++a)
+              out[a * G.stride + idx] = own[a];
         }
     }
   } // namespace tile
@@ -500,28 +634,26 @@ namespace lbfem_amd
     return hipGetLastError();
   }
 
-  // The variants (TileVariant): kTileWide 32 x TY tiles of 256 threads, TY = 16 where
-  // the staged fields fit in about 55 KB of LDS, else 8 (two or one rows a thread);
-  // kTileSplit2 / kTileSplit4: 32 x 8 tiles, one node per thread, 2 or 4 threads
-  // per node; kTileSmall4: 16 x 8 tiles, 4 threads per node.
+  // The variants (TileVariant): kTileWide 32 x 8 tiles of 256 threads, kTileTall
+  // 32 x 16 tiles of 512 threads (less halo per node), kTileSplit2 / kTileSplit4
+  // 32 x 8 tiles with 2 or 4 threads per node, kTileSmall4 16 x 8 tiles with 4.
   template <typename T, int p, int MODE>
   hipError_t
   launch_variant(const int variant, const Geo &G, const TileParams &S, const T *fin, const T *feq, T *out, T *rhs,
                  hipStream_t stream)
   {
-    constexpr bool LL  = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
-    constexpr bool F32 = sizeof(T) == 4;
-    constexpr int  TYW = (p == 3 || (LL && !F32)) ? 8 : 16;
     switch (variant)
       {
         case kTileWide:
-          return launch_tile<tile::Cfg<T, p, 32, TYW, TYW / 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<tile::Cfg<T, p, 32, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
+        case kTileTall:
+          return launch_tile<tile::Cfg<T, p, 32, 16, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileSplit2:
-          return launch_tile<tile::Cfg<T, p, 32, 8, 1, 2, MODE>>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<tile::Cfg<T, p, 32, 8, 2, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileSplit4:
-          return launch_tile<tile::Cfg<T, p, 32, 8, 1, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<tile::Cfg<T, p, 32, 8, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileSmall4:
-          return launch_tile<tile::Cfg<T, p, 16, 8, 1, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
+          return launch_tile<tile::Cfg<T, p, 16, 8, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
         default:
           return hipErrorInvalidValue;
       }
