@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--profile-headline", action="store_true")
     ap.add_argument("--profile-cells", type=int, default=256)
     args = ap.parse_args()
@@ -185,6 +187,61 @@ def main():
             log("sweep", sweep[-1])
             del s3, d3
 
+    # ---- batched: 96 seeded uneven instances (flow, N 32..512, Q1/Q2, Re, U0), Bardow lumped, one launch per
+    # (degree, scheme, precision) group per step, graph-replayed, autotuned shapes
+    batched = None
+    if not args.no_batch:
+        from lbfem_amd import workloads as WL
+        insts = WL.batch()
+        tot = sum(WL.nodes(i) for i in insts)
+        batched = {"workload": "96 instances seeded 20261203 (splitmix64): tgv/couette/cavity, N = 32 16^u, "
+                               "Q1 p=0.75 else Q2, Re = 50 64^u, U0 in [0.02, 0.1]; Bardow, lumped, CFL 0.5",
+                   "instances": len(insts), "n_nodes": tot}
+        for prec, name in ((A.FP64, "fp64"), (A.MIXED, "mixed")):
+            ss = [WL.make_scheme(A, i, prec) for i in insts]
+            b = A.Batch(ss)
+            b.autotune()
+            b.step(16)
+            b.synchronize()
+            kb = 160
+            t0 = time.perf_counter()
+            b.step(kb)
+            b.synchronize()
+            tb = time.perf_counter() - t0
+            b.close()
+            for x in ss:
+                x.synchronize()
+            k1 = 40
+            t0 = time.perf_counter()
+            for _ in range(k1):
+                for x in ss:
+                    x.step(1)
+            for x in ss:
+                x.synchronize()
+            t1 = time.perf_counter() - t0
+            batched[name] = {"ms_per_step": round(1e3 * tb / kb, 4), "MLUPS": round(9 * tot * kb / tb / 1e6, 1),
+                             "one_by_one_MLUPS": round(9 * tot * k1 / t1 / 1e6, 1),
+                             "vs_plain_headline": round(9 * tot * kb / tb / 1e6 / value, 2)}
+            log("batched", name, batched[name])
+            del ss, b
+
+    # ---- served: open-loop Poisson streams of seeded cavity/Couette requests, mixed precision, continuous
+    # batching; time-to-solution percentiles per arrival rate (bounded sample: N 32..256, 2000-20000 steps)
+    served = None
+    if not args.no_serve:
+        from lbfem_amd import serving, workloads as WL
+        served = {"workload": "cavity/Couette, N in {32, 64, 128, 256}, Q2 p=0.25, Re log-uniform [100, 5000], "
+                              "U0 in [0.02, 0.1], 2000-20000 steps; Bardow, lumped, CFL 0.5; mixed precision; "
+                              "budget 240 GB, at most 96 resident", "rates": []}
+        for rate in (10, 20, 50, 100, 200):
+            reqs = WL.request_stream(20261203 + rate, 32, rate=float(rate))
+            done, summ = serving.Server(A).run(reqs)
+            row = {"rate_per_s": rate, "requests": summ["requests"], "tts_p50_s": round(summ["tts_p50_s"], 4),
+                   "tts_p99_s": round(summ["tts_p99_s"], 4), "MLUPS": round(summ["MLUPS"], 1),
+                   "wall_s": round(summ["wall_s"], 3)}
+            served["rates"].append(row)
+            log("served", row)
+
     # ---- CPU baseline: the oracle (C++ restatement), one core, bounded sample
     cpu = None
     if not args.no_cpu:
@@ -227,6 +284,8 @@ def main():
         "tile_variant": variant, "autotune_us": [round(1e6 * t, 3) for t in tune_s],
         "variants": variants,
         "hbm_sweep": sweep,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

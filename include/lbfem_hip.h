@@ -194,6 +194,32 @@ int lbfem_scheme_autotune(lbfem_scheme *scheme, int force, int *variant, double 
 int lbfem_diagnostics(const lbfem_disc *disc, const double *f, const double *u_exact, double *u_prev, double *sums,
                       void *stream);
 
+/* ---------------------------------------------------------- batches
+ * No counterpart in the reference (one mesh per process there): many lumped
+ * TG2 schemes (any degree, scheme, FP64 or mixed, sizes and flows) stepped
+ * together. Members are grouped by (degree, scheme, precision); a step is one
+ * launch per group of the tile kernel over the group's concatenated tile list,
+ * and steps are replayed from captured HIP graphs (graph_steps per graph). Each
+ * member's populations are bitwise those it gets stepped alone. While batched,
+ * members are stepped only through the batch and read after
+ * lbfem_batch_synchronize; destroy the batch before its members. */
+typedef struct lbfem_batch lbfem_batch;
+int lbfem_batch_create(lbfem_scheme *const *schemes, int n, lbfem_batch **batch);
+int lbfem_batch_destroy(lbfem_batch *batch);
+/* steps per captured graph: even, 0 = launch every step eagerly (default 8) */
+int lbfem_batch_set_graph_steps(lbfem_batch *batch, int steps);
+int lbfem_batch_step(lbfem_batch *batch, int n_steps);
+int lbfem_batch_synchronize(lbfem_batch *batch);
+int lbfem_batch_get_stream(const lbfem_batch *batch, void **stream);
+/* groups, nodes (slaves included) and tiles of the batch */
+int lbfem_batch_info(const lbfem_batch *batch, int *n_groups, int64_t *n_nodes, int64_t *n_tiles);
+/* every group's tile variant and grid (0: one block per tile; else about that
+ * many persistent blocks, rounded up to a multiple of 8) */
+int lbfem_batch_set_shape(lbfem_batch *batch, int variant, int grid);
+/* times variant x grid for every group (no state change) and keeps the fastest;
+ * seconds: the sum over groups of the best time per step */
+int lbfem_batch_autotune(lbfem_batch *batch, double *seconds);
+
 /* ---------------------------------------------------------- measurement
  * No counterpart in the reference: a STREAM-like device copy of `bytes`
  * (a multiple of 16, 16-byte aligned pointers) on the stream, the measured

@@ -53,6 +53,27 @@ namespace
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept
+      : p(o.p)
+      , n(o.n)
+    {
+      o.p = nullptr;
+      o.n = 0;
+    }
+    DevBuf &
+    operator=(DevBuf &&o) noexcept
+    {
+      if (this != &o)
+        {
+          if (p)
+            (void)hipFree(p);
+          p   = o.p;
+          n   = o.n;
+          o.p = nullptr;
+          o.n = 0;
+        }
+      return *this;
+    }
     ~DevBuf()
     {
       if (p)
@@ -513,6 +534,167 @@ namespace
   as_stream(void *s)
   {
     return reinterpret_cast<hipStream_t>(s);
+  }
+} // namespace
+
+// ---------------------------------------------------------------------- batches
+// Many lumped-mass TG2 schemes stepped together: the members are grouped by
+// (degree, scheme, precision), and a step is one launch per group of the
+// batched tile kernel over the group's concatenated tile list.
+struct lbfem_batch
+{
+  struct Group
+  {
+    int              p = 1, mode = 0, variant = kTileWide, grid = 0; // grid 0: one block per tile
+    bool             mixed = false;
+    std::vector<int> members;
+    int              n_tiles = 0;
+    int64_t          n_nodes = 0;
+    DevBuf<char>     table; // BatchEntry<double or float>[members]
+  };
+  std::vector<lbfem_scheme *> members;
+  std::vector<int>            base_cur; // members' cur when the tables were written (parity 0)
+  std::vector<Group>          groups;
+  hipStream_t                 stream = nullptr;
+  int                         parity = 0;
+  int                         graph_steps = 8; // steps per captured graph, 0: no graphs
+  hipGraph_t                  graph[2]  = {nullptr, nullptr};
+  hipGraphExec_t              exec[2]   = {nullptr, nullptr};
+
+  void
+  drop_graphs()
+  {
+    for (int k = 0; k < 2; ++k)
+      {
+        if (exec[k])
+          (void)hipGraphExecDestroy(exec[k]);
+        if (graph[k])
+          (void)hipGraphDestroy(graph[k]);
+        exec[k]  = nullptr;
+        graph[k] = nullptr;
+      }
+  }
+
+  ~lbfem_batch()
+  {
+    drop_graphs();
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace
+{
+  template <typename T>
+  void
+  write_table(lbfem_batch &b, lbfem_batch::Group &g)
+  {
+    std::vector<BatchEntry<T>> E(g.members.size());
+    int                        begin = 0;
+    g.n_nodes                        = 0;
+    for (size_t k = 0; k < g.members.size(); ++k)
+      {
+        lbfem_scheme &s  = *b.members[g.members[k]];
+        const Geo    &G  = s.d->G;
+        const bool    ll = s.set.scheme == LBFEM_LEELIN;
+        int           tx, ty;
+        tile_dims(g.variant, tx, ty);
+        E[k].G          = G;
+        E[k].S          = tile_params(s, s.sweeps[0], ll);
+        E[k].S.collide  = 1;
+        E[k].S.tiles_x  = (G.P + 1 + tx - 1) / tx;
+        E[k].S.tiles_y  = (G.P + 1 + ty - 1) / ty;
+        if constexpr (sizeof(T) == 4)
+          {
+            E[k].f[0] = s.g32[0].p;
+            E[k].f[1] = s.g32[1].p;
+          }
+        else
+          {
+            E[k].f[0] = s.f[0].p;
+            E[k].f[1] = s.f[1].p;
+          }
+        E[k].par        = s.cur;
+        E[k].tile_begin = begin;
+        begin += E[k].S.tiles_x * E[k].S.tiles_y;
+        g.n_nodes += int64_t(G.np) * G.np;
+      }
+    g.n_tiles = begin;
+    g.table.alloc(E.size() * sizeof(BatchEntry<T>));
+    HIP_CHECK(hipMemcpy(g.table.p, E.data(), E.size() * sizeof(BatchEntry<T>), hipMemcpyHostToDevice));
+  }
+
+  // the tables at the members' current buffers (parity 0 from here)
+  void
+  rebuild_tables(lbfem_batch &b)
+  {
+    HIP_CHECK(hipStreamSynchronize(b.stream));
+    b.drop_graphs();
+    b.parity = 0;
+    for (size_t k = 0; k < b.members.size(); ++k)
+      b.base_cur[k] = b.members[k]->cur;
+    for (auto &g : b.groups)
+      {
+        if (g.mixed)
+          write_table<float>(b, g);
+        else
+          write_table<double>(b, g);
+      }
+  }
+
+  int
+  launch_group(lbfem_batch &b, lbfem_batch::Group &g, const int parity, const int variant, const int grid)
+  {
+    const int n = int(g.members.size());
+    if (g.mixed)
+      HIP_CHECK(tile_batch_launch<float>(g.p, g.mode, variant, reinterpret_cast<const BatchEntry<float> *>(g.table.p),
+                                         n, g.n_tiles, parity, grid, b.stream));
+    else
+      HIP_CHECK(tile_batch_launch<double>(g.p, g.mode, variant,
+                                          reinterpret_cast<const BatchEntry<double> *>(g.table.p), n, g.n_tiles,
+                                          parity, grid, b.stream));
+    return LBFEM_OK;
+  }
+
+  int
+  batch_step_eager(lbfem_batch &b, const int parity)
+  {
+    for (auto &g : b.groups)
+      if (const int st = launch_group(b, g, parity, g.variant, g.grid); st != LBFEM_OK)
+        return st;
+    return LBFEM_OK;
+  }
+
+  // graph[k]: graph_steps steps starting at parity k (graph_steps even, so parity returns to k)
+  int
+  capture_graphs(lbfem_batch &b)
+  {
+    for (int k = 0; k < 2; ++k)
+      {
+        HIP_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+        int st = LBFEM_OK;
+        for (int n = 0; n < b.graph_steps && st == LBFEM_OK; ++n)
+          st = batch_step_eager(b, (k + n) & 1);
+        hipGraph_t       gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b.stream, &gr);
+        if (st != LBFEM_OK)
+          {
+            if (gr)
+              (void)hipGraphDestroy(gr);
+            return st;
+          }
+        HIP_CHECK(ec);
+        b.graph[k] = gr;
+        HIP_CHECK(hipGraphInstantiate(&b.exec[k], gr, nullptr, nullptr, 0));
+      }
+    return LBFEM_OK;
+  }
+
+  void
+  sync_members(lbfem_batch &b)
+  {
+    for (size_t k = 0; k < b.members.size(); ++k)
+      b.members[k]->cur = (b.base_cur[k] + b.parity) & 1;
   }
 } // namespace
 
@@ -1242,6 +1424,242 @@ extern "C"
         }
       if (variant)
         *variant = s->tile_variant;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_create(lbfem_scheme *const *schemes, const int n, lbfem_batch **out)
+  {
+    return guarded([&]() -> int {
+      if (!schemes || !out || n <= 0)
+        return fail(LBFEM_EINVAL, "lbfem_batch_create: no schemes");
+      auto b = std::make_unique<lbfem_batch>();
+      for (int k = 0; k < n; ++k)
+        {
+          lbfem_scheme *s = schemes[k];
+          if (!s)
+            return fail(LBFEM_EINVAL, "null scheme");
+          if (!(s->set.mass.type == LBFEM_MASS_LUMPED && s->set.streaming == LBFEM_TG2))
+            return fail(LBFEM_EINVAL, "a batch steps schemes of the lumped mass and TG2 streaming (one fused kernel)");
+          if (!(s->dt > 0.))
+            return fail(LBFEM_EINVAL, "set the time step of every scheme before batching");
+          for (int j = 0; j < k; ++j)
+            if (schemes[j] == s)
+              return fail(LBFEM_EINVAL, "a scheme appears twice in the batch");
+          HIP_CHECK(hipStreamSynchronize(s->stream));
+          b->members.push_back(s);
+        }
+      b->base_cur.resize(n);
+      for (int k = 0; k < n; ++k)
+        {
+          const lbfem_scheme &s    = *b->members[k];
+          const int           mode = s.set.scheme == LBFEM_LEELIN ? kStepLeeLin : kStepBardow;
+          auto                it   = std::find_if(b->groups.begin(), b->groups.end(), [&](const auto &g) {
+            return g.p == s.d->G.p && g.mode == mode && g.mixed == s.mixed;
+          });
+          if (it == b->groups.end())
+            {
+              b->groups.emplace_back();
+              it        = b->groups.end() - 1;
+              it->p     = s.d->G.p;
+              it->mode  = mode;
+              it->mixed = s.mixed;
+            }
+          it->members.push_back(k);
+        }
+      // default shape: the variant the size-based rule picks for the group's total node count
+      for (auto &g : b->groups)
+        {
+          int64_t nodes = 0;
+          for (const int k : g.members)
+            nodes += int64_t(b->members[k]->d->G.np) * b->members[k]->d->G.np;
+          g.variant = nodes < 64LL * 4 * 1024 ? kTileSplit4 : kTileWide;
+        }
+      HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+      rebuild_tables(*b);
+      *out = b.release();
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_destroy(lbfem_batch *b)
+  {
+    return guarded([&]() -> int {
+      if (b)
+        {
+          (void)hipStreamSynchronize(b->stream);
+          delete b;
+        }
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_set_graph_steps(lbfem_batch *b, const int steps)
+  {
+    return guarded([&]() -> int {
+      if (!b || steps < 0 || (steps & 1))
+        return fail(LBFEM_EINVAL, "steps per graph: an even number >= 0");
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      b->drop_graphs();
+      b->graph_steps = steps;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_step(lbfem_batch *b, const int n_steps)
+  {
+    return guarded([&]() -> int {
+      if (!b || n_steps < 0)
+        return fail(LBFEM_EINVAL, "invalid argument");
+      int left = n_steps;
+      if (b->graph_steps > 0 && left >= b->graph_steps)
+        {
+          if (!b->exec[0])
+            if (const int st = capture_graphs(*b); st != LBFEM_OK)
+              return st;
+          for (; left >= b->graph_steps; left -= b->graph_steps)
+            HIP_CHECK(hipGraphLaunch(b->exec[b->parity], b->stream));
+        }
+      for (; left > 0; --left)
+        {
+          if (const int st = batch_step_eager(*b, b->parity); st != LBFEM_OK)
+            return st;
+          b->parity ^= 1;
+        }
+      sync_members(*b);
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_synchronize(lbfem_batch *b)
+  {
+    return guarded([&]() -> int {
+      if (!b)
+        return fail(LBFEM_EINVAL, "null argument");
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_get_stream(const lbfem_batch *b, void **stream)
+  {
+    return guarded([&]() -> int {
+      if (!b || !stream)
+        return fail(LBFEM_EINVAL, "null argument");
+      *stream = static_cast<void *>(b->stream);
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_info(const lbfem_batch *b, int *n_groups, int64_t *n_nodes, int64_t *n_tiles)
+  {
+    return guarded([&]() -> int {
+      if (!b)
+        return fail(LBFEM_EINVAL, "null argument");
+      int64_t nodes = 0, tiles = 0;
+      for (const auto &g : b->groups)
+        {
+          nodes += g.n_nodes;
+          tiles += g.n_tiles;
+        }
+      if (n_groups)
+        *n_groups = int(b->groups.size());
+      if (n_nodes)
+        *n_nodes = nodes;
+      if (n_tiles)
+        *n_tiles = tiles;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_set_shape(lbfem_batch *b, const int variant, const int grid)
+  {
+    return guarded([&]() -> int {
+      if (!b || variant < 0 || variant >= kTileVariants || grid < 0)
+        return fail(LBFEM_EINVAL, "invalid tile variant or grid");
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      for (auto &g : b->groups)
+        {
+          g.variant = variant;
+          g.grid    = grid;
+        }
+      rebuild_tables(*b);
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_autotune(lbfem_batch *b, double *seconds)
+  {
+    return guarded([&]() -> int {
+      if (!b)
+        return fail(LBFEM_EINVAL, "null argument");
+      rebuild_tables(*b); // parity 0 at the members' current buffers
+      hipEvent_t e0, e1;
+      HIP_CHECK(hipEventCreate(&e0));
+      HIP_CHECK(hipEventCreate(&e1));
+      const int grids[3] = {0, 512, 2048};
+      double    total    = 0.;
+      for (auto &g : b->groups)
+        {
+          double best = 1e300;
+          int    bv = g.variant, bg = g.grid;
+          for (int v = 0; v < kTileVariants; ++v)
+            {
+              // tile counts depend on the variant
+              g.variant = v;
+              if (g.mixed)
+                write_table<float>(*b, g);
+              else
+                write_table<double>(*b, g);
+              for (const int grid : grids)
+                {
+                  // launches at a fixed parity read one buffer and write the other: no state changes
+                  for (int k = 0; k < 2; ++k)
+                    if (const int st = launch_group(*b, g, b->parity, v, grid); st != LBFEM_OK)
+                      return st;
+                  float ms = 0.f;
+                  HIP_CHECK(hipEventRecord(e0, b->stream));
+                  for (int k = 0; k < 3; ++k)
+                    if (const int st = launch_group(*b, g, b->parity, v, grid); st != LBFEM_OK)
+                      return st;
+                  HIP_CHECK(hipEventRecord(e1, b->stream));
+                  HIP_CHECK(hipEventSynchronize(e1));
+                  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                  const int n = std::max(5, std::min(100, int(2.0 / std::max(1e-4, double(ms) / 3))));
+                  HIP_CHECK(hipEventRecord(e0, b->stream));
+                  for (int k = 0; k < n; ++k)
+                    if (const int st = launch_group(*b, g, b->parity, v, grid); st != LBFEM_OK)
+                      return st;
+                  HIP_CHECK(hipEventRecord(e1, b->stream));
+                  HIP_CHECK(hipEventSynchronize(e1));
+                  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                  const double t = 1e-3 * ms / n;
+                  if (t < best)
+                    {
+                      best = t;
+                      bv   = v;
+                      bg   = grid;
+                    }
+                }
+            }
+          g.variant = bv;
+          g.grid    = bg;
+          total += best;
+        }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      rebuild_tables(*b); // the chosen shapes
+      if (seconds)
+        *seconds = total;
       return LBFEM_OK;
     });
   }

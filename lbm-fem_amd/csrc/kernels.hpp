@@ -74,6 +74,57 @@ namespace lbfem_amd
       }
   }
 
+  // tile shape of a variant
+  inline void
+  tile_dims(const int variant, int &tx, int &ty)
+  {
+    tx = variant == kTileSmall4 ? 16 : 32;
+    ty = variant == kTileTall ? 16 : 8;
+  }
+
+  // one instance of a batched fused step (device table entry)
+  template <typename T>
+  struct BatchEntry
+  {
+    Geo        G;
+    TileParams S;          // tiles_x, tiles_y of the batch's variant
+    T         *f[2];       // the instance's population buffers
+    int        par;        // buffer read at parity 0
+    int        tile_begin; // first tile in the batch's tile list
+  };
+
+  template <typename T, int p>
+  hipError_t tile_batch_launch_degree(int mode, int variant, const BatchEntry<T> *table, int n_inst, int n_tiles,
+                                      int parity, int grid, hipStream_t stream);
+#define LBFEM_BATCH_EXTERN(T, P)                                                                                   \
+  extern template hipError_t tile_batch_launch_degree<T, P>(int, int, const BatchEntry<T> *, int, int, int, int,   \
+                                                            hipStream_t);
+  LBFEM_BATCH_EXTERN(double, 1)
+  LBFEM_BATCH_EXTERN(double, 2)
+  LBFEM_BATCH_EXTERN(double, 3)
+  LBFEM_BATCH_EXTERN(float, 1)
+  LBFEM_BATCH_EXTERN(float, 2)
+  LBFEM_BATCH_EXTERN(float, 3)
+#undef LBFEM_BATCH_EXTERN
+
+  template <typename T>
+  inline hipError_t
+  tile_batch_launch(const int p, const int mode, const int variant, const BatchEntry<T> *table, const int n_inst,
+                    const int n_tiles, const int parity, const int grid, hipStream_t stream)
+  {
+    switch (p)
+      {
+        case 1:
+          return tile_batch_launch_degree<T, 1>(mode, variant, table, n_inst, n_tiles, parity, grid, stream);
+        case 2:
+          return tile_batch_launch_degree<T, 2>(mode, variant, table, n_inst, n_tiles, parity, grid, stream);
+        case 3:
+          return tile_batch_launch_degree<T, 3>(mode, variant, table, n_inst, n_tiles, parity, grid, stream);
+        default:
+          return hipErrorInvalidValue;
+      }
+  }
+
   // The variant used until a measurement picks one: below about four wide-tile
   // wavefronts per SIMD of the device (256 CUs x 4 SIMDs) a node's populations
   // are split over four threads, so that the device fills.

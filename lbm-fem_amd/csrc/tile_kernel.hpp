@@ -28,6 +28,8 @@
 // atomics; the periodic seam is a wrap of the halo.
 #pragma once
 
+#include <algorithm>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -621,6 +623,51 @@ namespace lbfem_amd
     tile::body<C>(G, S, tile, fin, feq_in, out, rhs_out, sm);
   }
 
+  // Many meshes in one launch: the tiles of all instances of the table form one
+  // list (instance k owns tiles [tile_begin_k, tile_begin_k+1)); block b takes
+  // tiles b, b + gridDim.x, ... in the XCD-aware order (gridDim.x a multiple of
+  // 8), so that a grid of a few blocks per CU is persistent and a grid of one
+  // block per tile is not. Each instance flips its own population buffers: the
+  // step reads f[(par + parity) & 1] and writes the other.
+  template <class C>
+  __global__ void __launch_bounds__(C::NT)
+  k_tile_batch(const BatchEntry<typename C::T> *__restrict__ E, const int n_inst, const int n_tiles,
+               const int parity)
+  {
+    __shared__ tile::Smem<C> sm;
+    const int nb8 = (n_tiles + 7) & ~7;
+    for (int b = blockIdx.x; b < nb8; b += gridDim.x)
+      {
+        const int t = (b & 7) * (nb8 >> 3) + (b >> 3);
+        if (t >= n_tiles)
+          continue;
+        int lo = 0, hi = n_inst - 1; // the instance: last tile_begin <= t
+        while (lo < hi)
+          {
+            const int mid = (lo + hi + 1) >> 1;
+            if (E[mid].tile_begin <= t)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+        const BatchEntry<typename C::T> &e   = E[lo];
+        const int                        cur = (e.par + parity) & 1;
+        tile::body<C>(e.G, e.S, t - e.tile_begin, e.f[cur], nullptr, e.f[cur ^ 1], nullptr, sm);
+        __syncthreads(); // the next tile reuses the LDS
+      }
+  }
+
+  template <class C>
+  hipError_t
+  launch_batch(const BatchEntry<typename C::T> *table, const int n_inst, const int n_tiles, const int parity,
+               const int grid, hipStream_t stream)
+  {
+    const int nb8 = (n_tiles + 7) & ~7;
+    const int g   = grid > 0 ? std::min(nb8, (grid + 7) & ~7) : nb8;
+    hipLaunchKernelGGL((k_tile_batch<C>), dim3(g), dim3(C::NT), 0, stream, table, n_inst, n_tiles, parity);
+    return hipGetLastError();
+  }
+
   template <class C>
   hipError_t
   launch_tile(const Geo &G, TileParams S, const typename C::T *fin, const typename C::T *feq, typename C::T *out,
@@ -654,6 +701,45 @@ namespace lbfem_amd
           return launch_tile<tile::Cfg<T, p, 32, 8, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileSmall4:
           return launch_tile<tile::Cfg<T, p, 16, 8, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
+        default:
+          return hipErrorInvalidValue;
+      }
+  }
+
+  template <typename T, int p, int MODE>
+  hipError_t
+  launch_batch_variant(const int variant, const BatchEntry<T> *table, const int n_inst, const int n_tiles,
+                       const int parity, const int grid, hipStream_t stream)
+  {
+    switch (variant)
+      {
+        case kTileWide:
+          return launch_batch<tile::Cfg<T, p, 32, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        case kTileTall:
+          return launch_batch<tile::Cfg<T, p, 32, 16, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        case kTileSplit2:
+          return launch_batch<tile::Cfg<T, p, 32, 8, 2, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        case kTileSplit4:
+          return launch_batch<tile::Cfg<T, p, 32, 8, 4, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        case kTileSmall4:
+          return launch_batch<tile::Cfg<T, p, 16, 8, 4, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        default:
+          return hipErrorInvalidValue;
+      }
+  }
+
+  // the fused lumped step of many instances of one degree and form
+  template <typename T, int p>
+  hipError_t
+  tile_batch_launch_degree(const int mode, const int variant, const BatchEntry<T> *table, const int n_inst,
+                           const int n_tiles, const int parity, const int grid, hipStream_t stream)
+  {
+    switch (mode)
+      {
+        case kStepBardow:
+          return launch_batch_variant<T, p, kStepBardow>(variant, table, n_inst, n_tiles, parity, grid, stream);
+        case kStepLeeLin:
+          return launch_batch_variant<T, p, kStepLeeLin>(variant, table, n_inst, n_tiles, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }

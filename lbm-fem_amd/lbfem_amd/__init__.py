@@ -105,6 +105,15 @@ def lib():
             "lbfem_scheme_stage_times": (i, [_vp, p]),
             "lbfem_diagnostics": (i, [_vp, _vp, _vp, _vp, p, _vp]),
             "lbfem_hbm_copy": (i, [_vp, _vp, C.c_size_t, _vp]),
+            "lbfem_batch_create": (i, [C.POINTER(_vp), i, C.POINTER(_vp)]),
+            "lbfem_batch_destroy": (i, [_vp]),
+            "lbfem_batch_set_graph_steps": (i, [_vp, i]),
+            "lbfem_batch_step": (i, [_vp, i]),
+            "lbfem_batch_synchronize": (i, [_vp]),
+            "lbfem_batch_get_stream": (i, [_vp, C.POINTER(_vp)]),
+            "lbfem_batch_info": (i, [_vp, C.POINTER(C.c_int), i64, i64]),
+            "lbfem_batch_set_shape": (i, [_vp, i, i]),
+            "lbfem_batch_autotune": (i, [_vp, p]),
             "lbfem_scheme_set_tile_variant": (i, [_vp, i]),
             "lbfem_scheme_get_stream": (i, [_vp, C.POINTER(_vp)]),
             "lbfem_scheme_get_tile_variant": (i, [_vp, C.POINTER(C.c_int)]),
@@ -384,3 +393,51 @@ def hbm_copy(dst, src, stream=None):
     if dst.numel() * dst.element_size() != nbytes:
         raise LbfemError(EINVAL, "hbm_copy: tensors differ in size")
     check(lib().lbfem_hbm_copy(C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()), nbytes, _stream(stream)))
+
+
+class Batch:
+    """lbfem_batch: lumped TG2 schemes stepped together, one launch per (degree, scheme, precision) group
+    per step, steps replayed from captured HIP graphs. Members are read after synchronize(); the batch keeps
+    references to them, and is destroyed (close()) before they are."""
+
+    def __init__(self, schemes, graph_steps=8):
+        self.schemes = list(schemes)
+        arr = (_vp * len(self.schemes))(*[s.h.value for s in self.schemes])
+        h = _vp()
+        check(lib().lbfem_batch_create(arr, len(self.schemes), C.byref(h)))
+        self.h = h
+        if graph_steps != 8:
+            check(lib().lbfem_batch_set_graph_steps(self.h, int(graph_steps)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lbfem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n=1):
+        check(lib().lbfem_batch_step(self.h, int(n)))
+
+    def synchronize(self):
+        check(lib().lbfem_batch_synchronize(self.h))
+
+    def set_shape(self, variant, grid=0):
+        check(lib().lbfem_batch_set_shape(self.h, int(variant), int(grid)))
+
+    def autotune(self):
+        t = C.c_double(0.0)
+        check(lib().lbfem_batch_autotune(self.h, C.byref(t)))
+        return t.value
+
+    def info(self):
+        g, n, t = C.c_int(0), C.c_int64(0), C.c_int64(0)
+        check(lib().lbfem_batch_info(self.h, C.byref(g), C.byref(n), C.byref(t)))
+        return {"groups": g.value, "nodes": n.value, "tiles": t.value}
+
+    @property
+    def stream_handle(self):
+        h = _vp()
+        check(lib().lbfem_batch_get_stream(self.h, C.byref(h)))
+        return h.value or 0
