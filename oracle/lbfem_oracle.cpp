@@ -417,9 +417,10 @@ struct ora_disc
   assemble(const double *Rc, double *out) const
   {
     const int nn = p + 1;
-    int       cxs[2], kxs[2], cys[2], kys[2];
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < np; ++j)
       {
+        int       cxs[2], kxs[2], cys[2], kys[2];
         const int my_ = around(j, per[1], cys, kys);
         for (int i = 0; i < np; ++i)
           {
@@ -757,6 +758,10 @@ namespace
     for (int al = 0; al < NM; ++al)
       {
         const double ex = e[2 * (al + 1)], ey = e[2 * (al + 1) + 1];
+        // cells are independent (each writes its own block of Rc); with OpenMP each
+        // thread has its own scratch, and the result is the serial one bit for bit
+#pragma omp parallel firstprivate(G, H, gx, gy, hgx, hgy, v, Fx, Fy)
+#pragma omp for collapse(2) schedule(static)
         for (int cy = 0; cy < N; ++cy)
           for (int cx = 0; cx < N; ++cx)
             {
@@ -944,6 +949,7 @@ namespace
     *its        = 0;
     if (type == ORA_MASS_LUMPED) // mass.h:189-191
       {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i)
           x[i] = r[i] / d.ml[i];
         return ORA_OK;
@@ -1163,9 +1169,10 @@ extern "C"
   {
     // include/lbfem/collision.h:152-168
     const int n = d->n;
-    double    gi[Q], geq[Q], gt[Q];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
       {
+        double gi[Q], geq[Q], gt[Q];
         for (int a = 0; a < Q; ++a)
           gi[a] = g[a * n + i];
         const Moments m = moments(gi);
@@ -1181,9 +1188,10 @@ extern "C"
   {
     // include/lbfem/collision.h:105-113
     const int n = d->n;
-    double    fi[Q], e[Q];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
       {
+        double fi[Q], e[Q];
         for (int a = 0; a < Q; ++a)
           fi[a] = f[a * n + i];
         equilibrium(impose(*d, i, moments(fi)), e);
@@ -1198,9 +1206,10 @@ extern "C"
     // include/lbfem/collision.h:119-145
     const int    n   = d->n;
     const double inv = 1. / (1. + theta);
-    double       fh[Q], eh[Q];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
       {
+        double fh[Q], eh[Q];
         fh[0] = inv * (f[i] + theta * feq[i]);
         for (int a = 1; a < Q; ++a)
           fh[a] = inv * (f[a * n + i] + theta * feq[a * n + i] + incr[(a - 1) * n + i]);

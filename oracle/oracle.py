@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblbfem_oracle.so")
+# LBFEM_ORACLE_OMP=1 loads the OpenMP build (bitwise the serial results; the multi-core CPU baseline)
+LIB_PATH = os.path.join(HERE, "liblbfem_oracle_omp.so" if os.environ.get("LBFEM_ORACLE_OMP") == "1"
+                        else "liblbfem_oracle.so")
 
 TGV, COUETTE, CAVITY = 0, 1, 2
 LEELIN, BARDOW = 0, 1
