@@ -194,6 +194,14 @@ int lbfem_scheme_autotune(lbfem_scheme *scheme, int force, int *variant, double 
 int lbfem_diagnostics(const lbfem_disc *disc, const double *f, const double *u_exact, double *u_prev, double *sums,
                       void *stream);
 
+/* The stream function of a nodal velocity field on a walled box mesh
+ * (include/lbfem/io.h:31-93, compute_stream_function): (grad phi, grad psi) =
+ * (dphi/dx, v) - (dphi/dy, u), psi = 0 on the walls, CG with Jacobi to 1e-12 |b|.
+ * u, v, psi: host arrays of n_nodes (lexicographic); n_periodic must be 0.
+ * Host computation (post-processing), no device needed. */
+int lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const double *v, double *psi,
+                          int *iterations);
+
 /* ---------------------------------------------------------- batches
  * No counterpart in the reference (one mesh per process there): many lumped
  * TG2 schemes (any degree, scheme, FP64 or mixed, sizes and flows) stepped

@@ -50,6 +50,12 @@ def build(verbose=True, jobs=None):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # the C++ driver over the C ABI (host code only: g++, no HIP headers)
+    drv = os.path.join(OUT_DIR, "cgdbe_hip")
+    subprocess.check_call([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
+                           "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "apps", "cgdbe_hip.cpp"),
+                           "-L", OUT_DIR, "-llbfem_hip", "-Wl,-rpath,$ORIGIN", "-o", drv + ".tmp"])
+    os.replace(drv + ".tmp", drv)
     if verbose:
         print(f"[build] {LIB} ({time.time() - t0:.1f} s)")
     return LIB

@@ -1664,6 +1664,115 @@ extern "C"
     });
   }
 
+  // Stream function of a nodal velocity field (include/lbfem/io.h:31-93): with
+  // u = -dpsi/dy, v = dpsi/dx, (grad phi, grad psi) = (dphi/dx, v) - (dphi/dy, u)
+  // and psi = 0 on the walls, solved by deal.II's SolverCG sequence with the
+  // Jacobi preconditioner to 1e-12 |b|, at most 10000 iterations. On the tensor
+  // mesh the Laplacian is Kx (x) My + Mx (x) Ky and the right-hand side
+  // (Dx^T (x) My) v - (Mx (x) Dy^T) u, from the same assembled 1-D bands as the
+  // streaming operator. Host arithmetic: post-processing, not the time step.
+  int
+  lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const double *v, double *psi,
+                        int *iterations)
+  {
+    return guarded([&]() -> int {
+      if (!desc || !u || !v || !psi)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (desc->n_cells < 2 || desc->degree < 1 || desc->degree > kMaxDegree || !(desc->L > 0.) ||
+          desc->stretch < 0.)
+        return fail(LBFEM_EINVAL, "invalid mesh description (N >= 2, degree 1..3, L > 0)");
+      if (desc->n_periodic != 0)
+        return fail(LBFEM_EINVAL, "the stream function is computed for walled boxes (n_periodic = 0)");
+      const Basis1D B(desc->degree);
+      const Axis1D  ax(B, box_vertices(desc->n_cells, desc->L, desc->stretch), false);
+      const int     p = desc->degree, np = ax.P + 1, w = 2 * p + 1;
+      const size_t  n = size_t(np) * np;
+      auto          boundary = [&](const int i, const int j) { return i == 0 || j == 0 || i == ax.P || j == ax.P; };
+      // y = A x over the interior rows (constrained rows: identity)
+      auto apply = [&](const std::vector<double> &x, std::vector<double> &y, const std::vector<double> *Bx,
+                       const std::vector<double> *By, const std::vector<double> *Cx, const std::vector<double> *Cy) {
+        for (int j = 0; j < np; ++j)
+          for (int i = 0; i < np; ++i)
+            {
+              double s = 0.;
+              for (int oy = 0; oy < w; ++oy)
+                {
+                  const int jj = j + oy - p;
+                  if (jj < 0 || jj >= np)
+                    continue;
+                  const double by = (*By)[size_t(j) * w + oy], cy = (*Cy)[size_t(j) * w + oy];
+                  for (int ox = 0; ox < w; ++ox)
+                    {
+                      const int ii = i + ox - p;
+                      if (ii < 0 || ii >= np)
+                        continue;
+                      s += ((*Bx)[size_t(i) * w + ox] * by + (*Cx)[size_t(i) * w + ox] * cy) * x[size_t(jj) * np + ii];
+                    }
+                }
+              y[size_t(j) * np + i] = s;
+            }
+      };
+      std::vector<double> b(n), zero(n, 0.), uu(u, u + n), vv(v, v + n), tmp(n);
+      // b = (Dx^T (x) My) v - (Mx (x) Dy^T) u
+      apply(vv, b, &ax.DT, &ax.M, &zero, &zero);
+      apply(uu, tmp, &zero, &zero, &ax.M, &ax.DT);
+      std::vector<double> diag(n);
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i)
+          {
+            const size_t k = size_t(j) * np + i;
+            b[k]           = boundary(i, j) ? 0. : b[k] - tmp[k];
+            diag[k]        = boundary(i, j) ? 1. :
+                                              ax.K[size_t(i) * w + p] * ax.M[size_t(j) * w + p] +
+                                         ax.M[size_t(i) * w + p] * ax.K[size_t(j) * w + p];
+          }
+      auto laplace = [&](const std::vector<double> &x, std::vector<double> &y) {
+        apply(x, y, &ax.K, &ax.M, &ax.M, &ax.K);
+        for (int j = 0; j < np; ++j)
+          for (int i = 0; i < np; ++i)
+            if (boundary(i, j))
+              y[size_t(j) * np + i] = x[size_t(j) * np + i];
+      };
+      auto dot = [&](const std::vector<double> &a, const std::vector<double> &c) {
+        double s = 0.;
+        for (size_t k = 0; k < n; ++k)
+          s += a[k] * c[k];
+        return s;
+      };
+      // SolverCG (deal.II): x = 0, r = A x - b = -b
+      std::vector<double> x(n, 0.), r(n), pp(n, 0.), vq(n);
+      for (size_t k = 0; k < n; ++k)
+        r[k] = -b[k];
+      const double tol = 1e-12 * std::sqrt(dot(b, b));
+      double       res = std::sqrt(dot(r, r)), rho = 0., rho_prev = 1.;
+      int          it = 0;
+      for (; res > tol; ++it)
+        {
+          if (it >= 10000)
+            return fail(LBFEM_ENOCONV, "stream function: CG did not converge in 10000 iterations");
+          for (size_t k = 0; k < n; ++k)
+            vq[k] = r[k] / diag[k];
+          rho               = dot(r, vq);
+          const double beta = it == 0 ? 0. : rho / rho_prev;
+          for (size_t k = 0; k < n; ++k)
+            pp[k] = beta * pp[k] - vq[k];
+          laplace(pp, vq);
+          const double alpha = rho / dot(pp, vq);
+          for (size_t k = 0; k < n; ++k)
+            {
+              x[k] += alpha * pp[k];
+              r[k] += alpha * vq[k];
+            }
+          rho_prev = rho;
+          res      = std::sqrt(std::abs(dot(r, r)));
+        }
+      std::copy(x.begin(), x.end(), psi);
+      if (iterations)
+        *iterations = it;
+      return LBFEM_OK;
+    });
+  }
+
   int
   lbfem_hbm_copy(void *dst, const void *src, size_t bytes, void *stream)
   {

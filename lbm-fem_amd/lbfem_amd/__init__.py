@@ -105,6 +105,7 @@ def lib():
             "lbfem_scheme_stage_times": (i, [_vp, p]),
             "lbfem_diagnostics": (i, [_vp, _vp, _vp, _vp, p, _vp]),
             "lbfem_hbm_copy": (i, [_vp, _vp, C.c_size_t, _vp]),
+            "lbfem_stream_function": (i, [C.POINTER(MeshDesc), p, p, p, C.POINTER(C.c_int)]),
             "lbfem_batch_create": (i, [C.POINTER(_vp), i, C.POINTER(_vp)]),
             "lbfem_batch_destroy": (i, [_vp]),
             "lbfem_batch_set_graph_steps": (i, [_vp, i]),
@@ -441,3 +442,15 @@ class Batch:
         h = _vp()
         check(lib().lbfem_batch_get_stream(self.h, C.byref(h)))
         return h.value or 0
+
+
+def stream_function(n_cells, degree, u, v, L=1.0, stretch=0.0):
+    """include/lbfem/io.h:31-93: psi with u = -dpsi/dy, v = dpsi/dx, psi = 0 on the walls of the box
+    (lexicographic nodal arrays); returns (psi, CG iterations). Host computation."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    psi = np.zeros_like(u)
+    its = C.c_int(0)
+    desc = MeshDesc(int(n_cells), int(degree), 0, float(L), float(stretch))
+    check(lib().lbfem_stream_function(C.byref(desc), _p(u), _p(v), _p(psi), C.byref(its)))
+    return psi, its.value
