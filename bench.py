@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--profile-headline", action="store_true")
     ap.add_argument("--profile-cells", type=int, default=256)
@@ -187,7 +188,8 @@ def main():
     # ---- consistent mass (CG tol 1e-10), and Q2 128^2
     for name, N, p, mass in (("q1_256_leelin_cg", 256, 1, A.MASS_CG), ("q2_128_leelin_lumped", 128, 2, A.MASS_LUMPED),
                              ("q2_128_leelin_cg", 128, 2, A.MASS_CG), ("q1_256_bardow_lumped", 256, 1, None),
-                             ("q1_256_leelin_lumped_mixed", 256, 1, None), ("q2_128_leelin_lumped_mixed", 128, 2, None)):
+                             ("q1_256_leelin_lumped_mixed", 256, 1, None), ("q2_128_leelin_lumped_mixed", 128, 2, None),
+                             ("q1_256_leelin_cg_mixed", 256, 1, A.MASS_CG)):
         leelin = "bardow" not in name
         d2, s2 = make_scheme(A, N, p, leelin=leelin, mass=mass, mixed=name.endswith("mixed"))
         steps = max(10, K // 4) if mass == A.MASS_CG else K
@@ -213,6 +215,50 @@ def main():
                           "achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)})
             log("sweep", sweep[-1])
             del s3, d3
+
+    # ---- precision: TGV (1,1) on Q1 1024^2, Bardow, consistent mass (CG 1e-10), FP64 and mixed (FP32 inner CG
+    # refined with FP64 residuals); L2 velocity error of the mixed run vs FP64 and both vs the analytic decay
+    precision = None
+    if not args.no_precision:
+        import numpy as np
+        runs = {}
+        steps_p = 100
+        # mixed_tol1e-6: FP32 storage puts round-off of about 1e-5 |b| into the right-hand side at this size,
+        # which a 1e-10 solve resolves at a cost of many iterations and no accuracy
+        for prec, name, tol in ((A.FP64, "fp64", 1e-10), (A.MIXED, "mixed", 1e-10), (A.MIXED, "mixed_tol1e-6", 1e-6)):
+            tc = A.test_cases.make("tgv", mach=0.1, modes=(1, 1))
+            tc.set_reynolds(100.0)
+            dp = A.Discretization(tc, 1024, 1)
+            sp = A.Bardow(dp, mass=A.MASS_CG, cg_tolerance=tol, precision=prec)
+            dtp = dp.time_step(0.25)
+            sp.set_time_step(dtp, tc.relaxation_time())
+            sp.set_initial_populations(tc)
+            sp.step(5)
+            sp.synchronize()
+            its0 = sp.cg_iterations
+            t0 = time.perf_counter()
+            sp.step(steps_p)
+            sp.synchronize()
+            tp = time.perf_counter() - t0
+            _, u, v = A.moments(sp.f)
+            runs[name] = {"ms_per_step": round(1e3 * tp / steps_p, 3),
+                          "MLUPS": round(9 * dp.n_nodes * steps_p / tp / 1e6, 1),
+                          "cg_its_per_solve": round((sp.cg_iterations - its0) / (8 * steps_p), 2), "u": u, "v": v}
+            xy, indep, w = dp.node, ~dp.constrained(), dp.lumped_mass()
+            del sp
+        _, ue, ve = tc.exact(xy[:, 0], xy[:, 1], (steps_p + 5) * dtp)
+        l2 = lambda u, v, ur, vr: float(np.sqrt((w[indep] * ((u - ur) ** 2 + (v - vr) ** 2)[indep]).sum()
+                                                / (w[indep] * (ur ** 2 + vr ** 2)[indep]).sum()))
+        precision = {"workload": "Taylor-Green (1,1), Q1 1024^2 cells, Bardow, consistent mass CG 1e-10, CFL 0.25, "
+                                 f"Re 100, {steps_p} steps after 5",
+                     "l2_velocity_mixed_vs_fp64": l2(runs["mixed"]["u"], runs["mixed"]["v"], runs["fp64"]["u"],
+                                                     runs["fp64"]["v"]),
+                     "l2_velocity_mixed_tol1e-6_vs_fp64": l2(runs["mixed_tol1e-6"]["u"], runs["mixed_tol1e-6"]["v"],
+                                                             runs["fp64"]["u"], runs["fp64"]["v"]),
+                     "l2_velocity_vs_analytic": {k: l2(r["u"], r["v"], ue, ve) for k, r in runs.items()}}
+        for k, r in runs.items():
+            precision[k] = {x: r[x] for x in ("ms_per_step", "MLUPS", "cg_its_per_solve")}
+        log("precision", precision)
 
     # ---- batched: 96 seeded uneven instances (flow, N 32..512, Q1/Q2, Re, U0), Bardow lumped, one launch per
     # (degree, scheme, precision) group per step, graph-replayed, autotuned shapes
@@ -308,6 +354,7 @@ def main():
         "tile_variant": variant, "autotune_us": [round(1e6 * t, 3) for t in tune_s],
         "variants": variants,
         "hbm_sweep": sweep,
+        "precision": precision,
         "batched": batched,
         "served": served,
     }

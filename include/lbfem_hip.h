@@ -141,7 +141,9 @@ typedef struct
   int                 streaming; /* LBFEM_TG2 | LBFEM_TG3 | LBFEM_TG3_SPLIT */
   lbfem_mass_settings mass;
   int                 precision; /* LBFEM_FP64 | LBFEM_MIXED: FP32 populations stored as
-                                    deviations f - w rho0, FP64 moment sums (TG2, lumped) */
+                                    deviations f - w rho0, FP64 moments and equilibria;
+                                    with the consistent mass, FP32 inner CG solves refined
+                                    with FP64 residuals (lumped or CG, not Richardson) */
 } lbfem_scheme_settings;
 
 int lbfem_scheme_create(const lbfem_disc *disc, const lbfem_scheme_settings *settings, lbfem_scheme **scheme);

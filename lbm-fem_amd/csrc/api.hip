@@ -176,6 +176,10 @@ struct lbfem_scheme
   bool                  mixed = false;
   int                   cur   = 0;
   DevBuf<double>        rhs, x[2], r, pb[2], q, ax_tmp;
+  // the mixed form's consistent mass: FP32 right-hand side and inner CG vectors
+  DevBuf<float>         rhs32, g32r, r32, p32[2], q32, d32;
+  DevBuf<CgState>       state32;
+  CgState              *host_state32 = nullptr; // pinned
   DevBuf<CgState>       state;
   DevBuf<double>        partial;
   DevBuf<unsigned>      counter;
@@ -197,6 +201,8 @@ struct lbfem_scheme
       (void)hipEventDestroy(m.first);
     if (host_state)
       (void)hipHostFree(host_state);
+    if (host_state32)
+      (void)hipHostFree(host_state32);
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -291,13 +297,14 @@ namespace
 
   // CG on the vectors of W (include/lbfem/mass.h:204-211): returns LBFEM_OK or
   // LBFEM_ENOCONV; adds the iterations of all vectors to *its.
+  template <typename V>
   int
-  run_cg(const lbfem_disc &d, const CgWork &W, double *x, const double *b, double *r, double *p0, double *p1,
-         double *q, CgState *host_state, hipStream_t stream, int64_t *its, const int check_from)
+  run_cg(const lbfem_disc &d, const CgWork &W, V *x, const V *b, V *r, V *p0, V *p1, V *q, CgState *host_state,
+         hipStream_t stream, int64_t *its, const int check_from, const bool fail_at_cap = true)
   {
     const Geo &G = d.G;
     HIP_CHECK(cg_start_launch(G, W, x, b, r, stream));
-    double    *pb[2] = {p0, p1};
+    V         *pb[2] = {p0, p1};
     const auto done  = [&](bool &finished) -> int {
       HIP_CHECK(hipMemcpyAsync(host_state, W.state, W.nv * sizeof(CgState), hipMemcpyDeviceToHost, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
@@ -312,7 +319,7 @@ namespace
         return LBFEM_OK;
       for (int v = 0; v < W.nv; ++v)
         *its += host_state[v].its;
-      if (failed)
+      if (failed && fail_at_cap)
         return fail(LBFEM_ENOCONV,
                     "CG: no convergence within " + std::to_string(W.max_its) + " iterations (SolverControl::NoConvergence)");
       return LBFEM_OK;
@@ -407,6 +414,104 @@ namespace
     return LBFEM_OK;
   }
 
+  // The mixed form's consistent-mass solves of one sweep: iterative refinement
+  // with the residual r = A x - b and its norms in FP64 (x in FP64) and inner
+  // CG solves of A d = -r on FP32 vectors (FP64 scalars and reductions), until
+  // |r| <= cg_tolerance |b| for every vector, as the FP64 CG's stopping test; at
+  // most 30 refinements, else NoConvergence. An inner solve stops at half the
+  // gap to the outer tolerance, or at 1e-5 |r| (what FP32 reaches), whichever
+  // is looser: from the extrapolated start a few inner iterations then finish
+  // the solve. The iterations counted are the inner ones.
+  int
+  run_mixed_cg(lbfem_scheme &s, const CgWork &W64, double *x64, const double *b64, int64_t *its)
+  {
+    const Geo &G         = s.d->G;
+    const int  max_outer = 30;
+    for (int outer = 0;; ++outer)
+      {
+        HIP_CHECK(cg_start_launch(G, W64, x64, b64, s.r.p, s.stream));
+        HIP_CHECK(hipMemcpyAsync(s.host_state, W64.state, W64.nv * sizeof(CgState), hipMemcpyDeviceToHost, s.stream));
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        CgWork W32  = W64;
+        W32.state   = s.state32.p;
+        W32.nv      = 0;
+        W32.tol_rel = 1.;
+        unsigned mask = 0;
+        for (int v = 0; v < W64.nv; ++v)
+          {
+            if (s.host_state[v].status == 2)
+              return fail(LBFEM_ENOCONV, "mixed CG: the residual is not finite (SolverControl::NoConvergence)");
+            if (s.host_state[v].status != 0)
+              continue;
+            W32.block[W32.nv] = W64.block[v];
+            W32.vec[W32.nv]   = W64.vec[v];
+            ++W32.nv;
+            mask |= 1u << W64.block[v];
+            const double gap = 0.5 * s.host_state[v].tol / s.host_state[v].res;
+            W32.tol_rel      = std::min(W32.tol_rel, std::max(1e-5, gap));
+          }
+        if (W32.nv == 0)
+          return LBFEM_OK;
+        if (outer == max_outer)
+          return fail(LBFEM_ENOCONV, "mixed CG: no convergence within " + std::to_string(max_outer) +
+                                       " refinements (SolverControl::NoConvergence)");
+        HIP_CHECK(negate_to_float_launch(G, s.r.p, s.g32r.p, s.stream));
+        int64_t   inner = 0;
+        const int st    = run_cg<float>(*s.d, W32, s.d32.p, s.g32r.p, s.r32.p, s.p32[0].p, s.p32[1].p, s.q32.p,
+                                     s.host_state32, s.stream, &inner, 1, false);
+        if (st != LBFEM_OK)
+          return st;
+        *its += inner;
+        HIP_CHECK(refine_add_launch(G, x64, s.d32.p, mask, s.stream));
+      }
+  }
+
+  // the mixed form's increments of one sweep: FP32 advection (collision fused), the
+  // right-hand side to FP64, the refined consistent-mass solves into x (FP64)
+  int
+  increment_mixed(lbfem_scheme &s, const int sw, const bool collide)
+  {
+    const lbfem_disc &d  = *s.d;
+    const Geo        &G  = d.G;
+    const auto       &e  = s.sweeps[sw];
+    const bool        ll = s.set.scheme == LBFEM_LEELIN;
+    TileParams        S  = tile_params(s, e, ll);
+    S.collide            = collide ? 1 : 0;
+    float *gstar         = collide ? s.g32[1 - s.cur].p : nullptr;
+    s.mark(-1);
+    HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, s.variant(), G, S, s.g32[s.cur].p,
+                          (const float *)nullptr, gstar, s.rhs32.p, s.stream));
+    s.mark(1);
+    if (collide)
+      s.cur = 1 - s.cur;
+    HIP_CHECK(to_double_launch(G, s.rhs32.p, s.rhs.p, s.stream));
+    unsigned mask = 0;
+    for (int a = 0; a < NM; ++a)
+      if (moves(e, a))
+        mask |= 1u << a;
+    double *xs = s.x[sw].p;
+    if (s.extrapolate)
+      HIP_CHECK(extrapolate_launch(G, xs, s.x[1].p, mask, s.stream));
+    CgWork W = cg_work(d, s.state.p, s.partial.p, s.counter.p, s.set.mass);
+    W.nv     = 0;
+    for (int a = 0; a < NM; ++a)
+      if (mask & (1u << a))
+        {
+          W.block[W.nv]    = a;
+          W.vec[W.nv].e[0] = e[2 * (a + 1)];
+          W.vec[W.nv].e[1] = e[2 * (a + 1) + 1];
+          W.vec[W.nv].coef = s.tg3 * s.dt * s.dt;
+          ++W.nv;
+        }
+    int64_t   its = 0;
+    const int st  = run_mixed_cg(s, W, xs, s.rhs.p, &its);
+    s.cg_iterations += its;
+    if (st != LBFEM_OK)
+      return st;
+    s.mark(2);
+    return LBFEM_OK;
+  }
+
   // One launch of the tile kernel the scheme's step uses, with variant v, on the
   // scheme's buffers without changing its state: the fused step writes the
   // spare population buffer, the advection the right-hand side (and, for
@@ -426,6 +531,12 @@ namespace
         else
           HIP_CHECK(tile_launch(ll ? kStepLeeLin : kStepBardow, v, G, S, s.f[s.cur].p, (const double *)nullptr,
                                 s.f[1 - s.cur].p, (double *)nullptr, s.stream));
+      }
+    else if (s.mixed)
+      {
+        S.collide = ll ? 0 : 1;
+        HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, v, G, S, s.g32[s.cur].p, (const float *)nullptr,
+                              ll ? nullptr : s.g32[1 - s.cur].p, s.rhs32.p, s.stream));
       }
     else
       {
@@ -481,6 +592,31 @@ namespace
                                 (const double *)nullptr, s.f[1 - s.cur].p, (double *)nullptr, s.stream));
         s.mark(1);
         s.cur = 1 - s.cur;
+        return LBFEM_OK;
+      }
+    if (s.mixed) // the mixed form with the consistent mass
+      {
+        if (ll)
+          {
+            const int st = increment_mixed(s, 0, false);
+            if (st != LBFEM_OK)
+              return st;
+            HIP_CHECK(pc_from_f_launch(G, s.g32[s.cur].p, s.x[0].p, s.dt / s.lambda, s.stream));
+            s.mark(0);
+            return LBFEM_OK;
+          }
+        for (int sw = 0; sw < int(s.sweeps.size()); ++sw)
+          {
+            const int st = increment_mixed(s, sw, sw == 0);
+            if (st != LBFEM_OK)
+              return st;
+            unsigned mask = 0;
+            for (int a = 0; a < NM; ++a)
+              if (moves(s.sweeps[sw], a))
+                mask |= 1u << a;
+            HIP_CHECK(add_increments_launch(G, s.g32[s.cur].p, s.x[sw].p, mask, s.stream));
+            s.mark(0);
+          }
         return LBFEM_OK;
       }
     if (ll) // schemes.h:92-98
@@ -1050,8 +1186,8 @@ extern "C"
         return fail(LBFEM_EINVAL, "LeeLin streams with TG2 only");
       if (set->precision != LBFEM_FP64 && set->precision != LBFEM_MIXED)
         return fail(LBFEM_EINVAL, "unknown precision");
-      if (set->precision == LBFEM_MIXED && !(m.type == LBFEM_MASS_LUMPED && set->streaming == LBFEM_TG2))
-        return fail(LBFEM_EINVAL, "the mixed-precision form is available for TG2 with the lumped mass");
+      if (set->precision == LBFEM_MIXED && m.type == LBFEM_MASS_RICHARDSON)
+        return fail(LBFEM_EINVAL, "the mixed-precision form is available with the lumped or the consistent mass");
       auto s = std::make_unique<lbfem_scheme>();
       s->d   = d;
       s->set = *set;
@@ -1111,10 +1247,21 @@ extern "C"
       if (m.type == LBFEM_MASS_CG)
         {
           s->r.alloc(NM * st);
-          s->pb[0].alloc(NM * st);
-          s->pb[1].alloc(NM * st);
-          s->q.alloc(NM * st);
           HIP_CHECK(hipHostMalloc(&s->host_state, sizeof(CgState) * NM));
+          if (s->mixed)
+            {
+              for (DevBuf<float> *b : {&s->rhs32, &s->g32r, &s->r32, &s->p32[0], &s->p32[1], &s->q32, &s->d32})
+                b->alloc(NM * st);
+              HIP_CHECK(hipMemset(s->d32.p, 0, NM * st * sizeof(float)));
+              s->state32.alloc(NM);
+              HIP_CHECK(hipHostMalloc(&s->host_state32, sizeof(CgState) * NM));
+            }
+          else
+            {
+              s->pb[0].alloc(NM * st);
+              s->pb[1].alloc(NM * st);
+              s->q.alloc(NM * st);
+            }
         }
       *out = s.release();
       return LBFEM_OK;

@@ -54,8 +54,8 @@ namespace lbfem_amd
   extern template hipError_t tile_launch_degree<float, 3>(int, int, const Geo &, const TileParams &, const float *,
                                                           const float *, float *, float *, hipStream_t);
 
-  // One launch of the tile kernel over the mesh. double: every mode; float (the
-  // mixed form's FP32 deviations from the rest state): the fused-step modes.
+  // One launch of the tile kernel over the mesh: double (the FP64 form) or float
+  // (the mixed form's FP32 deviations from the rest state), every mode.
   template <typename T>
   inline hipError_t
   tile_launch(const int mode, const int variant, const Geo &G, const TileParams &S, const T *fin, const T *feq,
@@ -181,6 +181,20 @@ namespace lbfem_amd
   // x += alpha p; r += alpha q; res = |r|; stopping test; rho = r.D^-1 r, beta
   hipError_t cg_update_launch(const Geo &G, const CgWork &W, int it, double *x, double *r, const double *p,
                               const double *q, hipStream_t stream);
+  // the same three stages on FP32 vectors (the mixed form's inner solves; scalars in FP64)
+  hipError_t cg_start_launch(const Geo &G, const CgWork &W, const float *x, const float *b, float *r,
+                             hipStream_t stream);
+  hipError_t cg_direction_launch(const Geo &G, const CgWork &W, int it, const float *p_old, const float *r,
+                                 float *p_new, float *q, hipStream_t stream);
+  hipError_t cg_update_launch(const Geo &G, const CgWork &W, int it, float *x, float *r, const float *p,
+                              const float *q, hipStream_t stream);
+  // the mixed form's refinement: rhs to FP64, -r to FP32, x += d (d zeroed), and the nodal updates on FP32
+  // deviations from an FP64 increment
+  hipError_t to_double_launch(const Geo &G, const float *x, double *y, hipStream_t stream);
+  hipError_t negate_to_float_launch(const Geo &G, const double *x, float *y, hipStream_t stream);
+  hipError_t refine_add_launch(const Geo &G, double *x, float *d, unsigned mask, hipStream_t stream);
+  hipError_t add_increments_launch(const Geo &G, float *g, const double *x, unsigned mask, hipStream_t stream);
+  hipError_t pc_from_f_launch(const Geo &G, float *f, const double *incr, double theta, hipStream_t stream);
   // y = (M + c K_e) x for nv vectors (constrained rows the identity)
   hipError_t vmult_launch(const Geo &G, const CgWork &W, const double *x, double *y, hipStream_t stream);
   // x_guess = 2 x - x_prev, x_prev = x (the extrapolated start, streaming.h:76-80), masked

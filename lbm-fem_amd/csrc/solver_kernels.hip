@@ -237,15 +237,17 @@ namespace lbfem_amd
   // A = M + c K_e = Mx (My + c ey^2 Ky) + Kx (c ex^2 My) + Dx^T (c ex ey Dy) + Dx (c ex ey Dy^T).
   // The operands of all active vectors are staged in LDS at once (one barrier), then
   // each thread applies the bands at its RO owned nodes of one column.
-  template <int p, int TX, int TY, int KIND>
+  // V: the vector type, double (the FP64 solves and the refinement residuals) or
+  // float (the inner solves of the mixed form); scalars and partial sums in FP64.
+  template <typename V, int p, int TX, int TY, int KIND>
   __global__ void __launch_bounds__(kThreads)
-  k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const double *x,
-             const double *b, const double *r_in, double *y, double *y2)
+  k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const V *x,
+             const V *b, const V *r_in, V *y, V *y2)
   {
     constexpr int W_ = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
     constexpr int TR = kThreads / TX, RO = TY / TR;
     static_assert(TY % TR == 0, "tile shape");
-    __shared__ double X[1][HY][HX];
+    __shared__ V X[1][HY][HX];
     __shared__ double red[3][kThreads / 64];
 
     const int nt   = tiles_x * tiles_y;
@@ -267,9 +269,9 @@ namespace lbfem_amd
         return;
       }
     const unsigned act = 1u << vsel;
-    double         beta[NM];
+    V beta[NM];
     for (int v = 0; v < NM; ++v)
-      beta[v] = (v == vsel && KIND == 2 && it > 1) ? W.state[v].beta : 0.;
+      beta[v] = (v == vsel && KIND == 2 && it > 1) ? V(W.state[v].beta) : V(0);
 
     // stage the operands (KIND 2: the new direction) of the halo'd block
     for (int k = tid; k < NH; k += kThreads)
@@ -278,12 +280,12 @@ namespace lbfem_amd
         const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
         const bool    ok  = gi >= 0 && gj >= 0;
         const int64_t idx = ok ? node_index(G, gi, gj) : 0;
-        const double  d   = ok ? mass_diag_inv(G, gi, gj) : 1.;
-        double xv[NM], rv[NM];
+        const V       d   = ok ? V(mass_diag_inv(G, gi, gj)) : V(1);
+        V             xv[NM], rv[NM];
 #pragma unroll
         for (int v = 0; v < NM; ++v)
           {
-            xv[v] = rv[v] = 0.;
+            xv[v] = rv[v] = V(0);
             if (ok && (act & (1u << v)))
               {
                 const int64_t vb = int64_t(W.block[v]) * G.stride;
@@ -297,10 +299,10 @@ namespace lbfem_amd
         for (int v = 0; v < NM; ++v)
           if (act & (1u << v))
             {
-              double val = xv[v];
+              V val = xv[v];
               if constexpr (KIND == 2)
                 {
-                  const double z = d * rv[v];
+                  const V z = d * rv[v];
                   val            = it > 1 ? beta[v] * xv[v] - z : -z;
                 }
               X[0][ky][kx] = val;
@@ -308,24 +310,24 @@ namespace lbfem_amd
       }
     const int ii = tid % TX, jr = (tid / TX) * RO;
     const int gi = i0 + ii;
-    double    xc[4][W_], yc[RO][4][W_];
+    V         xc[4][W_], yc[RO][4][W_];
 #pragma unroll
     for (int o = 0; o < W_; ++o)
       {
         const bool in = gi <= P;
-        xc[0][o]      = in ? band<p>(G.ax.M, gi, o - p) : 0.;
-        xc[1][o]      = in ? band<p>(G.ax.D, gi, o - p) : 0.;
-        xc[2][o]      = in ? band<p>(G.ax.K, gi, o - p) : 0.;
-        xc[3][o]      = in ? band<p>(G.ax.DT, gi, o - p) : 0.;
+        xc[0][o]      = in ? V(band<p>(G.ax.M, gi, o - p)) : V(0);
+        xc[1][o]      = in ? V(band<p>(G.ax.D, gi, o - p)) : V(0);
+        xc[2][o]      = in ? V(band<p>(G.ax.K, gi, o - p)) : V(0);
+        xc[3][o]      = in ? V(band<p>(G.ax.DT, gi, o - p)) : V(0);
 #pragma unroll
         for (int r = 0; r < RO; ++r)
           {
             const int  gj = j0 + jr + r;
             const bool iy = gj <= P;
-            yc[r][0][o]   = iy ? band<p>(G.ay.M, gj, o - p) : 0.;
-            yc[r][1][o]   = iy ? band<p>(G.ay.D, gj, o - p) : 0.;
-            yc[r][2][o]   = iy ? band<p>(G.ay.K, gj, o - p) : 0.;
-            yc[r][3][o]   = iy ? band<p>(G.ay.DT, gj, o - p) : 0.;
+            yc[r][0][o]   = iy ? V(band<p>(G.ay.M, gj, o - p)) : V(0);
+            yc[r][1][o]   = iy ? V(band<p>(G.ay.D, gj, o - p)) : V(0);
+            yc[r][2][o]   = iy ? V(band<p>(G.ay.K, gj, o - p)) : V(0);
+            yc[r][3][o]   = iy ? V(band<p>(G.ay.DT, gj, o - p)) : V(0);
           }
       }
     __syncthreads();
@@ -339,21 +341,21 @@ namespace lbfem_amd
       {
         if (!(act & (1u << v)))
           continue;
-        const double  c = W.vec[v].coef, ex = W.vec[v].e[0], ey = W.vec[v].e[1];
+        const V       c = V(W.vec[v].coef), ex = V(W.vec[v].e[0]), ey = V(W.vec[v].e[1]);
         const int64_t vb = int64_t(W.block[v]) * G.stride;
-        const double(*Xv)[HX] = X[0];
-        double acc[RO];
+        const V(*Xv)[HX] = X[0];
+        V acc[RO];
 #pragma unroll
         for (int r = 0; r < RO; ++r)
-          acc[r] = 0.;
-        if (c == 0. || (ex == 0. && ey == 0.)) // the mass matrix: Mx (x) My
+          acc[r] = V(0);
+        if (c == V(0) || (ex == V(0) && ey == V(0))) // the mass matrix: Mx (x) My
           {
 #pragma unroll
             for (int cc = 0; cc < W_; ++cc)
 #pragma unroll
               for (int r = 0; r < RO; ++r)
                 {
-                  double s = 0.;
+                  V s = V(0);
 #pragma unroll
                   for (int o = 0; o < W_; ++o)
                     s += yc[r][0][o] * Xv[jr + r + o][ii + cc];
@@ -362,17 +364,17 @@ namespace lbfem_amd
           }
         else
           {
-            const double kyy = c * ey * ey, kxx = c * ex * ex, kxy = c * ex * ey;
+            const V kyy = c * ey * ey, kxx = c * ex * ex, kxy = c * ex * ey;
 #pragma unroll
             for (int cc = 0; cc < W_; ++cc)
 #pragma unroll
               for (int r = 0; r < RO; ++r)
                 {
-                  double My = 0., Ky = 0., Dy = 0., DTy = 0.;
+                  V My = V(0), Ky = V(0), Dy = V(0), DTy = V(0);
 #pragma unroll
                   for (int o = 0; o < W_; ++o)
                     {
-                      const double xv = Xv[jr + r + o][ii + cc];
+                      const V xv = Xv[jr + r + o][ii + cc];
                       My += yc[r][0][o] * xv;
                       Ky += yc[r][2][o] * xv;
                       Dy += yc[r][1][o] * xv;
@@ -390,8 +392,8 @@ namespace lbfem_amd
               continue;
             const int64_t idx   = node_index(G, gi, gj);
             const bool    slave = is_slave(G, gi, gj);
-            double        own   = Xv[jr + r + p][ii + p];
-            double        a     = acc[r];
+            V             own   = Xv[jr + r + p][ii + p];
+            V             a     = acc[r];
             if (slave) // constrained rows are the identity (MatrixFreeOperators::Base::vmult),
               {        // on the slave's own entry (the staged tile holds its master's)
                 if constexpr (KIND == 2)
@@ -404,18 +406,19 @@ namespace lbfem_amd
               y[vb + idx] = a;
             else if constexpr (KIND == 1)
               {
-                const double bb = b[vb + idx];
-                const double rr = a - bb;
-                y[vb + idx]     = rr;
-                part[3 * v] += rr * rr;
-                part[3 * v + 1] += bb * bb;
-                part[3 * v + 2] += rr * (mass_diag_inv(G, gi, gj) * rr);
+                const V bb  = b[vb + idx];
+                const V rr  = a - bb;
+                y[vb + idx] = rr;
+                const double r8 = double(rr), b8 = double(bb);
+                part[3 * v] += r8 * r8;
+                part[3 * v + 1] += b8 * b8;
+                part[3 * v + 2] += r8 * (mass_diag_inv(G, gi, gj) * r8);
               }
             else
               {
                 y[vb + idx]  = own; // p_new
                 y2[vb + idx] = a;   // q = A p_new
-                part[3 * v] += own * a;
+                part[3 * v] += double(own) * double(a);
               }
           }
       }
@@ -501,9 +504,10 @@ namespace lbfem_amd
 
   // x += alpha p; r += alpha q, with the partials of |r|^2 and r.D^-1 r
   // (the stopping test and beta follow in k_cg_finalize<3>)
+  template <typename V>
   __global__ void __launch_bounds__(kThreads)
-  k_cg_update(const Geo G, const CgWork W, double *__restrict__ x, double *__restrict__ r,
-              const double *__restrict__ p, const double *__restrict__ q)
+  k_cg_update(const Geo G, const CgWork W, V *__restrict__ x, V *__restrict__ r, const V *__restrict__ p,
+              const V *__restrict__ q)
   {
     __shared__ double red[2 * NM][kThreads / 64];
     const int64_t     n = int64_t(G.np) * G.np;
@@ -511,21 +515,21 @@ namespace lbfem_amd
 #pragma unroll
     for (int k = 0; k < 2 * NM; ++k)
       part[k] = 0.;
-    double  alpha[NM];
+    V       alpha[NM];
     int64_t off[NM];
     bool    act[NM];
 #pragma unroll
     for (int v = 0; v < NM; ++v)
       {
         act[v]   = v < W.nv && W.state[v].status == 0;
-        alpha[v] = act[v] ? W.state[v].alpha : 0.;
+        alpha[v] = act[v] ? V(W.state[v].alpha) : V(0);
         off[v]   = v < W.nv ? int64_t(W.block[v]) * G.stride : 0;
       }
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
       {
         const int    i = int(k % G.np), j = int(k / G.np);
         const double d = mass_diag_inv(G, i, j);
-        double       xv[NM], pv[NM], qv[NM], rv[NM];
+        V            xv[NM], pv[NM], qv[NM], rv[NM];
 #pragma unroll
         for (int v = 0; v < NM; ++v)
           if (act[v])
@@ -540,10 +544,11 @@ namespace lbfem_amd
           if (act[v])
             {
               x[off[v] + k]   = xv[v] + alpha[v] * pv[v];
-              const double rr = rv[v] + alpha[v] * qv[v];
-              r[off[v] + k]   = rr;
-              part[2 * v] += rr * rr;
-              part[2 * v + 1] += rr * (d * rr);
+              const V rr    = rv[v] + alpha[v] * qv[v];
+              r[off[v] + k] = rr;
+              const double r8 = double(rr);
+              part[2 * v] += r8 * r8;
+              part[2 * v + 1] += r8 * (d * r8);
             }
       }
     block_sum<2 * NM>(part, red);
@@ -666,10 +671,10 @@ namespace lbfem_amd
     return hipGetLastError();
   }
 
-  template <int KIND>
+  template <int KIND, typename V>
   static hipError_t
-  operator_launch(const Geo &G, const CgWork &W, const int it, const double *x, const double *b, const double *r,
-                  double *y, double *y2, hipStream_t s)
+  operator_launch(const Geo &G, const CgWork &W, const int it, const V *x, const V *b, const V *r, V *y, V *y2,
+                  hipStream_t s)
   {
     constexpr int TX = 32, TY = 16;
     const int     tx = (G.P + 1 + TX - 1) / TX, ty = (G.P + 1 + TY - 1) / TY;
@@ -679,16 +684,16 @@ namespace lbfem_amd
     switch (G.p)
       {
         case 1:
-          hipLaunchKernelGGL((k_operator<1, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
-                             y, y2);
+          hipLaunchKernelGGL((k_operator<V, 1, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty,
+                             x, b, r, y, y2);
           break;
         case 2:
-          hipLaunchKernelGGL((k_operator<2, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
-                             y, y2);
+          hipLaunchKernelGGL((k_operator<V, 2, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty,
+                             x, b, r, y, y2);
           break;
         case 3:
-          hipLaunchKernelGGL((k_operator<3, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty, x, b, r,
-                             y, y2);
+          hipLaunchKernelGGL((k_operator<V, 3, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty,
+                             x, b, r, y, y2);
           break;
         default:
           return hipErrorInvalidValue;
@@ -698,30 +703,148 @@ namespace lbfem_amd
     return hipGetLastError();
   }
 
+  template <typename V>
+  static hipError_t
+  update_launch(const Geo &G, const CgWork &W, const int it, V *x, V *r, const V *p, const V *q, hipStream_t s)
+  {
+    int nb = nodal_blocks(G);
+    nb     = std::min(nb, std::min(W.max_blocks, 1024));
+    hipLaunchKernelGGL(k_cg_update<V>, dim3(nb), dim3(kThreads), 0, s, G, W, x, r, p, q);
+    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kThreads), 0, s, W, nb, it);
+    return hipGetLastError();
+  }
+
   hipError_t
   vmult_launch(const Geo &G, const CgWork &W, const double *x, double *y, hipStream_t s)
   {
-    return operator_launch<0>(G, W, 0, x, nullptr, nullptr, y, nullptr, s);
+    return operator_launch<0, double>(G, W, 0, x, nullptr, nullptr, y, nullptr, s);
   }
   hipError_t
   cg_start_launch(const Geo &G, const CgWork &W, const double *x, const double *b, double *r, hipStream_t s)
   {
-    return operator_launch<1>(G, W, 0, x, b, nullptr, r, nullptr, s);
+    return operator_launch<1, double>(G, W, 0, x, b, nullptr, r, nullptr, s);
   }
   hipError_t
   cg_direction_launch(const Geo &G, const CgWork &W, const int it, const double *p_old, const double *r,
                       double *p_new, double *q, hipStream_t s)
   {
-    return operator_launch<2>(G, W, it, p_old, nullptr, r, p_new, q, s);
+    return operator_launch<2, double>(G, W, it, p_old, nullptr, r, p_new, q, s);
   }
   hipError_t
   cg_update_launch(const Geo &G, const CgWork &W, const int it, double *x, double *r, const double *p, const double *q,
                    hipStream_t s)
   {
-    int nb = nodal_blocks(G);
-    nb     = std::min(nb, std::min(W.max_blocks, 1024));
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, G, W, x, r, p, q);
-    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kThreads), 0, s, W, nb, it);
+    return update_launch<double>(G, W, it, x, r, p, q, s);
+  }
+  hipError_t
+  cg_start_launch(const Geo &G, const CgWork &W, const float *x, const float *b, float *r, hipStream_t s)
+  {
+    return operator_launch<1, float>(G, W, 0, x, b, nullptr, r, nullptr, s);
+  }
+  hipError_t
+  cg_direction_launch(const Geo &G, const CgWork &W, const int it, const float *p_old, const float *r,
+                      float *p_new, float *q, hipStream_t s)
+  {
+    return operator_launch<2, float>(G, W, it, p_old, nullptr, r, p_new, q, s);
+  }
+  hipError_t
+  cg_update_launch(const Geo &G, const CgWork &W, const int it, float *x, float *r, const float *p, const float *q,
+                   hipStream_t s)
+  {
+    return update_launch<float>(G, W, it, x, r, p, q, s);
+  }
+
+  // ------------------------------------------------------------------ mixed-form helpers
+  // y = s x over NM blocks (the right-hand side to FP64, the negated residual to FP32)
+  template <typename A, typename B>
+  __global__ void __launch_bounds__(kThreads)
+  k_convert(const Geo G, const A *__restrict__ x, B *__restrict__ y, const double sgn)
+  {
+    const int64_t n = int64_t(NM) * G.stride;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      y[k] = B(sgn * double(x[k]));
+  }
+
+  // x += d for the vectors in mask (the refinement's correction), d zeroed for the next inner solve
+  __global__ void __launch_bounds__(kThreads)
+  k_refine_add(const Geo G, double *__restrict__ x, float *__restrict__ d, const unsigned mask)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+#pragma unroll
+      for (int a = 0; a < NM; ++a)
+        if (mask & (1u << a))
+          {
+            x[a * G.stride + k] += double(d[a * G.stride + k]);
+            d[a * G.stride + k] = 0.f;
+          }
+  }
+
+  // Bardow: g_{a+1} += x_a on FP32 deviations (the sum in FP64, rounded once)
+  __global__ void __launch_bounds__(kThreads)
+  k_add_increments_mixed(const Geo G, float *g, const double *x, const unsigned mask)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+#pragma unroll
+      for (int a = 0; a < NM; ++a)
+        if (mask & (1u << a))
+          g[(a + 1) * G.stride + k] = float(double(g[(a + 1) * G.stride + k]) + x[a * G.stride + k]);
+  }
+
+  // Lee-Lin after the consistent-mass solve, on FP32 deviations: f <- PC(f, feq(f), x)
+  __global__ void __launch_bounds__(kThreads)
+  k_pc_from_f_mixed(const Geo G, float *f, const double *incr, const double theta)
+  {
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        float     v[Q], fe[Q], x[NM];
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          v[a] = f[a * G.stride + k];
+#pragma unroll
+        for (int a = 0; a < NM; ++a)
+          x[a] = float(incr[a * G.stride + k]);
+        double     wx = 0., wy = 0.;
+        const bool wall = wall_of(G, i, j, wx, wy);
+        equilibrium_node(v, fe, wall, wx, wy);
+        predictor_corrector_node(v, fe, x, theta, wall, wx, wy);
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          f[a * G.stride + k] = v[a];
+      }
+  }
+
+  hipError_t
+  to_double_launch(const Geo &G, const float *x, double *y, hipStream_t s)
+  {
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, y, 1.);
+    return hipGetLastError();
+  }
+  hipError_t
+  negate_to_float_launch(const Geo &G, const double *x, float *y, hipStream_t s)
+  {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, y, -1.);
+    return hipGetLastError();
+  }
+  hipError_t
+  refine_add_launch(const Geo &G, double *x, float *d, const unsigned mask, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_refine_add, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, d, mask);
+    return hipGetLastError();
+  }
+  hipError_t
+  add_increments_launch(const Geo &G, float *g, const double *x, const unsigned mask, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_add_increments_mixed, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, g, x, mask);
+    return hipGetLastError();
+  }
+  hipError_t
+  pc_from_f_launch(const Geo &G, float *f, const double *incr, const double theta, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_pc_from_f_mixed, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, f, incr, theta);
     return hipGetLastError();
   }
   hipError_t

@@ -1,6 +1,6 @@
 // Tile kernels (single mesh and batched) of the mixed form (FP32 deviations) on
-// Q3 meshes: every variant of every mode this form uses, in a translation unit
-// of their own so that the kernels compile in parallel.
+// Q3 meshes: every variant of every mode, in a translation unit of their own so
+// that the kernels compile in parallel.
 #include "tile_kernel.hpp"
 
 namespace lbfem_amd

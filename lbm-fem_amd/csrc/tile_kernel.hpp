@@ -745,7 +745,7 @@ namespace lbfem_amd
       }
   }
 
-  // every mode (double) or the fused-step modes (float, the mixed form) of one degree
+  // every mode of one degree
   template <typename T, int p>
   hipError_t
   tile_launch_degree(const int mode, const int variant, const Geo &G, const TileParams &S, const T *fin,
@@ -758,15 +758,9 @@ namespace lbfem_amd
         case kStepLeeLin:
           return launch_variant<T, p, kStepLeeLin>(variant, G, S, fin, feq, out, rhs, stream);
         case kAdvectBardow:
-          if constexpr (sizeof(T) == 8)
-            return launch_variant<T, p, kAdvectBardow>(variant, G, S, fin, feq, out, rhs, stream);
-          else
-            return hipErrorInvalidValue;
+          return launch_variant<T, p, kAdvectBardow>(variant, G, S, fin, feq, out, rhs, stream);
         case kAdvectLeeLin:
-          if constexpr (sizeof(T) == 8)
-            return launch_variant<T, p, kAdvectLeeLin>(variant, G, S, fin, feq, out, rhs, stream);
-          else
-            return hipErrorInvalidValue;
+          return launch_variant<T, p, kAdvectLeeLin>(variant, G, S, fin, feq, out, rhs, stream);
         default:
           return hipErrorInvalidValue;
       }

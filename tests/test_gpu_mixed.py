@@ -1,6 +1,8 @@
-"""The mixed-precision form (FP32 deviations f - w rho0, FP64 moment sums)
-against the FP64 form and the oracle. Tolerance: relative L2 velocity error
-<= 1e-5 against FP64 after the run, as the north-star accuracy bar states."""
+"""The mixed-precision form (FP32 deviations f - w rho0; FP64 moments,
+equilibria and corrector; with the consistent mass, FP32 inner CG solves refined
+with FP64 residuals to the same tolerance as the FP64 CG) against the FP64 form
+and the oracle. Tolerance: relative L2 velocity error <= 1e-5 against FP64 after
+the run."""
 import numpy as np
 import pytest
 
@@ -23,11 +25,12 @@ def l2_rel(u, v, ref_u, ref_v, w):
     return np.sqrt((w * ((u - ref_u) ** 2 + (v - ref_v) ** 2)).sum() / (w * (ref_u ** 2 + ref_v ** 2)).sum())
 
 
-def run(case, N, p, scheme, precision, steps, Re=100.0, mach=0.1, cfl=0.25, modes=(1, 1)):
+def run(case, N, p, scheme, precision, steps, Re=100.0, mach=0.1, cfl=0.25, modes=(1, 1), mass=A.MASS_LUMPED,
+        cg_tol=1e-10):
     tc = A.test_cases.make(case, mach=mach, modes=modes, stretch=1.2)
     tc.set_reynolds(Re)
     d = A.Discretization(tc, N, p)
-    s = (A.Bardow if scheme == A.BARDOW else A.LeeLin)(d, mass=A.MASS_LUMPED, precision=precision)
+    s = (A.Bardow if scheme == A.BARDOW else A.LeeLin)(d, mass=mass, precision=precision, cg_tolerance=cg_tol)
     dt = d.time_step(cfl)
     s.set_time_step(dt, tc.relaxation_time())
     s.set_initial_populations(tc)
@@ -94,9 +97,72 @@ def test_mixed_decay_matches_analytic():
     assert abs(e32 - e64) < 1e-5, (e32, e64)
 
 
-def test_mixed_rejected_outside_lumped_tg2():
+def test_mixed_rejected_with_richardson():
     tc = A.test_cases.make("tgv", mach=0.1)
     tc.set_reynolds(100.0)
     d = A.Discretization(tc, 8, 1)
     with pytest.raises(A.LbfemError):
-        A.Bardow(d, mass=A.MASS_CG, precision=A.MIXED)
+        A.Bardow(d, mass=A.MASS_RICHARDSON, precision=A.MIXED)
+
+
+# ------------------------------------------------------------------ consistent mass (refined CG)
+@pytest.mark.parametrize("case,p,scheme,streaming", [("tgv", 1, A.BARDOW, A.TG2), ("cavity", 1, A.LEELIN, A.TG2),
+                                                     ("cavity", 2, A.BARDOW, A.TG2), ("couette", 2, A.LEELIN, A.TG2),
+                                                     ("tgv", 1, A.BARDOW, A.TG3), ("tgv", 2, A.BARDOW, A.TG3_SPLIT)])
+def test_mixed_cg_one_step_vs_oracle(case, p, scheme, streaming):
+    N = {1: 20, 2: 10}[p]
+    tc = A.test_cases.make(case, mach=0.1, modes=(1, 1), stretch=1.2)
+    tc.set_reynolds(100.0)
+    d = A.Discretization(tc, N, p)
+    od = O.Disc(N, p, {"tgv": O.TGV, "couette": O.COUETTE, "cavity": O.CAVITY}[case], mach=0.1, modes=(1, 1),
+                stretch=1.2 if case == "cavity" else 0.0, reynolds=100.0)
+    dt, lam = od.time_step(0.25), od.relaxation_time
+    s = (A.Bardow if scheme == A.BARDOW else A.LeeLin)(d, streaming=streaming, mass=A.MASS_CG, cg_tolerance=1e-10,
+                                                      precision=A.MIXED)
+    s.set_time_step(dt, lam)
+    s.set_initial_populations()
+    o = O.Scheme(od, O.BARDOW if scheme == A.BARDOW else O.LEELIN, streaming, O.MASS_CG, cg_tol=1e-10)
+    o.set_time_step(dt, lam)
+    o.set_initial_populations()
+    f0 = o.populations()
+    s.step(2)
+    o.step(2)
+    f1 = o.populations()
+    dev = np.abs(f0 - O.W[:, None]).max()
+    err = np.abs(s.f - f1).max()
+    assert err < 2e-6 * max(dev, np.abs(f1 - f0).max()), (err, dev)
+    assert s.cg_iterations > 0
+
+
+@pytest.mark.parametrize("case,N,p,scheme,steps", [
+    ("tgv", 48, 1, A.BARDOW, 600),
+    ("cavity", 40, 1, A.LEELIN, 600),
+    ("cavity", 20, 2, A.BARDOW, 600),
+])
+def test_mixed_cg_velocity_error_vs_fp64(case, N, p, scheme, steps):
+    d, s64, dt, tc = run(case, N, p, scheme, A.FP64, steps, mass=A.MASS_CG)
+    _, s32, _, _ = run(case, N, p, scheme, A.MIXED, steps, mass=A.MASS_CG)
+    _, u64, v64 = velocity(s64.f)
+    _, u32, v32 = velocity(s32.f)
+    indep = ~d.constrained()
+    w = d.lumped_mass()[indep]
+    err = l2_rel(u32[indep], v32[indep], u64[indep], v64[indep], w)
+    print(f"CG {case} N={N} p={p} steps={steps}: relative L2 velocity error mixed vs FP64 = {err:.2e}; "
+          f"CG iterations per solve fp64 {s64.cg_iterations / (8 * steps):.2f}, mixed inner "
+          f"{s32.cg_iterations / (8 * steps):.2f}")
+    assert err < 1e-5, err
+
+
+def test_mixed_cg_decay_matches_analytic():
+    """TGV (1,1), Bardow, consistent mass: the mixed form follows exp(-nu (k1^2 + k2^2) t) as FP64 does."""
+    d, s32, dt, tc = run("tgv", 48, 1, A.BARDOW, A.MIXED, 500, mass=A.MASS_CG)
+    _, s64, _, _ = run("tgv", 48, 1, A.BARDOW, A.FP64, 500, mass=A.MASS_CG)
+    xy = d.node
+    indep = ~d.constrained()
+    w = d.lumped_mass()[indep]
+    _, ue, ve = tc.exact(xy[:, 0], xy[:, 1], 500 * dt)
+    _, u32, v32 = velocity(s32.f)
+    _, u64, v64 = velocity(s64.f)
+    e32 = l2_rel(u32[indep], v32[indep], ue[indep], ve[indep], w)
+    e64 = l2_rel(u64[indep], v64[indep], ue[indep], ve[indep], w)
+    assert abs(e32 - e64) < 1e-5, (e32, e64)
