@@ -15,6 +15,7 @@ CHILD = r"""
 import json, os, sys
 import numpy as np
 sys.path.insert(0, os.path.join(sys.argv[1], "oracle"))
+import oracle as O
 out = {}
 for case, scheme, stretch in ((O.CAVITY, O.LEELIN, 1.2), (O.TGV, O.BARDOW, 0.0), (O.COUETTE, O.BARDOW, 0.0)):
     d = O.Disc(12, 2, case, mach=0.1, modes=(1, 1), stretch=stretch, reynolds=200.0)
