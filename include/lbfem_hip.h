@@ -177,8 +177,9 @@ int lbfem_scheme_stage_times(const lbfem_scheme *scheme, double *seconds4);
 /* No counterpart in the reference (its deal.II kernels have no launch shape):
  * the shape of the scheme's tile kernel, a TileVariant (0: 32 x 8 tiles of
  * 256 threads, 1: 32 x 16 tiles of 512, 2 and 3: 32 x 8 tiles with a node's
- * populations split over 2 or 4 threads, 4: 16 x 8 tiles split over 4; -1
- * restores the size-based default).
+ * populations split over 2 or 4 threads, 4: 16 x 8 tiles split over 4, 5 and
+ * 6: 40 x 8 and 48 x 8 tiles of 320 and 384 threads; -1 restores the size-based
+ * default).
  * Every variant gives bitwise identical populations. */
 int lbfem_scheme_set_tile_variant(lbfem_scheme *scheme, int variant);
 int lbfem_scheme_get_tile_variant(const lbfem_scheme *scheme, int *variant);

@@ -35,7 +35,9 @@ namespace lbfem_amd
     kTileSplit2   = 2, // 32 x 8 nodes, 512 threads, the populations of a node split over 2 threads
     kTileSplit4   = 3, // 32 x 8 nodes, 1024 threads, split over 4
     kTileSmall4   = 4, // 16 x 8 nodes, 512 threads, split over 4
-    kTileVariants = 5
+    kTileWide40   = 5, // 40 x 8 nodes, 320 threads: 231 tiles (one per CU) on a 257^2 node grid
+    kTileWide48   = 6, // 48 x 8 nodes, 384 threads
+    kTileVariants = 7
   };
 
   template <typename T, int p>
@@ -78,7 +80,7 @@ namespace lbfem_amd
   inline void
   tile_dims(const int variant, int &tx, int &ty)
   {
-    tx = variant == kTileSmall4 ? 16 : 32;
+    tx = variant == kTileSmall4 ? 16 : variant == kTileWide40 ? 40 : variant == kTileWide48 ? 48 : 32;
     ty = variant == kTileTall ? 16 : 8;
   }
 

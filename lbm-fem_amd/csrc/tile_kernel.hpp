@@ -683,7 +683,9 @@ namespace lbfem_amd
 
   // The variants (TileVariant): kTileWide 32 x 8 tiles of 256 threads, kTileTall
   // 32 x 16 tiles of 512 threads (less halo per node), kTileSplit2 / kTileSplit4
-  // 32 x 8 tiles with 2 or 4 threads per node, kTileSmall4 16 x 8 tiles with 4.
+  // 32 x 8 tiles with 2 or 4 threads per node, kTileSmall4 16 x 8 tiles with 4,
+  // kTileWide40 / kTileWide48 40 x 8 and 48 x 8 tiles (5 and 6 wavefronts) whose
+  // tile counts fit the 256 CUs on meshes just past a power of two nodes.
   template <typename T, int p, int MODE>
   hipError_t
   launch_variant(const int variant, const Geo &G, const TileParams &S, const T *fin, const T *feq, T *out, T *rhs,
@@ -701,6 +703,10 @@ namespace lbfem_amd
           return launch_tile<tile::Cfg<T, p, 32, 8, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileSmall4:
           return launch_tile<tile::Cfg<T, p, 16, 8, 4, MODE>>(G, S, fin, feq, out, rhs, stream);
+        case kTileWide40:
+          return launch_tile<tile::Cfg<T, p, 40, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
+        case kTileWide48:
+          return launch_tile<tile::Cfg<T, p, 48, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
         default:
           return hipErrorInvalidValue;
       }
@@ -723,6 +729,10 @@ namespace lbfem_amd
           return launch_batch<tile::Cfg<T, p, 32, 8, 4, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
         case kTileSmall4:
           return launch_batch<tile::Cfg<T, p, 16, 8, 4, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        case kTileWide40:
+          return launch_batch<tile::Cfg<T, p, 40, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        case kTileWide48:
+          return launch_batch<tile::Cfg<T, p, 48, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }
