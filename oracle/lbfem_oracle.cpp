@@ -194,6 +194,7 @@ namespace
     int    id    = ORA_TGV;
     double L     = 1., rho0 = 1., U0 = 0., nu = 0.;
     double n1    = 1, n2 = 4, stretch = 0.;
+    double distort = 0.; // tgv: x -> x + eps L sin(2 pi x / L) sin(2 pi y / L) (1, 1)
 
     int
     n_periodic() const
@@ -249,6 +250,16 @@ namespace
     transforms() const
     {
       return id == ORA_CAVITY && stretch > 0.;
+    }
+    // TaylorGreenVortex::transform_mesh (test_cases.h:53-65), applied to a vertex
+    void
+    distort_vertex(double &x, double &y) const
+    {
+      if (id != ORA_TGV || distort == 0.)
+        return;
+      const double d = distort * L * std::sin(2 * PI * x / L) * std::sin(2 * PI * y / L);
+      x += d;
+      y += d;
     }
     std::optional<std::array<double, 2>>
     wall_velocity(const double x, const double y) const
@@ -324,8 +335,14 @@ struct ora_disc
   Basis1D  B{1};
   double   h_min = 0.;
 
-  std::vector<double> xv, yv; // vertices
-  std::vector<double> xn, yn; // node coordinates per direction
+  std::vector<double> xv, yv; // vertices per direction (before a distortion)
+  std::vector<double> xn, yn; // node coordinates per direction (tensor meshes)
+  // General quadrilaterals (a distorted mesh, or forced): the cells are the
+  // MappingQ1 images of their four vertices, with the Jacobian evaluated per
+  // quadrature point (discretization.h:121, 179). VX, VY: (N+1)^2 vertices.
+  bool                general = false;
+  std::vector<double> VX, VY;
+  std::vector<double> XN, YN; // node coordinates, per node
   std::vector<double> ml, diag;
   std::vector<int>    constrained;
   std::vector<int>    wall_node;
@@ -365,6 +382,31 @@ struct ora_disc
   hy(const int cy) const
   {
     return yv[cy + 1] - yv[cy];
+  }
+  // The Jacobian of the bilinear map of cell (cx, cy) at (xi, eta) in [0,1]^2,
+  // J[d][e] = dx_d / dxi_e, and its inverse K[e][d] = dxi_e / dx_d and det J.
+  void
+  jacobian(const int cx, const int cy, const double xi, const double eta, double K[2][2], double &det) const
+  {
+    const int    v00 = cy * (N + 1) + cx, v10 = v00 + 1, v01 = v00 + N + 1, v11 = v01 + 1;
+    const double xxi  = (1. - eta) * (VX[v10] - VX[v00]) + eta * (VX[v11] - VX[v01]);
+    const double yxi  = (1. - eta) * (VY[v10] - VY[v00]) + eta * (VY[v11] - VY[v01]);
+    const double xeta = (1. - xi) * (VX[v01] - VX[v00]) + xi * (VX[v11] - VX[v10]);
+    const double yeta = (1. - xi) * (VY[v01] - VY[v00]) + xi * (VY[v11] - VY[v10]);
+    det               = xxi * yeta - xeta * yxi;
+    const double id   = 1. / det;
+    K[0][0]           = yeta * id;
+    K[0][1]           = -xeta * id;
+    K[1][0]           = -yxi * id;
+    K[1][1]           = xxi * id;
+  }
+  // the bilinear map of cell (cx, cy) (MappingQ1::transform_unit_to_real_cell)
+  void
+  map_point(const int cx, const int cy, const double xi, const double eta, double &x, double &y) const
+  {
+    const int v00 = cy * (N + 1) + cx, v10 = v00 + 1, v01 = v00 + N + 1, v11 = v01 + 1;
+    x = (1. - eta) * ((1. - xi) * VX[v00] + xi * VX[v10]) + eta * ((1. - xi) * VX[v01] + xi * VX[v11]);
+    y = (1. - eta) * ((1. - xi) * VY[v00] + xi * VY[v10]) + eta * ((1. - xi) * VY[v01] + xi * VY[v11]);
   }
   void
   gather(const double *v, const int cx, const int cy, double *G) const
@@ -467,11 +509,11 @@ struct ora_disc
       }
   }
 
-  void reinit();
+  void reinit(bool force_general);
 };
 
 void
-ora_disc::reinit()
+ora_disc::reinit(const bool force_general)
 {
   // discretization.h:81-100: subdivided_hyper_cube(N, 0, L), periodic faces, transform
   const double L = tc.L;
@@ -506,6 +548,38 @@ ora_disc::reinit()
   xn[p * N] = xv[N];
   yn[p * N] = yv[N];
 
+  // general quadrilaterals: the vertices moved by the case's map, nodes through
+  // each cell's bilinear map (a node on a cell edge from the lower-left cell)
+  general = force_general || (tc.id == ORA_TGV && tc.distort != 0.);
+  XN.assign((size_t)np * np, 0.);
+  YN.assign((size_t)np * np, 0.);
+  if (general)
+    {
+      VX.assign((size_t)(N + 1) * (N + 1), 0.);
+      VY.assign((size_t)(N + 1) * (N + 1), 0.);
+      for (int cy = 0; cy <= N; ++cy)
+        for (int cx = 0; cx <= N; ++cx)
+          {
+            double x = xv[cx], y = yv[cy];
+            tc.distort_vertex(x, y);
+            VX[cy * (N + 1) + cx] = x;
+            VY[cy * (N + 1) + cx] = y;
+          }
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < np; ++i)
+          {
+            const int cx = std::min(i / p, N - 1), cy = std::min(j / p, N - 1);
+            map_point(cx, cy, B.node[i - p * cx], B.node[j - p * cy], XN[j * np + i], YN[j * np + i]);
+          }
+    }
+  else
+    for (int j = 0; j < np; ++j)
+      for (int i = 0; i < np; ++i)
+        {
+          XN[j * np + i] = xn[i];
+          YN[j * np + i] = yn[j];
+        }
+
   constrained.assign(n, 0);
   for (int j = 0; j < np; ++j)
     for (int i = 0; i < np; ++i)
@@ -521,7 +595,15 @@ ora_disc::reinit()
   for (int cy = 0; cy < N; ++cy)
     for (int cx = 0; cx < N; ++cx)
       {
-        const double J = hx(cx) * hy(cy);
+        const double J = general ? 0. : hx(cx) * hy(cy);
+        double       detq[25];
+        if (general)
+          for (int qy = 0; qy < nn; ++qy)
+            for (int qx = 0; qx < nn; ++qx)
+              {
+                double K[2][2];
+                jacobian(cx, cy, B.qx[qx], B.qx[qy], K, detq[qy * nn + qx]);
+              }
         for (int ky = 0; ky < nn; ++ky)
           for (int kx = 0; kx < nn; ++kx)
             {
@@ -530,7 +612,7 @@ ora_disc::reinit()
                 for (int qx = 0; qx < nn; ++qx)
                   {
                     const double phi = B.S[qx * nn + kx] * B.S[qy * nn + ky];
-                    const double jxw = J * B.qw[qx] * B.qw[qy];
+                    const double jxw = (general ? detq[qy * nn + qx] : J) * B.qw[qx] * B.qw[qy];
                     sl += phi * jxw; // sum_l phi_l = 1 at every point
                     sd += phi * phi * jxw;
                   }
@@ -550,7 +632,7 @@ ora_disc::reinit()
   wall_of.assign(n, -1);
   for (int j = 0; j < np; ++j)
     for (int i = 0; i < np; ++i)
-      if (const auto u = tc.wall_velocity(xn[i], yn[j]))
+      if (const auto u = tc.wall_velocity(XN[j * np + i], YN[j * np + i]))
         {
           wall_of[j * np + i] = (int)wall_node.size();
           wall_node.push_back(j * np + i);
@@ -628,6 +710,96 @@ namespace
             {
               a += B.S[qx * n + kx] * v[qy * n + qx] + B.D[qx * n + kx] * Fx[qy * n + qx] / hx;
               b += B.S[qx * n + kx] * Fy[qy * n + qx] / hy;
+            }
+          Vx[qy * n + kx] = a;
+          Wx[qy * n + kx] = b;
+        }
+    for (int ky = 0; ky < n; ++ky)
+      for (int kx = 0; kx < n; ++kx)
+        {
+          double a = 0.;
+          for (int qy = 0; qy < n; ++qy)
+            a += B.S[qy * n + ky] * Vx[qy * n + kx] + B.D[qy * n + ky] * Wx[qy * n + kx];
+          R[ky * n + kx] = a;
+        }
+  }
+
+  // General quadrilaterals: per quadrature point the inverse Jacobian K[e][d]
+  // = dxi_e / dx_d and JxW = det J w_qx w_qy of the cell's bilinear map
+  // (MappingQ1 with the Jacobians stored per point, docs/method.md:216-224).
+  struct CellGeo
+  {
+    double K[25][2][2];
+    double jxw[25];
+  };
+  void
+  cell_geometry(const ora_disc &d, const int cx, const int cy, CellGeo &g)
+  {
+    const int n = d.p + 1;
+    for (int qy = 0; qy < n; ++qy)
+      for (int qx = 0; qx < n; ++qx)
+        {
+          const int q = qy * n + qx;
+          double    det;
+          d.jacobian(cx, cy, d.B.qx[qx], d.B.qx[qy], g.K[q], det);
+          g.jxw[q] = det * d.B.qw[qx] * d.B.qw[qy];
+        }
+  }
+  // gradients at the Gauss points: reference derivatives, then grad = K^T grad_xi
+  void
+  cell_gradients_general(const ora_disc &d, const double *G, const CellGeo &g, double *gx, double *gy)
+  {
+    const int      n = d.p + 1;
+    const Basis1D &B = d.B;
+    double         Ax[25], Bx[25];
+    for (int ky = 0; ky < n; ++ky)
+      for (int qx = 0; qx < n; ++qx)
+        {
+          double a = 0., b = 0.;
+          for (int kx = 0; kx < n; ++kx)
+            {
+              a += B.D[qx * n + kx] * G[ky * n + kx];
+              b += B.S[qx * n + kx] * G[ky * n + kx];
+            }
+          Ax[ky * n + qx] = a;
+          Bx[ky * n + qx] = b;
+        }
+    for (int qy = 0; qy < n; ++qy)
+      for (int qx = 0; qx < n; ++qx)
+        {
+          double uxi = 0., ueta = 0.;
+          for (int ky = 0; ky < n; ++ky)
+            {
+              uxi += B.S[qy * n + ky] * Ax[ky * n + qx];
+              ueta += B.D[qy * n + ky] * Bx[ky * n + qx];
+            }
+          const int q = qy * n + qx;
+          gx[q]       = g.K[q][0][0] * uxi + g.K[q][1][0] * ueta;
+          gy[q]       = g.K[q][0][1] * uxi + g.K[q][1][1] * ueta;
+        }
+  }
+  // r_loc = sum_q [phi v + grad phi . F] with grad phi . F = grad_xi phi . (K F);
+  // v, Fx, Fy already multiplied by JxW
+  void
+  cell_integrate_general(const ora_disc &d, const double *v, const double *Fx, const double *Fy, const CellGeo &g,
+                         double *R)
+  {
+    const int      n = d.p + 1;
+    const Basis1D &B = d.B;
+    double         Fxi[25], Feta[25], Vx[25], Wx[25];
+    for (int q = 0; q < n * n; ++q)
+      {
+        Fxi[q]  = g.K[q][0][0] * Fx[q] + g.K[q][0][1] * Fy[q];
+        Feta[q] = g.K[q][1][0] * Fx[q] + g.K[q][1][1] * Fy[q];
+      }
+    for (int qy = 0; qy < n; ++qy)
+      for (int kx = 0; kx < n; ++kx)
+        {
+          double a = 0., b = 0.;
+          for (int qx = 0; qx < n; ++qx)
+            {
+              a += B.S[qx * n + kx] * v[qy * n + qx] + B.D[qx * n + kx] * Fxi[qy * n + qx];
+              b += B.S[qx * n + kx] * Feta[qy * n + qx];
             }
           Vx[qy * n + kx] = a;
           Wx[qy * n + kx] = b;
@@ -765,6 +937,31 @@ namespace
         for (int cy = 0; cy < N; ++cy)
           for (int cx = 0; cx < N; ++cx)
             {
+              if (d.general) // the same point operation with per-point Jacobians
+                {
+                  CellGeo g;
+                  cell_geometry(d, cx, cy, g);
+                  d.gather(f + (al + 1) * n, cx, cy, G.data());
+                  cell_gradients_general(d, G.data(), g, gx.data(), gy.data());
+                  if (leelin)
+                    {
+                      d.gather(feq + (al + 1) * n, cx, cy, H.data());
+                      cell_gradients_general(d, H.data(), g, hgx.data(), hgy.data());
+                    }
+                  for (int q = 0; q < nq; ++q)
+                    {
+                      const double jxw = g.jxw[q];
+                      const double s   = ex * gx[q] + ey * gy[q];
+                      double       vv  = a * s;
+                      if (leelin)
+                        vv += b * (ex * hgx[q] + ey * hgy[q]);
+                      v[q]  = vv * jxw;
+                      Fx[q] = (c * ex) * s * jxw;
+                      Fy[q] = (c * ey) * s * jxw;
+                    }
+                  cell_integrate_general(d, v.data(), Fx.data(), Fy.data(), g, Rc.data() + (size_t)(cy * N + cx) * nq);
+                  continue;
+                }
               const double hx = d.hx(cx), hy = d.hy(cy);
               d.gather(f + (al + 1) * n, cx, cy, G.data());
               cell_gradients(d, G.data(), hx, hy, gx.data(), gy.data());
@@ -827,13 +1024,21 @@ namespace
                   s += B.S[qy * nn + ky] * T[ky * nn + qx];
                 u[qy * nn + qx] = s;
               }
+          CellGeo g;
+          if (d.general)
+            cell_geometry(d, cx, cy, g);
           if (c != 0.)
-            cell_gradients(d, G.data(), hx, hy, gx.data(), gy.data());
+            {
+              if (d.general)
+                cell_gradients_general(d, G.data(), g, gx.data(), gy.data());
+              else
+                cell_gradients(d, G.data(), hx, hy, gx.data(), gy.data());
+            }
           for (int qy = 0; qy < nn; ++qy)
             for (int qx = 0; qx < nn; ++qx)
               {
                 const int    q   = qy * nn + qx;
-                const double jxw = hx * hy * B.qw[qx] * B.qw[qy];
+                const double jxw = d.general ? g.jxw[q] : hx * hy * B.qw[qx] * B.qw[qy];
                 v[q]             = u[q] * jxw;
                 if (c != 0.)
                   {
@@ -844,7 +1049,10 @@ namespace
                 else
                   Fx[q] = Fy[q] = 0.;
               }
-          cell_integrate(d, v.data(), Fx.data(), Fy.data(), hx, hy, Rc.data() + (size_t)(cy * N + cx) * nq);
+          if (d.general)
+            cell_integrate_general(d, v.data(), Fx.data(), Fy.data(), g, Rc.data() + (size_t)(cy * N + cx) * nq);
+          else
+            cell_integrate(d, v.data(), Fx.data(), Fy.data(), hx, hy, Rc.data() + (size_t)(cy * N + cx) * nq);
         }
     d.assemble(Rc.data(), y);
     for (int i = 0; i < n; ++i)
@@ -988,20 +1196,34 @@ struct ora_scheme
 
 extern "C"
 {
+  // distort: the TGV mesh distortion eps (test_cases.h:53-65); general: 1 to take
+  // the general-quadrilateral path on any mesh (a tensor mesh then gives the
+  // tensor path's operators up to round-off)
+  ora_disc *
+  ora_disc_create2(const int n_cells, const int degree, const int test_case, const double mach, const int mode1,
+                   const int mode2, const double stretch, const double distort, const int general)
+  {
+    if (n_cells < 1 || degree < 1 || degree > 4 || test_case < 0 || test_case > 2)
+      return nullptr;
+    if (distort != 0. && test_case != ORA_TGV) // the distortion is the vortex's (cgdbe.cc:77)
+      return nullptr;
+    if (general && test_case != ORA_TGV) // general cells here have no wall faces
+      return nullptr;
+    ora_disc *d   = new ora_disc(n_cells, degree);
+    d->tc.id      = test_case;
+    d->tc.U0      = mach * std::sqrt(cs2); // test_case.h:28-30
+    d->tc.n1      = mode1;
+    d->tc.n2      = mode2;
+    d->tc.stretch = stretch;
+    d->tc.distort = distort;
+    d->reinit(general != 0);
+    return d;
+  }
   ora_disc *
   ora_disc_create(const int n_cells, const int degree, const int test_case, const double mach, const int mode1,
                   const int mode2, const double stretch)
   {
-    if (n_cells < 1 || degree < 1 || degree > 4 || test_case < 0 || test_case > 2)
-      return nullptr;
-    ora_disc *d  = new ora_disc(n_cells, degree);
-    d->tc.id     = test_case;
-    d->tc.U0     = mach * std::sqrt(cs2); // test_case.h:28-30
-    d->tc.n1     = mode1;
-    d->tc.n2     = mode2;
-    d->tc.stretch = stretch;
-    d->reinit();
-    return d;
+    return ora_disc_create2(n_cells, degree, test_case, mach, mode1, mode2, stretch, 0., 0);
   }
   void
   ora_disc_destroy(ora_disc *d)
@@ -1039,9 +1261,19 @@ extern "C"
     for (int j = 0; j < d->np; ++j)
       for (int i = 0; i < d->np; ++i)
         {
-          xy[2 * (j * d->np + i)]     = d->xn[i];
-          xy[2 * (j * d->np + i) + 1] = d->yn[j];
+          xy[2 * (j * d->np + i)]     = d->XN[j * d->np + i];
+          xy[2 * (j * d->np + i) + 1] = d->YN[j * d->np + i];
         }
+  }
+  // the (N+1)^2 vertices of a general mesh (x fastest); 0 for a tensor mesh
+  int
+  ora_disc_general_vertices(const ora_disc *d, double *x, double *y)
+  {
+    if (!d->general)
+      return 0;
+    std::copy(d->VX.begin(), d->VX.end(), x);
+    std::copy(d->VY.begin(), d->VY.end(), y);
+    return 1;
   }
   void
   ora_disc_vertices(const ora_disc *d, double *x, double *y)
@@ -1108,7 +1340,7 @@ extern "C"
     for (int j = 0; j < d->np; ++j)
       for (int i = 0; i < d->np; ++i)
         {
-          d->tc.initial_populations(d->xn[i], d->yn[j], lambda, fi);
+          d->tc.initial_populations(d->XN[j * d->np + i], d->YN[j * d->np + i], lambda, fi);
           for (int a = 0; a < Q; ++a)
             f[a * d->n + j * d->np + i] = fi[a];
         }
@@ -1119,7 +1351,7 @@ extern "C"
     for (int j = 0; j < d->np; ++j)
       for (int i = 0; i < d->np; ++i)
         {
-          const Moments m = d->tc.exact(d->xn[i], d->yn[j], t);
+          const Moments m = d->tc.exact(d->XN[j * d->np + i], d->YN[j * d->np + i], t);
           const int     k = j * d->np + i;
           rho_u[3 * k]    = m.rho;
           rho_u[3 * k + 1] = m.ux;
@@ -1432,7 +1664,7 @@ extern "C"
           for (int a = 0; a < Q; ++a)
             fi[a] = f[a * d->n + k];
           const Moments m  = moments(fi);
-          const Moments ex = d->tc.exact(d->xn[i], d->yn[j], t);
+          const Moments ex = d->tc.exact(d->XN[k], d->YN[k], t);
           const double  w  = d->ml[k];
           s[0] += w * ((m.ux - ex.ux) * (m.ux - ex.ux) + (m.uy - ex.uy) * (m.uy - ex.uy));
           s[1] += w * (ex.ux * ex.ux + ex.uy * ex.uy);

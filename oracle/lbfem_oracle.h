@@ -30,6 +30,10 @@ enum { ORA_OK = 0, ORA_EINVAL = 1, ORA_ENOCONV = 2 };
 /* Test case + discretization (test_cases.h, discretization.h:78-144). */
 ora_disc *ora_disc_create(int n_cells, int degree, int test_case, double mach, int mode1, int mode2,
                           double stretch);
+/* distort: the vortex's mesh distortion eps (test_cases.h:53-65); general = 1
+ * takes the general-quadrilateral path (per-point Jacobians) on any mesh */
+ora_disc *ora_disc_create2(int n_cells, int degree, int test_case, double mach, int mode1, int mode2,
+                           double stretch, double distort, int general);
 void      ora_disc_destroy(ora_disc *d);
 int       ora_disc_n_nodes(const ora_disc *d);
 int       ora_disc_n_per_dir(const ora_disc *d);
@@ -38,6 +42,7 @@ double    ora_disc_h_min(const ora_disc *d);
 double    ora_disc_time_step(const ora_disc *d, double cfl);
 void      ora_disc_nodes(const ora_disc *d, double *xy);              /* 2 n */
 void      ora_disc_vertices(const ora_disc *d, double *x, double *y); /* N+1 each */
+int       ora_disc_general_vertices(const ora_disc *d, double *x, double *y); /* (N+1)^2 each; 0: tensor mesh */
 void      ora_disc_lumped_mass(const ora_disc *d, double *ml);        /* n */
 void      ora_disc_mass_diagonal(const ora_disc *d, double *diag);    /* n */
 void      ora_disc_constrained(const ora_disc *d, int *flag);         /* n, 1 = periodic slave */

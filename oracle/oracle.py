@@ -43,6 +43,9 @@ def lib():
         vp = C.c_void_p
         sig = {
             "ora_disc_create": (vp, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double]),
+            "ora_disc_create2": (vp, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_int]),
+            "ora_disc_general_vertices": (C.c_int, [vp, _P, _P]),
             "ora_disc_destroy": (None, [vp]),
             "ora_disc_n_nodes": (C.c_int, [vp]),
             "ora_disc_n_per_dir": (C.c_int, [vp]),
@@ -112,8 +115,10 @@ def element_matrices(p, h, e):
 class Disc:
     """Discretization + TestCase of the oracle (discretization.h, test_cases.h)."""
 
-    def __init__(self, n_cells, degree=1, case=TGV, mach=0.1, modes=(1, 4), stretch=0.0, reynolds=None):
-        self.h = lib().ora_disc_create(n_cells, degree, case, mach, modes[0], modes[1], stretch)
+    def __init__(self, n_cells, degree=1, case=TGV, mach=0.1, modes=(1, 4), stretch=0.0, reynolds=None,
+                 distort=0.0, general=False):
+        self.h = lib().ora_disc_create2(n_cells, degree, case, mach, modes[0], modes[1], stretch, distort,
+                                        int(general))
         if not self.h:
             raise ValueError("invalid oracle discretization")
         self.N, self.p, self.case = n_cells, degree, case
@@ -149,6 +154,12 @@ class Disc:
         y = np.zeros(self.N + 1)
         lib().ora_disc_vertices(self.h, _p(x), _p(y))
         return x, y
+
+    def general_vertices(self):
+        """(N+1)^2 vertex coordinates (x fastest) of a general mesh, or None for a tensor mesh."""
+        x = np.zeros((self.N + 1) ** 2)
+        y = np.zeros((self.N + 1) ** 2)
+        return (x, y) if lib().ora_disc_general_vertices(self.h, _p(x), _p(y)) else None
 
     def lumped_mass(self):
         v = np.zeros(self.n)

@@ -398,3 +398,118 @@ def test_taylor_green_published_decay():
     assert abs(Eex - 0.13534) < 1e-5
     assert abs(E - 0.13513) < 2e-5, E
     assert abs(err - 9.3e-4) < 0.5e-4, err
+
+
+# ------------------------------------------------- general quadrilaterals (next row 4)
+def _distorted_tgv_error(eps, streaming, cfl, N=48):
+    """apps/cgdbe/cgdbe.cc:237-241 (n = ceil(T / dt), dt = T / n) and the relative L2 velocity error of the
+    diagnostics (cgdbe.cc:353-386) for the vortex of docs/results.md:222-227: 48^2 Q1, modes (1, 4), Re 100,
+    Ma 0.05, t = 0.25 t_ref, Bardow, consistent mass (CG 1e-8, mass.h:37)."""
+    import math
+    d = O.Disc(N, 1, O.TGV, mach=0.05, modes=(1, 4), reynolds=100, distort=eps)
+    T = 0.25 * d.reference_time
+    dt = d.time_step(cfl)
+    n = math.ceil(T / dt)
+    s = O.Scheme(d, O.BARDOW, streaming, O.MASS_CG, cg_tol=1e-8)
+    s.set_time_step(T / n, d.relaxation_time)
+    s.set_initial_populations()
+    try:
+        s.step(n)
+    except RuntimeError:
+        return float("inf")
+    sums = d.diagnostics(s.populations(), T)
+    return math.sqrt(sums[0] / sums[1]) if np.isfinite(sums[0]) else float("inf")
+
+
+@pytest.mark.parametrize("streaming,cfl,eps,published", [
+    (O.TG2, 0.25, 0.04, 2.07e-2),
+    (O.TG3, 1.0, 0.04, 3.38e-2),
+    (O.TG3, 0.5, 0.02, 3.17e-2),
+    (O.TG3_SPLIT, 1.0, 0.02, 1.23e-2),
+    (O.TG2, 0.45, 0.04, None),  # "unstable"
+])
+def test_distorted_vortex_published_errors(streaming, cfl, eps, published):
+    """docs/results.md:226-233 (general quadrilaterals, --distort eps): the published three-digit errors,
+    and the published instability of TG2 at nominal CFL 0.45 on the +-25 % mesh."""
+    err = _distorted_tgv_error(eps, streaming, cfl)
+    if published is None:
+        assert not err < 1.0
+    else:
+        # the published value to its three digits: half a unit of the last one
+        assert abs(err - published) <= 0.5 * 10 ** (np.floor(np.log10(published)) - 2) + 1e-12, err
+
+
+def test_general_cells_on_a_cartesian_mesh_match_the_tensor_path():
+    """The per-point Jacobian path (docs/method.md:216-224) on undistorted cells is the tensor path:
+    lumped mass, Jacobi diagonal, advection and the TG3 operator to round-off, p = 1..3."""
+    rng = np.random.default_rng(7)
+    for p in (1, 2, 3):
+        dt_ = O.Disc(6, p, O.TGV, mach=0.1, reynolds=100)
+        dg = O.Disc(6, p, O.TGV, mach=0.1, reynolds=100, general=True)
+        assert dt_.general_vertices() is None and dg.general_vertices() is not None
+        assert np.allclose(dg.lumped_mass(), dt_.lumped_mass(), rtol=1e-14, atol=0)
+        assert np.allclose(dg.mass_diagonal(), dt_.mass_diagonal(), rtol=1e-14, atol=0)
+        assert np.allclose(dg.nodes(), dt_.nodes(), rtol=0, atol=1e-15)
+        f = 1 + 0.1 * rng.standard_normal((9, dt_.n))
+        feq = 1 + 0.1 * rng.standard_normal((9, dt_.n))
+        for leelin in (False, True):
+            a = dt_.advection(f, 0.01, 0.003, feq=feq if leelin else None, leelin=leelin)
+            b = dg.advection(f, 0.01, 0.003, feq=feq if leelin else None, leelin=leelin)
+            assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a))
+        x = rng.standard_normal(dt_.n)
+        a = dt_.vmult(x, c=1e-3, e=(1.0, -1.0))
+        b = dg.vmult(x, c=1e-3, e=(1.0, -1.0))
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a))
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_general_cells_area_consistency_and_symmetry(p):
+    """What the mathematics fixes on a distorted periodic mesh (test_cases.h:53-65, eps = 0.04):
+    - the cells tile [0, L]^2 and det J is bilinear, so the lumped masses of the masters sum to L^2;
+    - x and y are in Q_p on MappingQ1 cells, so a linear field's gradient is exact: away from the periodic
+      seam the advection right-hand side is a s M_L,i with s = e . grad f (advection.h:219-241);
+    - the first-order part C_e is skew and K_e symmetric (Gauss(p+1) integrates adj(J)-weighted products
+      exactly), checked on random vectors through the operator's dt-polynomial."""
+    N = 8
+    d = O.Disc(N, p, O.TGV, mach=0.1, reynolds=100, distort=0.04)
+    L = 1.0
+    ml = d.lumped_mass()
+    con = d.constrained()
+    assert abs(ml[~con].sum() - L * L) < 1e-13
+    xy = d.nodes()
+    np1 = d.np1
+    alpha, beta = 0.7, -0.3
+    f = np.ones((9, d.n))
+    f[1:] += 1e-2 * (alpha * xy[:, 0] + beta * xy[:, 1])
+    dtv = 0.01
+    rhs = d.advection(f, dtv, 1.0)
+    ii, jj = np.meshgrid(np.arange(np1), np.arange(np1))
+    ii, jj = ii.ravel(), jj.ravel()
+    inner = (ii > p) & (ii < np1 - 1 - p) & (jj > p) & (jj < np1 - 1 - p)
+    for a in range(1, 9):
+        s = 1e-2 * (O.EX[a] * alpha + O.EY[a] * beta)
+        expect = -dtv * s * ml
+        assert np.max(np.abs(rhs[a - 1][inner] - expect[inner])) <= 1e-13 * np.max(np.abs(expect)) + 1e-18
+    # C_e and K_e from r(dt) = -dt C f - dt^2 / 2 K f (Bardow, advection.h:223), no walls
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(d.n)
+    v = rng.standard_normal(d.n)
+    u[con] = 0
+    v[con] = 0
+
+    def parts(x):
+        F = np.zeros((9, d.n))
+        F[1:] = x
+        r1 = d.advection(F, 1.0, 1.0, remove_mass_flux=False)
+        r2 = d.advection(F, 2.0, 1.0, remove_mass_flux=False)
+        K = -(r2 - 2 * r1)  # -dt^2/2 K with dt = 2 minus twice dt = 1
+        Cm = -(r1 + 0.5 * K)
+        return Cm, K
+
+    Cu, Ku = parts(u)
+    Cv, Kv = parts(v)
+    for a in range(8):
+        assert abs(v @ Cu[a] + u @ Cv[a]) <= 1e-12 * np.abs(v @ Cu[a]).max(initial=1.0)
+        assert abs(v @ Ku[a] - u @ Kv[a]) <= 1e-12 * abs(v @ Ku[a])
+    # and M through the mass operator
+    assert abs(v @ d.vmult(u) - u @ d.vmult(v)) <= 1e-13 * abs(v @ d.vmult(u))
