@@ -50,7 +50,10 @@ int lbfem_device_count(int *count);
  * for the structured box meshes of the reference: N x N Q_p cells on
  * [0, L]^2, periodic in the first n_periodic directions, vertices optionally
  * moved by the tanh clustering of LidDrivenCavity::transform_mesh
- * (include/lbfem/test_cases.h:215-225) when stretch > 0. */
+ * (include/lbfem/test_cases.h:215-225) when stretch > 0, or by the
+ * TaylorGreenVortex distortion (test_cases.h:53-65, --distort) when
+ * distort != 0: general quadrilaterals, periodic box only (n_periodic 2,
+ * stretch 0), FP64 only, not batched. */
 typedef struct
 {
   int    n_cells;    /* N, elements per direction (MeshSettings::n_cells) */
@@ -58,6 +61,7 @@ typedef struct
   int    n_periodic; /* 0, 1 (x) or 2 (TestCase::n_periodic) */
   double L;          /* box size (TestCase::L) */
   double stretch;    /* tanh clustering gamma, 0 = uniform */
+  double distort;    /* x -> x + distort L sin(2 pi x/L) sin(2 pi y/L) (1, 1), 0 = none */
 } lbfem_mesh_desc;
 
 typedef struct

@@ -51,6 +51,7 @@ namespace
     std::string         scheme = "bardow", mass = "cg", streaming = "tg2", precision = "fp64";
     int                 refine = 6, cells = 0, degree = 1, richardson = 2;
     double              mach = 0.1, cfl = 0., t_end = 0., stretch = 1.2, steady_tol = 1e-4, cg_tol = 1e-8;
+    double              distort = 0.; // tgv: x += eps L sin(2 pi x/L) sin(2 pi y/L) (1, 1) (cgdbe.cc:77)
     std::array<int, 2>  modes = {{1, 4}};
     std::vector<double> reynolds;
     unsigned            max_steps = 0, n_diagnostic = 20;
@@ -78,7 +79,7 @@ namespace
     "  --streaming tg2|tg3|tg3-split [tg2]   --refine n [6] | --cells N   --degree p [1]\n"
     "  --mach Ma [0.1]   --modes n1,n2 [1,4]   --reynolds Re[,Re...] [100 | 10 | 400]\n"
     "  --cfl c [0.25; cavity 0.4, with leelin 0.5]   --tend t [1; cavity 100]   --steps n\n"
-    "  --stretch gamma [1.2]   --steady-tol eps [1e-4]   --cg-tol eps [1e-8]   --fused\n"
+    "  --stretch gamma [1.2]   --distort eps [0]   --steady-tol eps [1e-4]   --cg-tol eps [1e-8]   --fused\n"
     "  --checkpoint name   --restart name   --precision fp64|mixed [fp64]   --no-output\n";
 
   std::vector<double>
@@ -139,6 +140,8 @@ namespace
           prm.max_steps = unsigned(std::stoul(val()));
         else if (a == "--stretch")
           prm.stretch = std::stod(val());
+        else if (a == "--distort")
+          prm.distort = std::stod(val());
         else if (a == "--steady-tol")
           prm.steady_tol = std::stod(val());
         else if (a == "--cg-tol")
@@ -325,7 +328,8 @@ namespace
       steady       = flow.steady();
 
       N = prm.cells > 0 ? prm.cells : (1 << prm.refine);
-      lbfem_mesh_desc mesh{N, prm.degree, flow.n_periodic(), flow.L, flow.stretch};
+      lbfem_mesh_desc mesh{N, prm.degree, flow.n_periodic(), flow.L, flow.stretch,
+                           prm.test_case == "tgv" ? prm.distort : 0.};
       check(lbfem_disc_create(&mesh, &disc));
       check(lbfem_disc_get_info(disc, &info));
       n = size_t(info.n_nodes);
@@ -579,7 +583,7 @@ namespace
               u[i] = rows[i][2];
               v[i] = rows[i][3];
             }
-          lbfem_mesh_desc mesh{N, prm.degree, 0, flow.L, flow.stretch};
+          lbfem_mesh_desc mesh{N, prm.degree, 0, flow.L, flow.stretch, 0.};
           int             its = 0;
           check(lbfem_stream_function(&mesh, u.data(), v.data(), psi.data(), &its));
           for (size_t i = 0; i < n; ++i)

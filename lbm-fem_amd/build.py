@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lbfem_amd", "_lib")
 LIB = os.path.join(OUT_DIR, "liblbfem_hip.so")
-SOURCES = ["api.hip", "solver_kernels.hip"] + [f"tile_{t}_p{p}.hip" for t in ("f64", "f32") for p in (1, 2, 3)]
+SOURCES = ["api.hip", "solver_kernels.hip", "quad_kernels.hip"] + [f"tile_{t}_p{p}.hip" for t in ("f64", "f32") for p in (1, 2, 3)]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",

@@ -101,7 +101,7 @@ namespace
   // D2Q9 directions (include/lbfem/d2q9.h:18-19)
   constexpr double kE[Q][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}, {-1, -1}, {0, -1}, {1, -1}};
 
-  constexpr int kMaxCgBlocks = 1 << 16;
+  constexpr int kMaxCgBlocks = 1 << 17; // blocks of one CG operator launch (partial sums)
 } // namespace
 
 struct lbfem_disc
@@ -117,6 +117,11 @@ struct lbfem_disc
   std::vector<int32_t>     wall_nodes;
   std::vector<double>      wall_velocity;
   Geo                      G{};
+  // general quadrilaterals (distort != 0): node coordinates, lumped mass and
+  // diagonal of M per node (host), and the device copies behind G.gen
+  bool                     general = false;
+  std::vector<double>      gen_xy, gen_ml, gen_diag;
+  DevBuf<double>           gen_buf;
   // scratch of the operator-level entry points
   DevBuf<CgState>  cg_state;
   DevBuf<double>   partial, s_r, s_p0, s_p1, s_q, s_ax;
@@ -151,6 +156,153 @@ struct lbfem_disc
         dev.b0[l] = l <= a.p ? a.b0[l] : 0.;
         dev.b1[l] = l <= a.p ? a.b1[l] : 0.;
       }
+  }
+
+  // The distorted mesh of TaylorGreenVortex::transform_mesh (test_cases.h:53-65):
+  // vertices moved by x -> x + eps L sin(2 pi x/L) sin(2 pi y/L) (1, 1), nodes
+  // through each cell's bilinear map (MappingQ1 support points,
+  // discretization.h:127-135; a node on a cell edge from its lower-left cell),
+  // and the lumped mass (row sums of M) and diag(M) from QGauss(p+1) with the
+  // Jacobian per point, summed per node over its cells (lower row first, left
+  // cell first); constrained entries 1 (discretization.h:123-125).
+  void
+  build_general(const double eps)
+  {
+    const int           N1 = N + 1, n1 = p + 1;
+    const double        L  = desc.L;
+    std::vector<double> vx(size_t(N1) * N1), vy(size_t(N1) * N1);
+    for (int j = 0; j <= N; ++j)
+      for (int i = 0; i <= N; ++i)
+        {
+          const double x = L * i / N, y = L * j / N;
+          const double dd = eps * L * std::sin(2 * kPi * x / L) * std::sin(2 * kPi * y / L);
+          vx[size_t(j) * N1 + i] = x + dd;
+          vy[size_t(j) * N1 + i] = y + dd;
+        }
+    const auto corner = [&](const int cx, const int cy, double c[8]) {
+      const size_t a = size_t(cy) * N1 + cx, b = a + N1;
+      c[0] = vx[a], c[1] = vy[a], c[2] = vx[a + 1], c[3] = vy[a + 1];
+      c[4] = vx[b], c[5] = vy[b], c[6] = vx[b + 1], c[7] = vy[b + 1];
+    };
+    gen_xy.assign(size_t(2) * n, 0.);
+    for (int j = 0; j < np; ++j)
+      for (int i = 0; i < np; ++i)
+        {
+          const int cx = std::min(i / p, N - 1), cy = std::min(j / p, N - 1);
+          double    c[8];
+          corner(cx, cy, c);
+          const double xi = B->node[i - p * cx], eta = B->node[j - p * cy];
+          const size_t k  = size_t(j) * np + i;
+          gen_xy[2 * k] = (1. - eta) * ((1. - xi) * c[0] + xi * c[2]) + eta * ((1. - xi) * c[4] + xi * c[6]);
+          gen_xy[2 * k + 1] = (1. - eta) * ((1. - xi) * c[1] + xi * c[3]) + eta * ((1. - xi) * c[5] + xi * c[7]);
+        }
+    // per cell and local node: int phi_k and int phi_k^2
+    std::vector<double> Rl(size_t(N) * N * n1 * n1), Rd(Rl.size());
+    for (int cy = 0; cy < N; ++cy)
+      for (int cx = 0; cx < N; ++cx)
+        {
+          double c[8], det[16];
+          corner(cx, cy, c);
+          for (int qy = 0; qy < n1; ++qy)
+            for (int qx = 0; qx < n1; ++qx)
+              {
+                const double xi = B->qx[qx], eta = B->qx[qy];
+                const double xxi  = (1. - eta) * (c[2] - c[0]) + eta * (c[6] - c[4]);
+                const double yxi  = (1. - eta) * (c[3] - c[1]) + eta * (c[7] - c[5]);
+                const double xeta = (1. - xi) * (c[4] - c[0]) + xi * (c[6] - c[2]);
+                const double yeta = (1. - xi) * (c[5] - c[1]) + xi * (c[7] - c[3]);
+                det[qy * n1 + qx] = xxi * yeta - xeta * yxi;
+              }
+          for (int ky = 0; ky < n1; ++ky)
+            for (int kx = 0; kx < n1; ++kx)
+              {
+                double sl = 0., sd = 0.;
+                for (int qy = 0; qy < n1; ++qy)
+                  for (int qx = 0; qx < n1; ++qx)
+                    {
+                      const double phi = B->S[qx][kx] * B->S[qy][ky];
+                      const double jxw = det[qy * n1 + qx] * B->qw[qx] * B->qw[qy];
+                      sl += phi * jxw;
+                      sd += phi * phi * jxw;
+                    }
+                const size_t o = (size_t(cy) * N + cx) * n1 * n1 + ky * n1 + kx;
+                Rl[o]          = sl;
+                Rd[o]          = sd;
+              }
+        }
+    const auto around = [&](const int i, int *c, int *k) {
+      int m = 0;
+      if (i % p == 0)
+        {
+          c[m] = i == 0 ? N - 1 : i / p - 1; // periodic: the master at 0 also closes cell N - 1
+          k[m++] = p;
+          c[m]   = i / p;
+          k[m++] = 0;
+        }
+      else
+        {
+          c[m]   = i / p;
+          k[m++] = i % p;
+        }
+      return m;
+    };
+    gen_ml.assign(n, 1.);
+    gen_diag.assign(n, 1.);
+    for (int j = 0; j < P; ++j)
+      {
+        int       cys[2], kys[2];
+        const int my = around(j, cys, kys);
+        for (int i = 0; i < P; ++i)
+          {
+            int       cxs[2], kxs[2];
+            const int mx = around(i, cxs, kxs);
+            double    sl = 0., sd = 0.;
+            for (int b = 0; b < my; ++b)
+              for (int a = 0; a < mx; ++a)
+                {
+                  const size_t o = (size_t(cys[b]) * N + cxs[a]) * n1 * n1 + kys[b] * n1 + kxs[a];
+                  sl += Rl[o];
+                  sd += Rd[o];
+                }
+            gen_ml[size_t(j) * np + i]   = sl;
+            gen_diag[size_t(j) * np + i] = sd;
+          }
+      }
+    // device: vertices (x, y) pairs, ml, 1/ml, 1/diag, basis tables
+    std::vector<double> host;
+    host.reserve(size_t(2) * N1 * N1 + 3 * size_t(n) + 40);
+    for (size_t k = 0; k < vx.size(); ++k)
+      {
+        host.push_back(vx[k]);
+        host.push_back(vy[k]);
+      }
+    const size_t o_ml = host.size();
+    host.insert(host.end(), gen_ml.begin(), gen_ml.end());
+    const size_t o_mli = host.size();
+    for (const double v : gen_ml)
+      host.push_back(1. / v);
+    const size_t o_di = host.size();
+    for (const double v : gen_diag)
+      host.push_back(1. / v);
+    const size_t o_b = host.size();
+    for (int q = 0; q < 4; ++q)
+      for (int k = 0; k < 4; ++k)
+        host.push_back(B->S[q][k]);
+    for (int q = 0; q < 4; ++q)
+      for (int k = 0; k < 4; ++k)
+        host.push_back(B->D[q][k]);
+    for (int q = 0; q < 4; ++q)
+      host.push_back(B->qx[q]);
+    for (int q = 0; q < 4; ++q)
+      host.push_back(B->qw[q]);
+    gen_buf.upload(host);
+    G.gen.vert     = gen_buf.p;
+    G.gen.ml       = gen_buf.p + o_ml;
+    G.gen.ml_inv   = gen_buf.p + o_mli;
+    G.gen.diag_inv = gen_buf.p + o_di;
+    G.gen.basis    = gen_buf.p + o_b;
+    G.gen.N        = N;
+    general        = true;
   }
 
   void
@@ -355,8 +507,11 @@ namespace
     if (collide)
       gstar = s.f[1 - s.cur].p;
     s.mark(-1);
-    HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, s.variant(), G, S, fin, (const double *)nullptr, gstar,
-                          s.rhs.p, s.stream));
+    if (d.general)
+      HIP_CHECK(quad_advect_launch(ll, G, S, fin, nullptr, gstar, s.rhs.p, s.stream));
+    else
+      HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, s.variant(), G, S, fin, (const double *)nullptr,
+                            gstar, s.rhs.p, s.stream));
     s.mark(1);
     if (collide)
       s.cur = 1 - s.cur;
@@ -578,7 +733,7 @@ namespace
     const lbfem_disc &d = *s.d;
     const Geo        &G = d.G;
     const bool        ll = s.set.scheme == LBFEM_LEELIN;
-    if (s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2)
+    if (s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2 && !d.general)
       {
         // one fused kernel: nodal prologue, advection, M_L^-1, nodal epilogue
         TileParams S = tile_params(s, s.sweeps[0], ll);
@@ -868,6 +1023,9 @@ extern "C"
       if (desc->n_cells < 2 || desc->degree < 1 || desc->degree > kMaxDegree || desc->n_periodic < 0 ||
           desc->n_periodic > 2 || !(desc->L > 0.) || desc->stretch < 0.)
         return fail(LBFEM_EINVAL, "invalid mesh description (N >= 2, degree 1..3, n_periodic 0..2, L > 0)");
+      if (desc->distort != 0. && (desc->n_periodic != 2 || desc->stretch != 0. || !std::isfinite(desc->distort)))
+        return fail(LBFEM_EINVAL, "the distortion maps the vortex's periodic box (n_periodic 2, stretch 0, "
+                                  "cgdbe.cc:77, test_cases.h:53-65)");
       auto d    = std::make_unique<lbfem_disc>();
       d->desc   = *desc;
       d->p      = desc->degree;
@@ -891,7 +1049,10 @@ extern "C"
       for (auto &s : d->G.walls.side)
         s = nullptr;
       d->G.walls.velocity = nullptr;
-      *out                = d.release();
+      d->G.gen            = GeneralDev{};
+      if (desc->distort != 0.)
+        d->build_general(desc->distort);
+      *out = d.release();
       return LBFEM_OK;
     });
   }
@@ -923,6 +1084,11 @@ extern "C"
   {
     if (!d || !xy)
       return fail(LBFEM_EINVAL, "null argument");
+    if (d->general)
+      {
+        std::copy(d->gen_xy.begin(), d->gen_xy.end(), xy);
+        return LBFEM_OK;
+      }
     for (int j = 0; j < d->np; ++j)
       for (int i = 0; i < d->np; ++i)
         {
@@ -948,6 +1114,11 @@ extern "C"
   {
     if (!d || !ml)
       return fail(LBFEM_EINVAL, "null argument");
+    if (d->general)
+      {
+        std::copy(d->gen_ml.begin(), d->gen_ml.end(), ml);
+        return LBFEM_OK;
+      }
     for (int j = 0; j < d->np; ++j)
       for (int i = 0; i < d->np; ++i)
         {
@@ -962,6 +1133,11 @@ extern "C"
   {
     if (!d || !diag)
       return fail(LBFEM_EINVAL, "null argument");
+    if (d->general)
+      {
+        std::copy(d->gen_diag.begin(), d->gen_diag.end(), diag);
+        return LBFEM_OK;
+      }
     for (int j = 0; j < d->np; ++j)
       for (int i = 0; i < d->np; ++i)
         {
@@ -1045,8 +1221,13 @@ extern "C"
       TileParams S       = tile_params(s, ee, kind == LBFEM_LEELIN);
       S.collide          = 0;
       S.remove_mass_flux = remove_mass_flux ? 1 : 0;
-      HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, tile_default_variant(d->G), d->G, S,
-                            f, kind == LBFEM_LEELIN ? feq : nullptr, (double *)nullptr, rhs, as_stream(stream)));
+      if (d->general) // no wall faces on the periodic box
+        HIP_CHECK(quad_advect_launch(kind == LBFEM_LEELIN, d->G, S, f, kind == LBFEM_LEELIN ? feq : nullptr, nullptr,
+                                     rhs, as_stream(stream)));
+      else
+        HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, tile_default_variant(d->G), d->G,
+                              S, f, kind == LBFEM_LEELIN ? feq : nullptr, (double *)nullptr, rhs,
+                              as_stream(stream)));
       return LBFEM_OK;
     });
   }
@@ -1188,6 +1369,8 @@ extern "C"
         return fail(LBFEM_EINVAL, "unknown precision");
       if (set->precision == LBFEM_MIXED && m.type == LBFEM_MASS_RICHARDSON)
         return fail(LBFEM_EINVAL, "the mixed-precision form is available with the lumped or the consistent mass");
+      if (set->precision == LBFEM_MIXED && d->general)
+        return fail(LBFEM_EINVAL, "general quadrilaterals (distort) are stepped in FP64 only");
       auto s = std::make_unique<lbfem_scheme>();
       s->d   = d;
       s->set = *set;
@@ -1226,7 +1409,7 @@ extern "C"
         s->sweeps = {all};
       s->tg3         = set->streaming == LBFEM_TG2 ? 0. : 1. / 6.; // streaming.h:47
       s->extrapolate = m.type == LBFEM_MASS_CG && set->streaming != LBFEM_TG3_SPLIT;
-      const bool lumped_fused = m.type == LBFEM_MASS_LUMPED && set->streaming == LBFEM_TG2;
+      const bool lumped_fused = m.type == LBFEM_MASS_LUMPED && set->streaming == LBFEM_TG2 && !d->general;
       if (!lumped_fused)
         {
           s->rhs.alloc(NM * st);
@@ -1518,6 +1701,15 @@ extern "C"
         return fail(LBFEM_EINVAL, "null argument");
       if (!(s->dt > 0.))
         return fail(LBFEM_EINVAL, "set the time step before tuning");
+      if (s->d->general) // one cell kernel shape: nothing to tune
+        {
+          if (seconds)
+            for (int v = 0; v < kTileVariants; ++v)
+              seconds[v] = -1.;
+          if (variant)
+            *variant = s->variant();
+          return LBFEM_OK;
+        }
       const TuneKey key = tune_key(*s);
       auto         &cache = tune_cache();
       auto          hit   = cache.find(key);
@@ -1589,6 +1781,8 @@ extern "C"
             return fail(LBFEM_EINVAL, "null scheme");
           if (!(s->set.mass.type == LBFEM_MASS_LUMPED && s->set.streaming == LBFEM_TG2))
             return fail(LBFEM_EINVAL, "a batch steps schemes of the lumped mass and TG2 streaming (one fused kernel)");
+          if (s->d->general)
+            return fail(LBFEM_EINVAL, "a batch steps tensor-product meshes only (no distort)");
           if (!(s->dt > 0.))
             return fail(LBFEM_EINVAL, "set the time step of every scheme before batching");
           for (int j = 0; j < k; ++j)

@@ -53,13 +53,59 @@ namespace lbfem_amd
     const double  *velocity; // 2 per wall
   };
 
+  // General quadrilaterals (the vortex's distorted periodic mesh): the cells are
+  // the bilinear images of their four vertices, so the operators are evaluated
+  // cell by cell with the Jacobian per quadrature point, and the lumped mass and
+  // the Jacobi diagonal are per node. All pointers null on a tensor mesh.
+  struct GeneralDev
+  {
+    const double *vert;     // (N+1)^2 vertices, (x, y) pairs, x fastest
+    const double *ml;       // lumped mass per node (constrained entries 1)
+    const double *ml_inv;   // its inverse
+    const double *diag_inv; // inverse diagonal of M per node (constrained entries 1)
+    const double *basis;    // S[4][4], D[4][4] (value, derivative of phi_k at Gauss point q: [q][k]), qx[4], qw[4]
+    int           N;        // cells per direction
+  };
+
   struct Geo
   {
-    AxisDev  ax, ay;
-    WallsDev walls;
-    int      p, P, np; // degree, p N, p N + 1
-    int64_t  stride;
+    AxisDev    ax, ay;
+    WallsDev   walls;
+    int        p, P, np; // degree, p N, p N + 1
+    int64_t    stride;
+    GeneralDev gen;
   };
+
+  // sum over a block of NT threads of up to K values per thread; the result is
+  // valid in thread 0 (a fixed order: wavefront shuffles, then the wavefronts)
+  template <int K, int NT>
+  __device__ __forceinline__ void
+  block_sum_n(double (&v)[K], double (*lds)[NT / 64])
+  {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      {
+        double s = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+          s += __shfl_down(s, off, 64);
+        if (lane == 0)
+          lds[k][wave] = s;
+      }
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        {
+          double s = 0.;
+#pragma unroll
+          for (int w = 0; w < NT / 64; ++w)
+            s += lds[k][w];
+          v[k] = s;
+        }
+    __syncthreads();
+  }
 
   __device__ __forceinline__ int64_t
   node_index(const Geo &G, const int i, const int j)

@@ -161,6 +161,9 @@ namespace lbfem_amd
     int    its, status; // status: 0 active, 1 converged, 2 failed (NoConvergence)
   };
 
+  // partial sums per block of the CG kernels (3 per vector), contiguous per block
+  constexpr int kCgPartials = 3 * NM;
+
   struct CgWork
   {
     int       nv;         // vectors
@@ -197,6 +200,23 @@ namespace lbfem_amd
   hipError_t refine_add_launch(const Geo &G, double *x, float *d, unsigned mask, hipStream_t stream);
   hipError_t add_increments_launch(const Geo &G, float *g, const double *x, unsigned mask, hipStream_t stream);
   hipError_t pc_from_f_launch(const Geo &G, float *f, const double *incr, double theta, hipStream_t stream);
+  // General quadrilaterals (quad_kernels.hip): the advection right-hand side
+  // (kAdvectBardow / kAdvectLeeLin semantics, no wall faces) and the mass
+  // operator stages of the CG (KIND 0 product, 1 start, 2 direction).
+  hipError_t quad_advect_launch(bool leelin, const Geo &G, const TileParams &S, const double *fin, const double *feq,
+                                double *out, double *rhs, hipStream_t stream);
+  template <int KIND>
+  hipError_t quad_operator_launch(const Geo &G, const CgWork &W, int it, const double *x, const double *b,
+                                  const double *r, double *y, double *y2, hipStream_t stream);
+  extern template hipError_t quad_operator_launch<0>(const Geo &, const CgWork &, int, const double *, const double *,
+                                                     const double *, double *, double *, hipStream_t);
+  extern template hipError_t quad_operator_launch<1>(const Geo &, const CgWork &, int, const double *, const double *,
+                                                     const double *, double *, double *, hipStream_t);
+  extern template hipError_t quad_operator_launch<2>(const Geo &, const CgWork &, int, const double *, const double *,
+                                                     const double *, double *, double *, hipStream_t);
+  // tiles of the general-mesh kernels (blocks of one operator launch per vector)
+  int quad_tile_count(const Geo &G);
+
   // y = (M + c K_e) x for nv vectors (constrained rows the identity)
   hipError_t vmult_launch(const Geo &G, const CgWork &W, const double *x, double *y, hipStream_t stream);
   // x_guess = 2 x - x_prev, x_prev = x (the extrapolated start, streaming.h:76-80), masked

@@ -13,6 +13,7 @@
 // and the last block to finish sums the partials in block order (a fixed
 // order, so results do not depend on scheduling) and updates the scalars.
 #include <algorithm>
+#include <type_traits>
 
 #include "device.hpp"
 #include "kernels.hpp"
@@ -22,7 +23,7 @@ namespace lbfem_amd
   namespace
   {
     constexpr int kThreads = 256;
-    constexpr int kPart    = 3 * NM; // partial sums per block, contiguous
+    constexpr int kPart    = kCgPartials; // partial sums per block, contiguous
 
     template <int p>
     __device__ __forceinline__ double
@@ -50,39 +51,26 @@ namespace lbfem_amd
     __device__ __forceinline__ double
     mass_diag_inv(const Geo &G, const int i, const int j)
     {
+      if (G.gen.diag_inv) // general cells: per node
+        return G.gen.diag_inv[node_index(G, i, j)];
       return is_slave(G, i, j) ? 1. : G.ax.diag_inv[i] * G.ay.diag_inv[j];
     }
 
-    // sum over the block of up to K values per thread; result valid in thread 0
+    // the inverse lumped mass, 1 on constrained rows
+    __device__ __forceinline__ double
+    lumped_inv(const Geo &G, const int i, const int j)
+    {
+      if (G.gen.ml_inv)
+        return G.gen.ml_inv[node_index(G, i, j)];
+      return is_slave(G, i, j) ? 1. : G.ax.ml_inv[i] * G.ay.ml_inv[j];
+    }
+
     template <int K>
     __device__ __forceinline__ void
     block_sum(double (&v)[K], double (*lds)[kThreads / 64])
     {
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        {
-          double s = v[k];
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1)
-            s += __shfl_down(s, off, 64);
-          if (lane == 0)
-            lds[k][wave] = s;
-        }
-      __syncthreads();
-      if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          {
-            double s = 0.;
-#pragma unroll
-            for (int w = 0; w < kThreads / 64; ++w)
-              s += lds[k][w];
-            v[k] = s;
-          }
-      __syncthreads();
+      block_sum_n<K, kThreads>(v, lds);
     }
-
   } // namespace
 
   // ------------------------------------------------------------------ nodal kernels
@@ -212,7 +200,7 @@ namespace lbfem_amd
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
       {
         const int    i = int(k % G.np), j = int(k / G.np);
-        const double m = is_slave(G, i, j) ? 1. : G.ax.ml_inv[i] * G.ay.ml_inv[j];
+        const double m = lumped_inv(G, i, j);
         for (int v = 0; v < nv; ++v)
           x[v * G.stride + k] = m * r[v * G.stride + k];
       }
@@ -225,7 +213,7 @@ namespace lbfem_amd
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
       {
         const int    i = int(k % G.np), j = int(k / G.np);
-        const double m = is_slave(G, i, j) ? 1. : G.ax.ml_inv[i] * G.ay.ml_inv[j];
+        const double m = lumped_inv(G, i, j);
         for (int v = 0; v < nv; ++v)
           x[v * G.stride + k] += m * (r[v * G.stride + k] - Ax[v * G.stride + k]);
       }
@@ -576,7 +564,7 @@ namespace lbfem_amd
           v[a] = f[a * G.stride + k];
         double rho, ux, uy;
         moments(v, rho, ux, uy);
-        const double w   = G.ax.ml[i] * G.ay.ml[j];
+        const double w   = G.gen.ml ? G.gen.ml[k] : G.ax.ml[i] * G.ay.ml[j];
         const double exu = u_exact ? u_exact[2 * k] : 0., exv = u_exact ? u_exact[2 * k + 1] : 0.;
         s[0] += w * ((ux - exu) * (ux - exu) + (uy - exv) * (uy - exv));
         s[1] += w * (exu * exu + exv * exv);
@@ -676,6 +664,20 @@ namespace lbfem_amd
   operator_launch(const Geo &G, const CgWork &W, const int it, const V *x, const V *b, const V *r, V *y, V *y2,
                   hipStream_t s)
   {
+    if (G.gen.vert != nullptr) // general quadrilaterals: the cell-wise operator (FP64 only)
+      {
+        if constexpr (std::is_same_v<V, double>)
+          {
+            const hipError_t e = quad_operator_launch<KIND>(G, W, it, x, b, r, y, y2, s);
+            if (e != hipSuccess)
+              return e;
+            if (KIND != 0)
+              hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, quad_tile_count(G), it);
+            return hipGetLastError();
+          }
+        else
+          return hipErrorInvalidValue;
+      }
     constexpr int TX = 32, TY = 16;
     const int     tx = (G.P + 1 + TX - 1) / TX, ty = (G.P + 1 + TY - 1) / TY;
     const int     nt = tx * ty, nb8 = (nt + 7) & ~7;

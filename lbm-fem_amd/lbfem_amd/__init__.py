@@ -42,7 +42,7 @@ class NoConvergence(LbfemError):
 
 class MeshDesc(C.Structure):
     _fields_ = [("n_cells", C.c_int), ("degree", C.c_int), ("n_periodic", C.c_int), ("L", C.c_double),
-                ("stretch", C.c_double)]
+                ("stretch", C.c_double), ("distort", C.c_double)]
 
 
 class DiscInfo(C.Structure):
@@ -166,7 +166,7 @@ class Discretization:
 
     def __init__(self, tc: TestCase, n_cells: int, degree: int = 1):
         self.tc = tc
-        desc = MeshDesc(n_cells, degree, tc.n_periodic, tc.L, tc.stretch)
+        desc = MeshDesc(n_cells, degree, tc.n_periodic, tc.L, tc.stretch, getattr(tc, "distort", 0.0))
         h = _vp()
         check(lib().lbfem_disc_create(C.byref(desc), C.byref(h)))
         self.h = h
@@ -451,6 +451,6 @@ def stream_function(n_cells, degree, u, v, L=1.0, stretch=0.0):
     v = np.ascontiguousarray(v, dtype=np.float64)
     psi = np.zeros_like(u)
     its = C.c_int(0)
-    desc = MeshDesc(int(n_cells), int(degree), 0, float(L), float(stretch))
+    desc = MeshDesc(int(n_cells), int(degree), 0, float(L), float(stretch), 0.0)
     check(lib().lbfem_stream_function(C.byref(desc), _p(u), _p(v), _p(psi), C.byref(its)))
     return psi, its.value

@@ -37,6 +37,7 @@ class TestCase:
     n_periodic = 0
     steady = False
     stretch = 0.0
+    distort = 0.0
 
     def __init__(self, mach):
         self.L = 1.0
@@ -67,13 +68,16 @@ class TestCase:
 
 
 class TaylorGreenVortex(TestCase):
-    """include/lbfem/test_cases.h:21-119 (Bardow et al., Eqs. 14-16), modes (n1, n2)."""
+    """include/lbfem/test_cases.h:21-119 (Bardow et al., Eqs. 14-16), modes (n1, n2); distort > 0 maps the
+    mesh by x -> x + eps L sin(2 pi x / L) sin(2 pi y / L) (1, 1) into general quadrilaterals
+    (test_cases.h:53-65)."""
 
     n_periodic = 2
 
-    def __init__(self, mach, n1=1, n2=4):
+    def __init__(self, mach, n1=1, n2=4, distort=0.0):
         super().__init__(mach)
         self.n1, self.n2 = n1, n2
+        self.distort = float(distort)
 
     @property
     def k1(self):
@@ -155,10 +159,10 @@ class LidDrivenCavity(TestCase):
         return side | lid, np.where(lid, self.U0, 0.0), np.zeros_like(x)
 
 
-def make(case, mach=0.1, modes=(1, 4), stretch=1.2):
+def make(case, mach=0.1, modes=(1, 4), stretch=1.2, distort=0.0):
     """apps/cgdbe/cgdbe.cc:92-103."""
     if case == "tgv":
-        return TaylorGreenVortex(mach, *modes)
+        return TaylorGreenVortex(mach, *modes, distort=distort)
     if case == "couette":
         return CouetteFlow(mach)
     if case == "cavity":

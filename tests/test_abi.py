@@ -38,3 +38,17 @@ def test_library_exports_every_declared_symbol():
 def test_error_reporting_without_device_work():
     assert A.lib().lbfem_disc_create(None, None) == A.EINVAL
     assert "null" in A.last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(A.LIB_PATH), reason="liblbfem_hip.so not built")
+def test_distortion_is_the_periodic_vortex_mesh_only():
+    """--distort maps the vortex's periodic box (apps/cgdbe/cgdbe.cc:77, test_cases.h:53-65): a walled or
+    stretched box with a distortion is rejected before any device work."""
+    import ctypes as C
+    h = C.c_void_p()
+    for n_periodic, stretch in ((0, 0.0), (1, 0.0), (2, 1.2)):
+        desc = A.MeshDesc(8, 1, n_periodic, 1.0, stretch, 0.02)
+        assert A.lib().lbfem_disc_create(C.byref(desc), C.byref(h)) == A.EINVAL
+        assert "distortion" in A.last_error()
+    desc = A.MeshDesc(8, 1, 2, 1.0, 0.0, float("nan"))
+    assert A.lib().lbfem_disc_create(C.byref(desc), C.byref(h)) == A.EINVAL
