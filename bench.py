@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-general", action="store_true")
     ap.add_argument("--profile-headline", action="store_true")
     ap.add_argument("--profile-cells", type=int, default=256)
     args = ap.parse_args()
@@ -201,6 +202,37 @@ def main():
                           "cg_its_per_solve": round(its, 2), "tile_variant": s2.tile_variant}
         log(name, variants[name])
         del s2, d2
+
+    # ---- general quadrilaterals: the vortex (1,4) on the distorted mesh (eps 0.04, cells +-25 %), Q1 1024^2,
+    # Re 100, CFL 0.25: the fused lumped step (one cell-kernel launch) and the consistent mass (CG 1e-8)
+    general = None
+    if not args.no_general:
+        general = {"workload": "Taylor-Green (1,4) on the distorted mesh x += 0.04 L sin(2 pi x) sin(2 pi y) (1,1), "
+                               "Q1 1024^2 cells, Re 100, Ma 0.1, CFL 0.25"}
+        for name, leelin, mass in (("bardow_lumped", False, A.MASS_LUMPED), ("leelin_lumped", True, A.MASS_LUMPED),
+                                   ("bardow_cg", False, A.MASS_CG)):
+            tc = A.test_cases.TaylorGreenVortex(0.1, 1, 4, distort=0.04)
+            tc.set_reynolds(100.0)
+            dg = A.Discretization(tc, 1024, 1)
+            cls = A.LeeLin if leelin else A.Bardow
+            sg = cls(dg, mass=mass, cg_tolerance=1e-8)
+            sg.set_time_step(dg.time_step(0.25), tc.relaxation_time())
+            sg.set_initial_populations(tc)
+            steps = max(10, K // 4) if mass == A.MASS_CG else K
+            its0 = sg.cg_iterations
+            secg = timed(torch, sg, steps, W)
+            row = {"ms_per_step": round(1e3 * secg / steps, 4), "MLUPS": round(9 * dg.n_nodes * steps / secg / 1e6, 1)}
+            if mass == A.MASS_CG:
+                row["cg_its_per_solve"] = round((sg.cg_iterations - its0) / (8 * (steps + W)), 2)
+            else:
+                tk = kernel_time(torch, sg, steps)
+                # populations in and out (144 B per node) and the vertices (16 B per cell corner, about per node)
+                gbs = (144 + 16) * dg.n_nodes / tk / 1e9
+                row.update({"kernel_us": round(1e6 * tk, 2), "achieved_GBs": round(gbs, 1),
+                            "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)})
+            general[name] = row
+            log("general", name, row)
+            del sg, dg
 
     # ---- HBM-resident sweep (working set > 256 MB Infinity Cache)
     sweep = []
@@ -356,6 +388,7 @@ def main():
         "variants": variants,
         "hbm_sweep": sweep,
         "precision": precision,
+        "general_quadrilaterals": general,
         "batched": batched,
         "served": served,
     }

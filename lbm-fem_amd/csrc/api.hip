@@ -508,7 +508,7 @@ namespace
       gstar = s.f[1 - s.cur].p;
     s.mark(-1);
     if (d.general)
-      HIP_CHECK(quad_advect_launch(ll, G, S, fin, nullptr, gstar, s.rhs.p, s.stream));
+      HIP_CHECK(quad_advect_launch(ll ? kAdvectLeeLin : kAdvectBardow, G, S, fin, nullptr, gstar, s.rhs.p, s.stream));
     else
       HIP_CHECK(tile_launch(ll ? kAdvectLeeLin : kAdvectBardow, s.variant(), G, S, fin, (const double *)nullptr,
                             gstar, s.rhs.p, s.stream));
@@ -733,7 +733,19 @@ namespace
     const lbfem_disc &d = *s.d;
     const Geo        &G = d.G;
     const bool        ll = s.set.scheme == LBFEM_LEELIN;
-    if (s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2 && !d.general)
+    if (s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2 && d.general)
+      {
+        // general quadrilaterals: one fused cell-kernel launch per step
+        TileParams S = tile_params(s, s.sweeps[0], ll);
+        S.collide    = 1;
+        s.mark(-1);
+        HIP_CHECK(quad_advect_launch(ll ? kStepLeeLin : kStepBardow, G, S, s.f[s.cur].p, nullptr, s.f[1 - s.cur].p,
+                                     nullptr, s.stream));
+        s.mark(1);
+        s.cur = 1 - s.cur;
+        return LBFEM_OK;
+      }
+    if (s.set.mass.type == LBFEM_MASS_LUMPED && s.set.streaming == LBFEM_TG2)
       {
         // one fused kernel: nodal prologue, advection, M_L^-1, nodal epilogue
         TileParams S = tile_params(s, s.sweeps[0], ll);
@@ -1222,8 +1234,8 @@ extern "C"
       S.collide          = 0;
       S.remove_mass_flux = remove_mass_flux ? 1 : 0;
       if (d->general) // no wall faces on the periodic box
-        HIP_CHECK(quad_advect_launch(kind == LBFEM_LEELIN, d->G, S, f, kind == LBFEM_LEELIN ? feq : nullptr, nullptr,
-                                     rhs, as_stream(stream)));
+        HIP_CHECK(quad_advect_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, d->G, S, f,
+                                     kind == LBFEM_LEELIN ? feq : nullptr, nullptr, rhs, as_stream(stream)));
       else
         HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, tile_default_variant(d->G), d->G,
                               S, f, kind == LBFEM_LEELIN ? feq : nullptr, (double *)nullptr, rhs,
@@ -1409,7 +1421,7 @@ extern "C"
         s->sweeps = {all};
       s->tg3         = set->streaming == LBFEM_TG2 ? 0. : 1. / 6.; // streaming.h:47
       s->extrapolate = m.type == LBFEM_MASS_CG && set->streaming != LBFEM_TG3_SPLIT;
-      const bool lumped_fused = m.type == LBFEM_MASS_LUMPED && set->streaming == LBFEM_TG2 && !d->general;
+      const bool lumped_fused = m.type == LBFEM_MASS_LUMPED && set->streaming == LBFEM_TG2;
       if (!lumped_fused)
         {
           s->rhs.alloc(NM * st);

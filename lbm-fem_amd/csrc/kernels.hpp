@@ -201,9 +201,9 @@ namespace lbfem_amd
   hipError_t add_increments_launch(const Geo &G, float *g, const double *x, unsigned mask, hipStream_t stream);
   hipError_t pc_from_f_launch(const Geo &G, float *f, const double *incr, double theta, hipStream_t stream);
   // General quadrilaterals (quad_kernels.hip): the advection right-hand side
-  // (kAdvectBardow / kAdvectLeeLin semantics, no wall faces) and the mass
+  // and the fused lumped step (TileMode semantics, no wall faces), and the mass
   // operator stages of the CG (KIND 0 product, 1 start, 2 direction).
-  hipError_t quad_advect_launch(bool leelin, const Geo &G, const TileParams &S, const double *fin, const double *feq,
+  hipError_t quad_advect_launch(int mode, const Geo &G, const TileParams &S, const double *fin, const double *feq,
                                 double *out, double *rhs, hipStream_t stream);
   template <int KIND>
   hipError_t quad_operator_launch(const Geo &G, const CgWork &W, int it, const double *x, const double *b,

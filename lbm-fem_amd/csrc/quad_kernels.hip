@@ -50,7 +50,6 @@ namespace lbfem_amd
       static constexpr int WC = (CX + 1) * (CY + 1);                        // cells evaluated
       static constexpr int HX = p * (CX + 1) + 1, HY = p * (CY + 1) + 1;    // window nodes
       static constexpr int OWN = (p * CX + 1) * (p * CY + 1);               // owned nodes, at most
-      static constexpr int RPT = (OWN + kQT - 1) / kQT;                     // owned nodes per thread
       static_assert(WC <= kQT, "one cell per thread");
     };
 
@@ -297,15 +296,21 @@ namespace lbfem_amd
   // Bardow v = a e.grad g, Lee-Lin v = e.grad(a f + b feq); flux c e (e.grad f).
   // Bardow may collide on load (writing g* of the owned nodes to out); Lee-Lin
   // takes the equilibria given or computes them from f. Constrained rows get 0.
-  template <int p, bool LL>
+  // STEP: the whole lumped TG2 step in one launch (schemes.h:92-98, 118-124,
+  // mass.h:189-191): the increments x = M_L^-1 r of an owned node are kept in
+  // LDS and the node is updated in place of writing r: Bardow f = g* + x,
+  // Lee-Lin the predictor-corrector with the node's own equilibria.
+  template <int p, bool LL, bool STEP>
   __global__ void __launch_bounds__(kQT)
   k_quad_advect(const Geo G, const TileParams S, const int tiles_x, const double *__restrict__ fin,
                 const double *__restrict__ feq_in, double *__restrict__ out, double *__restrict__ rhs)
   {
     using C              = QT<p>;
     constexpr int n1     = C::n1, NQ = C::NQ, HX = C::HX, HY = C::HY, NF = LL ? 2 * NM : NM;
-    __shared__ double F[NF][HY][HX];
+    constexpr int NF0    = STEP ? 1 : 0, NX = STEP ? C::OWN : 1;
+    __shared__ double F[NF + NF0][HY][HX]; // STEP: plane NF holds population 0
     __shared__ double R[C::WC][NQ];
+    __shared__ double XI[STEP ? NM : 1][NX];
     __shared__ Basis  B;
 
     const int nt = tiles_x * ((G.gen.N + C::CY - 1) / C::CY);
@@ -332,7 +337,9 @@ namespace lbfem_amd
           if (S.collide)
             collide_node(v, S.omega, false, 0., 0.);
         const bool owned = gi >= bx.oi0 && gi < bx.oi1 && gj >= bx.oj0 && gj < bx.oj1;
-        if (owned && out != nullptr)
+        if constexpr (STEP)
+          F[NF][lj][li] = v[0];
+        if (!STEP && owned && out != nullptr)
           {
             if (gi == P || gj == P) // a slave: its own entries, collided
               {
@@ -387,7 +394,10 @@ namespace lbfem_amd
         if (ex == 0. && ey == 0.) // at rest in this sweep: r = 0 (streaming.h:118-122)
           {
             for (int k = tid; k < nown; k += kQT)
-              ra[node_index(G, bx.oi0 + k % ow, bx.oj0 + k / ow)] = 0.;
+              if constexpr (STEP)
+                XI[al - 1][k] = 0.;
+              else
+                ra[node_index(G, bx.oi0 + k % ow, bx.oj0 + k / ow)] = 0.;
             continue;
           }
         if (has_cell)
@@ -440,11 +450,58 @@ namespace lbfem_amd
         __syncthreads();
         for (int k = tid; k < nown; k += kQT)
           {
-            const int gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
-            ra[node_index(G, gi, gj)] = (gi < P && gj < P) ? gather_node<p>(R, bx, gi, gj) : 0.; // 0 on slaves
+            const int    gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
+            const double r  = (gi < P && gj < P) ? gather_node<p>(R, bx, gi, gj) : 0.; // 0 on slaves
+            if constexpr (STEP)
+              XI[al - 1][k] = r;
+            else
+              ra[node_index(G, gi, gj)] = r;
           }
         __syncthreads(); // R is rewritten by the next population
       }
+
+    if constexpr (STEP)
+      for (int k = tid; k < nown; k += kQT)
+        {
+          const int     gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
+          const int64_t idx   = node_index(G, gi, gj);
+          const bool    slave = gi == P || gj == P;
+          double        own[Q], x[NM];
+          if (slave) // its own entries (the window holds its master's); only nodal work applies
+            {
+#pragma unroll
+              for (int a = 0; a < Q; ++a)
+                own[a] = fin[a * G.stride + idx];
+              if constexpr (!LL)
+                if (S.collide)
+                  collide_node(own, S.omega, false, 0., 0.);
+            }
+          else
+            {
+              const int li = gi - bx.oi0 + p, lj = gj - bx.oj0 + p;
+              own[0]       = F[NF][lj][li];
+#pragma unroll
+              for (int a = 1; a < Q; ++a)
+                own[a] = F[a - 1][lj][li];
+            }
+          const double mli = slave ? 0. : G.gen.ml_inv[idx];
+#pragma unroll
+          for (int a = 0; a < NM; ++a)
+            x[a] = XI[a][k] * mli;
+          if constexpr (LL)
+            {
+              double fe[Q];
+              equilibrium_node(own, fe, false, 0., 0.);
+              predictor_corrector_node(own, fe, x, S.theta, false, 0., 0.);
+            }
+          else
+#pragma unroll
+            for (int a = 1; a < Q; ++a)
+              own[a] += x[a - 1];
+#pragma unroll
+          for (int a = 0; a < Q; ++a)
+            out[a * G.stride + idx] = own[a];
+        }
   }
 
   // ------------------------------------------------------------------ mass operator
@@ -543,14 +600,10 @@ namespace lbfem_amd
       }
     __syncthreads();
 
-    double part[3] = {0., 0., 0.};
-#pragma unroll
-    for (int rr = 0; rr < C::RPT; ++rr)
+    double    part[3] = {0., 0., 0.};
+    const int ow = bx.oi1 - bx.oi0, nown = ow * (bx.oj1 - bx.oj0);
+    for (int k = tid; k < nown; k += kQT)
       {
-        const int k  = tid + rr * kQT;
-        const int ow = bx.oi1 - bx.oi0;
-        if (k >= ow * (bx.oj1 - bx.oj0))
-          continue;
         const int     gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
         const int64_t idx   = node_index(G, gi, gj);
         const bool    slave = gi == P || gj == P;
@@ -627,21 +680,34 @@ namespace lbfem_amd
 
   template <int p>
   static hipError_t
-  quad_advect_degree(const bool leelin, const Geo &G, const TileParams &S, const double *fin, const double *feq,
+  quad_advect_degree(const int mode, const Geo &G, const TileParams &S, const double *fin, const double *feq,
                      double *out, double *rhs, hipStream_t s)
   {
     int tx, nt;
     quad_tiles<p>(G, tx, nt);
-    const int nb8 = (nt + 7) & ~7;
-    if (leelin)
-      hipLaunchKernelGGL((k_quad_advect<p, true>), dim3(nb8), dim3(kQT), 0, s, G, S, tx, fin, feq, out, rhs);
-    else
-      hipLaunchKernelGGL((k_quad_advect<p, false>), dim3(nb8), dim3(kQT), 0, s, G, S, tx, fin, feq, out, rhs);
+    const dim3 grid((nt + 7) & ~7), block(kQT);
+    switch (mode)
+      {
+        case kStepBardow:
+          hipLaunchKernelGGL((k_quad_advect<p, false, true>), grid, block, 0, s, G, S, tx, fin, feq, out, rhs);
+          break;
+        case kStepLeeLin:
+          hipLaunchKernelGGL((k_quad_advect<p, true, true>), grid, block, 0, s, G, S, tx, fin, feq, out, rhs);
+          break;
+        case kAdvectBardow:
+          hipLaunchKernelGGL((k_quad_advect<p, false, false>), grid, block, 0, s, G, S, tx, fin, feq, out, rhs);
+          break;
+        case kAdvectLeeLin:
+          hipLaunchKernelGGL((k_quad_advect<p, true, false>), grid, block, 0, s, G, S, tx, fin, feq, out, rhs);
+          break;
+        default:
+          return hipErrorInvalidValue;
+      }
     return hipGetLastError();
   }
 
   hipError_t
-  quad_advect_launch(const bool leelin, const Geo &G, const TileParams &S, const double *fin, const double *feq,
+  quad_advect_launch(const int mode, const Geo &G, const TileParams &S, const double *fin, const double *feq,
                      double *out, double *rhs, hipStream_t s)
   {
     if (G.gen.vert == nullptr)
@@ -649,11 +715,11 @@ namespace lbfem_amd
     switch (G.p)
       {
         case 1:
-          return quad_advect_degree<1>(leelin, G, S, fin, feq, out, rhs, s);
+          return quad_advect_degree<1>(mode, G, S, fin, feq, out, rhs, s);
         case 2:
-          return quad_advect_degree<2>(leelin, G, S, fin, feq, out, rhs, s);
+          return quad_advect_degree<2>(mode, G, S, fin, feq, out, rhs, s);
         case 3:
-          return quad_advect_degree<3>(leelin, G, S, fin, feq, out, rhs, s);
+          return quad_advect_degree<3>(mode, G, S, fin, feq, out, rhs, s);
         default:
           return hipErrorInvalidValue;
       }
