@@ -307,10 +307,13 @@ namespace lbfem_amd
   {
     using C              = QT<p>;
     constexpr int n1     = C::n1, NQ = C::NQ, HX = C::HX, HY = C::HY, NF = LL ? 2 * NM : NM;
-    constexpr int NF0    = STEP ? 1 : 0, NX = STEP ? C::OWN : 1;
-    __shared__ double F[NF + NF0][HY][HX]; // STEP: plane NF holds population 0
+    // Lee-Lin's step keeps population 0 (plane NF) and the increments of the owned
+    // nodes in LDS for the nodal epilogue; Bardow's step updates population by population
+    constexpr bool    KEEP = STEP && LL;
+    constexpr int     NF0 = KEEP ? 1 : 0, NX = KEEP ? C::OWN : 1;
+    __shared__ double F[NF + NF0][HY][HX];
     __shared__ double R[C::WC][NQ];
-    __shared__ double XI[STEP ? NM : 1][NX];
+    __shared__ double XI[KEEP ? NM : 1][NX];
     __shared__ Basis  B;
 
     const int nt = tiles_x * ((G.gen.N + C::CY - 1) / C::CY);
@@ -337,8 +340,26 @@ namespace lbfem_amd
           if (S.collide)
             collide_node(v, S.omega, false, 0., 0.);
         const bool owned = gi >= bx.oi0 && gi < bx.oi1 && gj >= bx.oj0 && gj < bx.oj1;
-        if constexpr (STEP)
+        if constexpr (KEEP)
           F[NF][lj][li] = v[0];
+        if (STEP && !LL && owned) // Bardow's step: g*_0 now, a slave's whole (collided) update now
+          {
+            if (gi == P || gj == P)
+              {
+                const int64_t own = node_index(G, gi, gj);
+                double        w[Q];
+#pragma unroll
+                for (int a = 0; a < Q; ++a)
+                  w[a] = fin[a * G.stride + own];
+                if (S.collide)
+                  collide_node(w, S.omega, false, 0., 0.);
+#pragma unroll
+                for (int a = 0; a < Q; ++a)
+                  out[a * G.stride + own] = w[a];
+              }
+            else
+              out[idx] = v[0];
+          }
         if (!STEP && owned && out != nullptr)
           {
             if (gi == P || gj == P) // a slave: its own entries, collided
@@ -385,6 +406,19 @@ namespace lbfem_amd
     if (has_cell)
       cv = cell_vertices(G, pmod(bx.cx0 - 1 + lcx, G.gen.N), pmod(bx.cy0 - 1 + lcy, G.gen.N));
     __syncthreads();
+    // Q1: the cell's inverse Jacobians and JxW at its four points, once for all
+    // populations (40 registers); higher degrees recompute them per population
+    constexpr bool GEO = p == 1;
+    double         gk[GEO ? NQ : 1][5];
+    if constexpr (GEO)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        {
+          const int qx = q % n1, qy = q / n1;
+          double    det;
+          inverse_jacobian(cv, B.qx[qx], B.qx[qy], gk[q][0], gk[q][1], gk[q][2], gk[q][3], det);
+          gk[q][4] = det * B.qw[qx] * B.qw[qy];
+        }
 
     const int ow = bx.oi1 - bx.oi0, nown = ow * (bx.oj1 - bx.oj0);
     for (int al = 1; al < Q; ++al)
@@ -394,10 +428,18 @@ namespace lbfem_amd
         if (ex == 0. && ey == 0.) // at rest in this sweep: r = 0 (streaming.h:118-122)
           {
             for (int k = tid; k < nown; k += kQT)
-              if constexpr (STEP)
-                XI[al - 1][k] = 0.;
-              else
-                ra[node_index(G, bx.oi0 + k % ow, bx.oj0 + k / ow)] = 0.;
+              {
+                const int gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
+                if constexpr (KEEP)
+                  XI[al - 1][k] = 0.;
+                else if constexpr (STEP)
+                  {
+                    if (gi < P && gj < P)
+                      out[al * G.stride + node_index(G, gi, gj)] = F[al - 1][gj - bx.oj0 + p][gi - bx.oi0 + p];
+                  }
+                else
+                  ra[node_index(G, gi, gj)] = 0.;
+              }
             continue;
           }
         if (has_cell)
@@ -425,9 +467,22 @@ namespace lbfem_amd
 #pragma unroll
               for (int qx = 0; qx < n1; ++qx)
                 {
-                  double k00, k01, k10, k11, det;
-                  inverse_jacobian(cv, B.qx[qx], B.qx[qy], k00, k01, k10, k11, det);
-                  const double jxw = det * B.qw[qx] * B.qw[qy];
+                  double k00, k01, k10, k11, jxw;
+                  if constexpr (GEO)
+                    {
+                      const int q = qy * n1 + qx;
+                      k00         = gk[q][0];
+                      k01         = gk[q][1];
+                      k10         = gk[q][2];
+                      k11         = gk[q][3];
+                      jxw         = gk[q][4];
+                    }
+                  else
+                    {
+                      double det;
+                      inverse_jacobian(cv, B.qx[qx], B.qx[qy], k00, k01, k10, k11, det);
+                      jxw = det * B.qw[qx] * B.qw[qy];
+                    }
                   const double gx  = k00 * uxi[qy][qx] + k10 * ueta[qy][qx];
                   const double gy  = k01 * uxi[qy][qx] + k11 * ueta[qy][qx];
                   const double s   = ex * gx + ey * gy;
@@ -452,15 +507,23 @@ namespace lbfem_amd
           {
             const int    gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
             const double r  = (gi < P && gj < P) ? gather_node<p>(R, bx, gi, gj) : 0.; // 0 on slaves
-            if constexpr (STEP)
+            if constexpr (KEEP)
               XI[al - 1][k] = r;
+            else if constexpr (STEP)
+              {
+                if (gi < P && gj < P) // g* + M_L^-1 r
+                  {
+                    const int64_t idx = node_index(G, gi, gj);
+                    out[al * G.stride + idx] = F[al - 1][gj - bx.oj0 + p][gi - bx.oi0 + p] + r * G.gen.ml_inv[idx];
+                  }
+              }
             else
               ra[node_index(G, gi, gj)] = r;
           }
         __syncthreads(); // R is rewritten by the next population
       }
 
-    if constexpr (STEP)
+    if constexpr (KEEP) // Lee-Lin: x = M_L^-1 r, then the predictor-corrector with the node's own equilibria
       for (int k = tid; k < nown; k += kQT)
         {
           const int     gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
@@ -472,9 +535,6 @@ namespace lbfem_amd
 #pragma unroll
               for (int a = 0; a < Q; ++a)
                 own[a] = fin[a * G.stride + idx];
-              if constexpr (!LL)
-                if (S.collide)
-                  collide_node(own, S.omega, false, 0., 0.);
             }
           else
             {
@@ -488,16 +548,9 @@ namespace lbfem_amd
 #pragma unroll
           for (int a = 0; a < NM; ++a)
             x[a] = XI[a][k] * mli;
-          if constexpr (LL)
-            {
-              double fe[Q];
-              equilibrium_node(own, fe, false, 0., 0.);
-              predictor_corrector_node(own, fe, x, S.theta, false, 0., 0.);
-            }
-          else
-#pragma unroll
-            for (int a = 1; a < Q; ++a)
-              own[a] += x[a - 1];
+          double fe[Q];
+          equilibrium_node(own, fe, false, 0., 0.);
+          predictor_corrector_node(own, fe, x, S.theta, false, 0., 0.);
 #pragma unroll
           for (int a = 0; a < Q; ++a)
             out[a * G.stride + idx] = own[a];
