@@ -148,6 +148,7 @@ def main():
     ap.add_argument("--no-general", action="store_true")
     ap.add_argument("--profile-headline", action="store_true")
     ap.add_argument("--profile-cells", type=int, default=256)
+    ap.add_argument("--profile-general", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -168,6 +169,21 @@ def main():
         timed(torch, s, K, W)
         gbs, ok = hbm_probe(torch, A, nbytes=256 << 20, reps=3)
         log(f"profile-headline: {K + W} steps of cavity Q1 {N}^2, copy probe 3 x 256 MiB at {gbs:.0f} GB/s (ok={ok})")
+        return
+
+    if args.profile_general:
+        # profiler passes of the general-mesh step kernel: distorted vortex, Q1 1024^2, Bardow lumped, then the
+        # copy probe over 256 MiB (known size)
+        tc = A.test_cases.TaylorGreenVortex(0.1, 1, 4, distort=0.04)
+        tc.set_reynolds(100.0)
+        dg = A.Discretization(tc, args.profile_cells, 1)
+        sg = A.Bardow(dg, mass=A.MASS_LUMPED)
+        sg.set_time_step(dg.time_step(0.25), tc.relaxation_time())
+        sg.set_initial_populations(tc)
+        timed(torch, sg, K, W)
+        gbs, ok = hbm_probe(torch, A, nbytes=256 << 20, reps=3)
+        log(f"profile-general: {K + W} steps of the distorted vortex Q1 {args.profile_cells}^2 (Bardow, lumped), "
+            f"copy probe 3 x 256 MiB at {gbs:.0f} GB/s (ok={ok})")
         return
 
     hbm_gbs, hbm_ok = hbm_probe(torch, A)
