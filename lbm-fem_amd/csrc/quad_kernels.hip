@@ -145,6 +145,21 @@ namespace lbfem_amd
                   double (&ueta)[p + 1][p + 1])
     {
       constexpr int n = p + 1;
+      if constexpr (p == 1)
+        {
+          // Q1: phi' = -1, +1 (exact): d/dxi is the x difference interpolated in y, and d/deta alike
+          const double dx0 = u[0][1] - u[0][0], dx1 = u[1][1] - u[1][0];
+          const double dy0 = u[1][0] - u[0][0], dy1 = u[1][1] - u[0][1];
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+            {
+              const double gxi = B.S[q][0] * dx0 + B.S[q][1] * dx1; // at eta_q, any xi
+              const double get = B.S[q][0] * dy0 + B.S[q][1] * dy1; // at xi_q, any eta
+              uxi[q][0] = uxi[q][1] = gxi;
+              ueta[0][q] = ueta[1][q] = get;
+            }
+          return;
+        }
       double        A[n][n], Bv[n][n]; // [ky][qx]
 #pragma unroll
       for (int ky = 0; ky < n; ++ky)
@@ -209,13 +224,39 @@ namespace lbfem_amd
           }
     }
 
-    // r[ky][kx] = sum_q phi v + dphi/dxi Fxi + dphi/deta Feta (FEEvaluation::integrate)
+    // r[ky][kx] = sum_q phi v + dphi/dxi Fxi + dphi/deta Feta (FEEvaluation::integrate),
+    // entry k written to R[k * ld]
     template <int p>
     __device__ __forceinline__ void
     integrate(const Basis &B, const double (&v)[p + 1][p + 1], const double (&fxi)[p + 1][p + 1],
-              const double (&feta)[p + 1][p + 1], double *R)
+              const double (&feta)[p + 1][p + 1], double *R, const int ld)
     {
       constexpr int n = p + 1;
+      if constexpr (p == 1)
+        {
+          // Q1: phi' = -1, +1 on [0,1] (exact), so the flux terms need only sums over one direction
+          double Vx[2][2]; // [qy][kx]
+#pragma unroll
+          for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+            for (int kx = 0; kx < 2; ++kx)
+              Vx[qy][kx] = B.S[0][kx] * v[qy][0] + B.S[1][kx] * v[qy][1];
+          const double sx0 = fxi[0][0] + fxi[0][1], sx1 = fxi[1][0] + fxi[1][1];     // sum over qx, per qy
+          const double sy0 = feta[0][0] + feta[1][0], sy1 = feta[0][1] + feta[1][1]; // sum over qy, per qx
+#pragma unroll
+          for (int ky = 0; ky < 2; ++ky)
+            {
+              const double ax = B.S[0][ky] * sx0 + B.S[1][ky] * sx1;
+#pragma unroll
+              for (int kx = 0; kx < 2; ++kx)
+                {
+                  const double by = B.S[0][kx] * sy0 + B.S[1][kx] * sy1;
+                  const double vv = B.S[0][ky] * Vx[0][kx] + B.S[1][ky] * Vx[1][kx];
+                  R[(ky * 2 + kx) * ld] = vv + (kx ? ax : -ax) + (ky ? by : -by);
+                }
+            }
+          return;
+        }
       double        V[n][n], Wm[n][n]; // [qy][kx]
 #pragma unroll
       for (int qy = 0; qy < n; ++qy)
@@ -241,7 +282,7 @@ namespace lbfem_amd
 #pragma unroll
             for (int qy = 0; qy < n; ++qy)
               a += B.S[qy][ky] * V[qy][kx] + B.D[qy][ky] * Wm[qy][kx];
-            R[ky * n + kx] = a;
+            R[(ky * n + kx) * ld] = a;
           }
     }
 
@@ -249,7 +290,7 @@ namespace lbfem_amd
     // of their entries for it, lower cell row first, left cell first
     template <int p>
     __device__ __forceinline__ double
-    gather_node(const double (*R)[QT<p>::NQ], const Box<p> &bx, const int gi, const int gj)
+    gather_node(const double (*R)[QT<p>::WC], const Box<p> &bx, const int gi, const int gj)
     {
       int cxs[2], kxs[2], cys[2], kys[2], mx = 0, my = 0;
       if (gi % p == 0)
@@ -285,7 +326,7 @@ namespace lbfem_amd
         for (int a = 0; a < mx; ++a)
           {
             const int lc = (cys[b] - (bx.cy0 - 1)) * bx.wcx + (cxs[a] - (bx.cx0 - 1));
-            s += R[lc][kys[b] * (p + 1) + kxs[a]];
+            s += R[kys[b] * (p + 1) + kxs[a]][lc];
           }
       return s;
     }
@@ -312,7 +353,7 @@ namespace lbfem_amd
     constexpr bool    KEEP = STEP && LL;
     constexpr int     NF0 = KEEP ? 1 : 0, NX = KEEP ? C::OWN : 1;
     __shared__ double F[NF + NF0][HY][HX];
-    __shared__ double R[C::WC][NQ];
+    __shared__ double R[NQ][C::WC]; // local node-major: a wavefront's cells are consecutive
     __shared__ double XI[KEEP ? NM : 1][NX];
     __shared__ Basis  B;
 
@@ -483,24 +524,19 @@ namespace lbfem_amd
                       inverse_jacobian(cv, B.qx[qx], B.qx[qy], k00, k01, k10, k11, det);
                       jxw = det * B.qw[qx] * B.qw[qy];
                     }
-                  const double gx  = k00 * uxi[qy][qx] + k10 * ueta[qy][qx];
-                  const double gy  = k01 * uxi[qy][qx] + k11 * ueta[qy][qx];
-                  const double s   = ex * gx + ey * gy;
-                  double       val;
+                  // e . grad = (K e) . grad_xi, and the flux c e (e . grad f) JxW tested with grad phi
+                  // is c (e . grad f) JxW (K e) . grad_xi phi
+                  const double w0 = k00 * ex + k01 * ey, w1 = k10 * ex + k11 * ey;
+                  const double sj = (w0 * uxi[qy][qx] + w1 * ueta[qy][qx]) * jxw;
                   if constexpr (LL)
-                    {
-                      const double hx = k00 * hxi[qy][qx] + k10 * heta[qy][qx];
-                      const double hy = k01 * hxi[qy][qx] + k11 * heta[qy][qx];
-                      val             = ex * hx + ey * hy;
-                    }
+                    v[qy][qx] = (w0 * hxi[qy][qx] + w1 * heta[qy][qx]) * jxw;
                   else
-                    val = S.a * s;
-                  v[qy][qx]       = val * jxw;
-                  const double Fx = (S.c * ex) * s * jxw, Fy = (S.c * ey) * s * jxw;
-                  fxi[qy][qx]     = k00 * Fx + k01 * Fy;
-                  feta[qy][qx]    = k10 * Fx + k11 * Fy;
+                    v[qy][qx] = S.a * sj;
+                  const double t = S.c * sj;
+                  fxi[qy][qx]    = t * w0;
+                  feta[qy][qx]   = t * w1;
                 }
-            integrate<p>(B, v, fxi, feta, R[tid]);
+            integrate<p>(B, v, fxi, feta, &R[0][tid], C::WC);
           }
         __syncthreads();
         for (int k = tid; k < nown; k += kQT)
@@ -571,7 +607,7 @@ namespace lbfem_amd
     using C          = QT<p>;
     constexpr int n1 = C::n1, NQ = C::NQ, HX = C::HX, HY = C::HY;
     __shared__ double X[HY][HX];
-    __shared__ double R[C::WC][NQ];
+    __shared__ double R[NQ][C::WC]; // local node-major: a wavefront's cells are consecutive
     __shared__ double red[3][kQT / 64];
     __shared__ Basis  B;
 
@@ -639,17 +675,15 @@ namespace lbfem_amd
               v[qy][qx]        = uq[qy][qx] * jxw;
               if (stiff)
                 {
-                  const double gx = k00 * uxi[qy][qx] + k10 * ueta[qy][qx];
-                  const double gy = k01 * uxi[qy][qx] + k11 * ueta[qy][qx];
-                  const double eg = ex * gx + ey * gy;
-                  const double Fx = (c * ex) * eg * jxw, Fy = (c * ey) * eg * jxw;
-                  fxi[qy][qx]     = k00 * Fx + k01 * Fy;
-                  feta[qy][qx]    = k10 * Fx + k11 * Fy;
+                  const double w0 = k00 * ex + k01 * ey, w1 = k10 * ex + k11 * ey; // K e
+                  const double t  = c * (w0 * uxi[qy][qx] + w1 * ueta[qy][qx]) * jxw;
+                  fxi[qy][qx]     = t * w0;
+                  feta[qy][qx]    = t * w1;
                 }
               else
                 fxi[qy][qx] = feta[qy][qx] = 0.;
             }
-        integrate<p>(B, v, fxi, feta, R[tid]);
+        integrate<p>(B, v, fxi, feta, &R[0][tid], C::WC);
       }
     __syncthreads();
 
