@@ -67,3 +67,16 @@ def test_checkpoint_restart_continues(tmp_path, args):
         assert np.abs(a - b).max() < 1e-11 * np.abs(a).max()
     else:
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_distorted_vortex_published_error(tmp_path):
+    """--distort through the driver: docs/results.md:226-229, 48^2 Q1, modes (1, 4), Re 100, Ma 0.05,
+    t = 0.25 t_ref, Bardow, CG 1e-8, TG2 at CFL 0.25 on the +-25 % mesh: rel. L2 error 2.07e-2."""
+    r = run(["--case", "tgv", "--cells", "48", "--mach", "0.05", "--modes", "1,4", "--reynolds", "100",
+             "--tend", "0.25", "--distort", "0.04", "--no-output"], tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    rows = [ln.split() for ln in r.stdout.splitlines()]
+    rows = [x for x in rows if len(x) == 6 and x[0].isdigit()]
+    err = float(rows[-1][4])
+    assert abs(err - 2.07e-2) <= 0.5e-4, err
