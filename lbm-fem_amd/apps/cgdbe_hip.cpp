@@ -365,6 +365,7 @@ namespace
                        prm.richardson, prm.cg_tol, 200, prm.fused ? 1 : 0};
       set.precision = prm.precision == "mixed" ? LBFEM_MIXED : LBFEM_FP64;
       check(lbfem_scheme_create(disc, &set, &scheme));
+      check(lbfem_scheme_enable_stage_timers(scheme, 1)); // StageTimers, always on as in the reference
 
       // time step (cgdbe.cc:236-242): steps of the first Reynolds number; a transient hits t_end exactly
       flow.set_reynolds(prm.reynolds.front());
@@ -659,8 +660,52 @@ namespace
                 << 1e3 * wall / std::max(1u, total_steps) << " ms/step)\n"
                 << "  throughput    : " << 1e-6 * double(n) * total_steps / wall
                 << " million node updates/s (MNUPS = MDoF/s, one DoF = all " << Q << " populations)\n";
+      const double it_per_solve = double(its) / (std::max(1u, total_steps) * solves);
       if (prm.mass == "cg")
-        std::cout << "  CG iterations : " << double(its) / (std::max(1u, total_steps) * solves) << " per mass solve\n";
+        std::cout << "  CG iterations : " << it_per_solve << " per mass solve\n";
+
+      // cgdbe.cc:479-540: the stage times with the reference's single-pass traffic model (a vector pass is
+      // 8 n bytes) and flop model (a multiply-add counts 2), per node and step. The fused lumped step runs
+      // collision, advection and the lumped solve in one kernel: its time is under advection.
+      double t[4] = {0., 0., 0., 0.};
+      check(lbfem_scheme_stage_times(scheme, t));
+      const bool   leelin = prm.scheme == "leelin";
+      const double sweeps = prm.streaming == "tg3-split" ? 2. : 1.;
+      const double gb     = 1e-9 * double(n) * 8. * total_steps;
+      struct Work
+      {
+        double passes, flops;
+      };
+      const Work   collision = leelin ? Work{53., 303.} : Work{18. + 3. * solves, 149. + solves};
+      const double cells_per_node = double(N) * N / double(n);
+      const double flops_cell     = 184. * (prm.degree == 1 ? 1. : 3.5 * prm.degree) * 8. * cells_per_node;
+      const Work   advection      = {sweeps * (leelin ? 24. : 16.), sweeps * flops_cell * (leelin ? 1.4 : 1.)};
+      const bool   fused_cg       = prm.fused;
+      const Work   per_solve      = prm.mass == "cg" ? Work{(fused_cg ? 8. : 10.) * it_per_solve + 4.,
+                                                             (68. + (fused_cg ? 21. : 11.)) * it_per_solve} :
+                                    prm.mass == "lumped" ? Work{2., 1.} :
+                                                           Work{3. + (fused_cg ? 5. : 7.) * prm.richardson,
+                                                                1. + 71. * prm.richardson};
+      const Work   solve_work     = {per_solve.passes * solves, per_solve.flops * solves};
+      const double seconds        = std::max(wall, 1e-300);
+      const auto   stage          = [&](const char *name, const double sec, const Work &w) {
+        const double s_ = std::max(sec, 1e-300);
+        std::cout << "  " << std::left << std::setw(12) << name << std::right << std::setw(9) << std::fixed
+                  << std::setprecision(3) << sec << " s " << std::setw(5) << std::setprecision(1)
+                  << 100 * sec / seconds << " % " << std::setw(6) << std::setprecision(2) << w.passes * gb / s_
+                  << " GB/s " << std::setw(6) << std::setprecision(2) << 1e-9 * w.flops * double(n) * total_steps / s_
+                  << " GFlop/s  (" << std::setprecision(0) << w.passes << " vector passes, " << w.flops
+                  << " flops per node and step, intensity " << std::setprecision(2) << w.flops / (w.passes * 8.)
+                  << " flop/byte)" << std::defaultfloat << "\n";
+      };
+      std::cout << "  breakdown of the time stepping (GPU stage timers; single-pass traffic model, flop model"
+                << (prm.mass == "lumped" && prm.streaming == "tg2" ? "; fused step: all under advection" : "")
+                << "):\n";
+      stage("collision", t[0], collision);
+      stage("advection", t[1], advection);
+      stage("mass solves", t[2], solve_work);
+      std::cout << "  other (timers, loop overhead): " << std::fixed << std::setprecision(3)
+                << std::max(0., wall - t[3]) << " s" << std::defaultfloat << "\n";
     }
 
     const Parameters             prm;

@@ -80,3 +80,9 @@ def test_distorted_vortex_published_error(tmp_path):
     rows = [x for x in rows if len(x) == 6 and x[0].isdigit()]
     err = float(rows[-1][4])
     assert abs(err - 2.07e-2) <= 0.5e-4, err
+    # the summary's stage breakdown (cgdbe.cc:519-538): the CG run spends its time in the mass solves
+    lines = {ln.split()[0] + (ln.split()[1] if ln.split()[0] == "mass" else ""): ln for ln in r.stdout.splitlines()
+             if ln.strip().startswith(("collision", "advection", "mass solves"))}
+    assert set(lines) == {"collision", "advection", "masssolves"}, lines
+    sec = {k: float(v.split("s ")[0].split()[-1]) for k, v in lines.items()}
+    assert sec["masssolves"] > sec["collision"] > 0 and sec["advection"] > 0, sec
