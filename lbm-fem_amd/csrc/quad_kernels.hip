@@ -351,10 +351,17 @@ namespace lbfem_amd
     // Lee-Lin's step keeps population 0 (plane NF) and the increments of the owned
     // nodes in LDS for the nodal epilogue; Bardow's step updates population by population
     constexpr bool    KEEP = STEP && LL;
-    constexpr int     NF0 = KEEP ? 1 : 0, NX = KEEP ? C::OWN : 1;
+    // Q1 owns at most one node per thread: its increments stay in registers (the population
+    // loop unrolled), which keeps the LDS of three blocks per CU
+    constexpr bool    XREG = KEEP && C::OWN <= kQT;
+    constexpr int     NF0 = KEEP ? 1 : 0, NX = (KEEP && !XREG) ? C::OWN : 1;
     __shared__ double F[NF + NF0][HY][HX];
     __shared__ double R[NQ][C::WC]; // local node-major: a wavefront's cells are consecutive
-    __shared__ double XI[KEEP ? NM : 1][NX];
+    __shared__ double XI[(KEEP && !XREG) ? NM : 1][NX];
+    double            xreg[NM];
+#pragma unroll
+    for (int a = 0; a < NM; ++a)
+      xreg[a] = 0.;
     __shared__ Basis  B;
 
     const int nt = tiles_x * ((G.gen.N + C::CY - 1) / C::CY);
@@ -462,6 +469,8 @@ namespace lbfem_amd
         }
 
     const int ow = bx.oi1 - bx.oi0, nown = ow * (bx.oj1 - bx.oj0);
+    constexpr int UNROLL_POP = XREG ? NM : 1;
+#pragma unroll UNROLL_POP
     for (int al = 1; al < Q; ++al)
       {
         const double ex = S.e[al][0], ey = S.e[al][1];
@@ -471,7 +480,9 @@ namespace lbfem_amd
             for (int k = tid; k < nown; k += kQT)
               {
                 const int gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
-                if constexpr (KEEP)
+                if constexpr (XREG)
+                  xreg[al - 1] = 0.;
+                else if constexpr (KEEP)
                   XI[al - 1][k] = 0.;
                 else if constexpr (STEP)
                   {
@@ -543,7 +554,9 @@ namespace lbfem_amd
           {
             const int    gi = bx.oi0 + k % ow, gj = bx.oj0 + k / ow;
             const double r  = (gi < P && gj < P) ? gather_node<p>(R, bx, gi, gj) : 0.; // 0 on slaves
-            if constexpr (KEEP)
+            if constexpr (XREG)
+              xreg[al - 1] = r;
+            else if constexpr (KEEP)
               XI[al - 1][k] = r;
             else if constexpr (STEP)
               {
@@ -583,7 +596,7 @@ namespace lbfem_amd
           const double mli = slave ? 0. : G.gen.ml_inv[idx];
 #pragma unroll
           for (int a = 0; a < NM; ++a)
-            x[a] = XI[a][k] * mli;
+            x[a] = (XREG ? xreg[a] : XI[a][k]) * mli;
           double fe[Q];
           equilibrium_node(own, fe, false, 0., 0.);
           predictor_corrector_node(own, fe, x, S.theta, false, 0., 0.);
