@@ -122,6 +122,36 @@ def test_many_steps_stable_case():
     assert rel(gs.f, osch.populations()) < 1e-10
 
 
+@pytest.mark.parametrize("scheme,cfl", [(A.LEELIN, 0.5), (A.BARDOW, 0.25)])
+def test_plain_vortex_1000_steps(scheme, cfl):
+    """The CPU-runnable plain-method case: the Taylor-Green vortex (1,1) on 64^2 Q1 cells, U0 = 0.01
+    (Ma = 0.01 sqrt 3), Re 100, consistent mass (CG 1e-12), 1000 steps: populations within 1e-10 of the
+    oracle and the CG iterations of every step within +-1 per solve. The reference's vortex lives on
+    [0, L]^2 with L = 1 (test_case.h:25-45) and starts from its Chapman-Enskog state (test_cases.h:104-118).
+    Bardow at CFL 0.5 (dt / lambda = 26) diverges in the oracle, as in the reference algorithm, within
+    300 steps; the stable settings are Lee-Lin at CFL 0.5 and Bardow at CFL 0.25."""
+    tc = A.test_cases.make("tgv", mach=0.01 * 3 ** 0.5, modes=(1, 1))
+    tc.set_reynolds(100.0)
+    gd = A.Discretization(tc, 64, 1)
+    od = O.Disc(64, 1, O.TGV, mach=0.01 * 3 ** 0.5, modes=(1, 1), reynolds=100.0)
+    dt, lam = od.time_step(cfl), od.relaxation_time
+    cls = A.Bardow if scheme == A.BARDOW else A.LeeLin
+    gs = cls(gd, mass=A.MASS_CG, cg_tolerance=1e-12)
+    gs.set_time_step(dt, lam)
+    gs.set_initial_populations()
+    osch = O.Scheme(od, O.BARDOW if scheme == A.BARDOW else O.LEELIN, O.TG2, O.MASS_CG, cg_tol=1e-12)
+    osch.set_time_step(dt, lam)
+    osch.set_initial_populations()
+    worst = 0
+    for _ in range(1000):
+        g0, o0 = gs.cg_iterations, osch.cg_iterations
+        gs.step(1)
+        osch.step(1)
+        worst = max(worst, abs((gs.cg_iterations - g0) - (osch.cg_iterations - o0)))
+    assert rel(gs.f, osch.populations()) < 1e-10
+    assert worst <= 8, worst  # 8 solves per step, +-1 each
+
+
 # ------------------------------------------------------------------ operators
 def _seeded_f(od, seed, amp=0.05):
     rng = np.random.default_rng(seed)
