@@ -513,3 +513,26 @@ def test_general_cells_area_consistency_and_symmetry(p):
         assert abs(v @ Ku[a] - u @ Kv[a]) <= 1e-12 * abs(v @ Ku[a])
     # and M through the mass operator
     assert abs(v @ d.vmult(u) - u @ d.vmult(v)) <= 1e-13 * abs(v @ d.vmult(u))
+
+
+@pytest.mark.parametrize("mass,cfl,stable", [
+    (O.MASS_CG, 0.45, True), (O.MASS_CG, 0.5, False), (O.MASS_LUMPED, 0.7, True), (O.MASS_LUMPED, 1.0, False),
+])
+def test_bardow_published_stability_limits(mass, cfl, stable):
+    """docs/results.md:40-43: stability in dt |e_x| / h at 32^2, Re 100 (the vortex (1,1), Ma 0.1, to
+    t = t_ref, n = ceil(T / dt), cgdbe.cc:237-241): Bardow consistent stable at 0.45, unstable at 0.5;
+    Bardow lumped stable at 0.7, unstable at 1.0."""
+    import math
+    d = O.Disc(32, 1, O.TGV, mach=0.1, modes=(1, 1), reynolds=100)
+    T = d.reference_time
+    n = math.ceil(T / d.time_step(cfl))
+    s = O.Scheme(d, O.BARDOW, O.TG2, mass, cg_tol=1e-8)
+    s.set_time_step(T / n, d.relaxation_time)
+    s.set_initial_populations()
+    try:
+        s.step(n)
+        sums = d.diagnostics(s.populations(), T)
+        err = math.sqrt(sums[0] / sums[1])
+    except RuntimeError:  # NoConvergence of the CG on a diverging field
+        err = float("inf")
+    assert (err < 0.1) == stable, err
