@@ -125,6 +125,18 @@ print(json.dumps({"MLUPS": round(9 * od.n * steps / dt / 1e6, 3), "steps": steps
 """
 
 
+def host_cpu():
+    """The host CPU model (lscpu's model name), for the baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_oracle_rate(threads):
     """The oracle's headline-workload rate on `threads` host threads, in a child process (no GPU there)."""
     import subprocess
@@ -146,6 +158,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-general", action="store_true")
+    ap.add_argument("--precision-steps", type=int, default=100)
     ap.add_argument("--profile-headline", action="store_true")
     ap.add_argument("--profile-cells", type=int, default=256)
     ap.add_argument("--profile-general", action="store_true")
@@ -270,44 +283,49 @@ def main():
     precision = None
     if not args.no_precision:
         import numpy as np
-        runs = {}
-        steps_p = 100
-        # mixed_tol1e-6: FP32 storage puts round-off of about 1e-5 |b| into the right-hand side at this size,
-        # which a 1e-10 solve resolves at a cost of many iterations and no accuracy
-        for prec, name, tol in ((A.FP64, "fp64", 1e-10), (A.MIXED, "mixed", 1e-10), (A.MIXED, "mixed_tol1e-6", 1e-6)):
-            tc = A.test_cases.make("tgv", mach=0.1, modes=(1, 1))
-            tc.set_reynolds(100.0)
-            dp = A.Discretization(tc, 1024, 1)
-            sp = A.Bardow(dp, mass=A.MASS_CG, cg_tolerance=tol, precision=prec)
-            dtp = dp.time_step(0.25)
-            sp.set_time_step(dtp, tc.relaxation_time())
-            sp.set_initial_populations(tc)
-            sp.step(5)
-            sp.synchronize()
-            its0 = sp.cg_iterations
-            t0 = time.perf_counter()
-            sp.step(steps_p)
-            sp.synchronize()
-            tp = time.perf_counter() - t0
-            _, u, v = A.moments(sp.f)
-            runs[name] = {"ms_per_step": round(1e3 * tp / steps_p, 3),
-                          "MLUPS": round(9 * dp.n_nodes * steps_p / tp / 1e6, 1),
-                          "cg_its_per_solve": round((sp.cg_iterations - its0) / (8 * steps_p), 2), "u": u, "v": v}
-            xy, indep, w = dp.node, ~dp.constrained(), dp.lumped_mass()
-            del sp
-        _, ue, ve = tc.exact(xy[:, 0], xy[:, 1], (steps_p + 5) * dtp)
-        l2 = lambda u, v, ur, vr: float(np.sqrt((w[indep] * ((u - ur) ** 2 + (v - vr) ** 2)[indep]).sum()
-                                                / (w[indep] * (ur ** 2 + vr ** 2)[indep]).sum()))
-        precision = {"workload": "Taylor-Green (1,1), Q1 1024^2 cells, Bardow, consistent mass CG 1e-10, CFL 0.25, "
-                                 f"Re 100, {steps_p} steps after 5",
-                     "l2_velocity_mixed_vs_fp64": l2(runs["mixed"]["u"], runs["mixed"]["v"], runs["fp64"]["u"],
-                                                     runs["fp64"]["v"]),
-                     "l2_velocity_mixed_tol1e-6_vs_fp64": l2(runs["mixed_tol1e-6"]["u"], runs["mixed_tol1e-6"]["v"],
-                                                             runs["fp64"]["u"], runs["fp64"]["v"]),
-                     "l2_velocity_vs_analytic": {k: l2(r["u"], r["v"], ue, ve) for k, r in runs.items()}}
-        for k, r in runs.items():
-            precision[k] = {x: r[x] for x in ("ms_per_step", "MLUPS", "cg_its_per_solve")}
-        log("precision", precision)
+        steps_p = args.precision_steps
+        precision = {"workload": "Taylor-Green (1,1), Bardow, consistent mass CG 1e-10, CFL 0.25, Re 100, "
+                                 f"{steps_p} steps after 5; Q1 1024^2 and Q2 512^2 cells"}
+        for label, N, p in (("q1_1024", 1024, 1), ("q2_512", 512, 2)):
+            runs = {}
+            # mixed_tol1e-6: FP32 storage puts round-off of about 1e-5 |b| into the right-hand side at this size,
+            # which a 1e-10 solve resolves at a cost of many iterations and no accuracy
+            for prec, name, tol in ((A.FP64, "fp64", 1e-10), (A.MIXED, "mixed", 1e-10),
+                                    (A.MIXED, "mixed_tol1e-6", 1e-6)):
+                tc = A.test_cases.make("tgv", mach=0.1, modes=(1, 1))
+                tc.set_reynolds(100.0)
+                dp = A.Discretization(tc, N, p)
+                sp = A.Bardow(dp, mass=A.MASS_CG, cg_tolerance=tol, precision=prec)
+                dtp = dp.time_step(0.25)
+                sp.set_time_step(dtp, tc.relaxation_time())
+                sp.set_initial_populations(tc)
+                sp.step(5)
+                sp.synchronize()
+                its0 = sp.cg_iterations
+                t0 = time.perf_counter()
+                sp.step(steps_p)
+                sp.synchronize()
+                tp = time.perf_counter() - t0
+                _, u, v = A.moments(sp.f)
+                runs[name] = {"ms_per_step": round(1e3 * tp / steps_p, 3),
+                              "MLUPS": round(9 * dp.n_nodes * steps_p / tp / 1e6, 1),
+                              "cg_its_per_solve": round((sp.cg_iterations - its0) / (8 * steps_p), 2), "u": u,
+                              "v": v}
+                xy, indep, w = dp.node, ~dp.constrained(), dp.lumped_mass()
+                log("precision", label, name, {k: runs[name][k] for k in ("ms_per_step", "cg_its_per_solve")})
+                del sp
+            _, ue, ve = tc.exact(xy[:, 0], xy[:, 1], (steps_p + 5) * dtp)
+            l2 = lambda u, v, ur, vr: float(np.sqrt((w[indep] * ((u - ur) ** 2 + (v - vr) ** 2)[indep]).sum()
+                                                    / (w[indep] * (ur ** 2 + vr ** 2)[indep]).sum()))
+            row = {"l2_velocity_mixed_vs_fp64": l2(runs["mixed"]["u"], runs["mixed"]["v"], runs["fp64"]["u"],
+                                                   runs["fp64"]["v"]),
+                   "l2_velocity_mixed_tol1e-6_vs_fp64": l2(runs["mixed_tol1e-6"]["u"], runs["mixed_tol1e-6"]["v"],
+                                                           runs["fp64"]["u"], runs["fp64"]["v"]),
+                   "l2_velocity_vs_analytic": {k: l2(r["u"], r["v"], ue, ve) for k, r in runs.items()}}
+            for k, r in runs.items():
+                row[k] = {x: r[x] for x in ("ms_per_step", "MLUPS", "cg_its_per_solve")}
+            precision[label] = row
+            log("precision", label, row)
 
     # ---- batched: 96 seeded uneven instances (flow, N 32..512, Q1/Q2, Re, U0), Bardow lumped, one launch per
     # (degree, scheme, precision) group per step, graph-replayed, autotuned shapes
@@ -376,7 +394,7 @@ def main():
                "sample": f"{many['steps']} steps of the same workload (cavity Q1 256^2, Lee-Lin, lumped) on "
                          f"{threads} host threads (OpenMP build of the oracle), {many['seconds']:.1f} s; one core: "
                          f"{one['MLUPS']} MLUPS ({one['steps']} steps, {one['seconds']:.1f} s)",
-               "single_core_MLUPS": one["MLUPS"]}
+               "single_core_MLUPS": one["MLUPS"], "host_cpu": host_cpu(), "host_nproc": os.cpu_count()}
         log("cpu", cpu)
 
     out = {
