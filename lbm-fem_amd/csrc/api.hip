@@ -454,7 +454,12 @@ namespace
   run_cg(const lbfem_disc &d, const CgWork &W, V *x, const V *b, V *r, V *p0, V *p1, V *q, CgState *host_state,
          hipStream_t stream, int64_t *its, const int check_from, const bool fail_at_cap = true)
   {
-    const Geo &G = d.G;
+    const Geo &G  = d.G;
+    CgWork     Wf = W;
+    // small grids: the update kernel reduces the direction stage's partials itself (one
+    // launch less per iteration); large ones keep the separate one-block finalisation
+    const int rows = cg_direction_rows(G);
+    Wf.alpha_rows  = int64_t(rows) * W.nv <= 4096 && rows + 1024 <= W.max_blocks ? rows : 0;
     HIP_CHECK(cg_start_launch(G, W, x, b, r, stream));
     V         *pb[2] = {p0, p1};
     const auto done  = [&](bool &finished) -> int {
@@ -479,8 +484,8 @@ namespace
     // iterations of converged vectors are no-ops, so the first checks can be skipped
     for (int it = 1; it <= W.max_its; ++it)
       {
-        HIP_CHECK(cg_direction_launch(G, W, it, pb[it % 2], r, pb[(it + 1) % 2], q, stream));
-        HIP_CHECK(cg_update_launch(G, W, it, x, r, pb[(it + 1) % 2], q, stream));
+        HIP_CHECK(cg_direction_launch(G, Wf, it, pb[it % 2], r, pb[(it + 1) % 2], q, stream));
+        HIP_CHECK(cg_update_launch(G, Wf, it, x, r, pb[(it + 1) % 2], q, stream));
         if (it >= check_from)
           {
             bool      finished = false;

@@ -175,7 +175,13 @@ namespace lbfem_amd
     double   *partial;    // device [nv * 4 * max_blocks]
     unsigned *counter;    // device, zero between uses
     int       max_blocks;
+    // > 0: the direction stage's partial rows, which the update kernel reduces itself
+    // (every block, in the order the finalisation uses) instead of a separate launch
+    int       alpha_rows;
   };
+
+  // partial rows the CG direction stage writes (its blocks per vector)
+  int cg_direction_rows(const Geo &G);
 
   // r = A x - b, tolerances, rho = r.D^-1 r (iteration 0 of SolverCG)
   hipError_t cg_start_launch(const Geo &G, const CgWork &W, const double *x, const double *b, double *r,

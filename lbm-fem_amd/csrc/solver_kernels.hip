@@ -320,15 +320,12 @@ namespace lbfem_amd
       }
     __syncthreads();
 
-    double part[3 * NM];
-#pragma unroll
-    for (int k = 0; k < 3 * NM; ++k)
-      part[k] = 0.;
-
-    for (int v = 0; v < W.nv; ++v)
+    // this block's vector only; its partial sums in scalars (an array indexed by a run-time
+    // vector number would live in scratch memory)
+    double part0 = 0., part1 = 0., part2 = 0.;
+    {
       {
-        if (!(act & (1u << v)))
-          continue;
+        const int     v = vsel;
         const V       c = V(W.vec[v].coef), ex = V(W.vec[v].e[0]), ey = V(W.vec[v].e[1]);
         const int64_t vb = int64_t(W.block[v]) * G.stride;
         const V(*Xv)[HX] = X[0];
@@ -398,21 +395,22 @@ namespace lbfem_amd
                 const V rr  = a - bb;
                 y[vb + idx] = rr;
                 const double r8 = double(rr), b8 = double(bb);
-                part[3 * v] += r8 * r8;
-                part[3 * v + 1] += b8 * b8;
-                part[3 * v + 2] += r8 * (mass_diag_inv(G, gi, gj) * r8);
+                part0 += r8 * r8;
+                part1 += b8 * b8;
+                part2 += r8 * (mass_diag_inv(G, gi, gj) * r8);
               }
             else
               {
                 y[vb + idx]  = own; // p_new
                 y2[vb + idx] = a;   // q = A p_new
-                part[3 * v] += double(own) * double(a);
+                part0 += double(own) * double(a);
               }
           }
       }
+    }
     if constexpr (KIND != 0)
       {
-        double pv[3] = {part[3 * vsel], part[3 * vsel + 1], part[3 * vsel + 2]};
+        double pv[3] = {part0, part1, part2};
         block_sum<3>(pv, red);
         if (tid == 0)
           for (int k = 0; k < (KIND == 1 ? 3 : 1); ++k)
@@ -426,7 +424,7 @@ namespace lbfem_amd
   // (alpha = rho / p.q); 3: update (|r|, stopping test, rho, beta).
   template <int STAGE>
   __global__ void __launch_bounds__(kThreads)
-  k_cg_finalize(const CgWork W, const int nparts, const int it)
+  k_cg_finalize(const CgWork W, const int nparts, const int it, const int row0)
   {
     constexpr int stride = STAGE == 3 ? 2 : 3, nk = STAGE == 1 ? 3 : STAGE == 2 ? 1 : 2;
     constexpr int K      = stride * NM;
@@ -447,47 +445,44 @@ namespace lbfem_amd
         if (v < W.nv && (STAGE == 1 || status0[v] == 0))
 #pragma unroll
           for (int k = 0; k < nk; ++k)
-            acc[stride * v + k] += W.partial[size_t(bb) * kPart + stride * v + k];
+            acc[stride * v + k] += W.partial[size_t(row0 + bb) * kPart + stride * v + k];
     block_sum<K>(acc, red);
     if (threadIdx.x == 0)
 #pragma unroll
       for (int k = 0; k < K; ++k)
         tot[k] = acc[k];
-    if (threadIdx.x != 0)
+    __syncthreads();
+    // the scalars of vector v in thread v: one load and one store of its state, in parallel
+    // (a loop over the vectors in one thread serialises their dependent global accesses)
+    const int v = threadIdx.x;
+    if (v >= W.nv || (STAGE != 1 && status0[v] != 0))
       return;
-    for (int v = 0; v < W.nv; ++v)
+    CgState st = W.state[v];
+    if constexpr (STAGE == 1)
       {
-        CgState &st = W.state[v];
-        if constexpr (STAGE == 1)
-          {
-            st.res      = sqrt(tot[3 * v]);
-            st.tol      = W.tol_rel * sqrt(tot[3 * v + 1]);
-            st.rho      = tot[3 * v + 2];
-            st.rho_prev = 0.;
-            st.its      = 0;
-            // SolverControl::check(0, res): success first, then failure at the cap
-            st.status = st.res <= st.tol ? 1 : (W.max_its <= 0 || isnan(st.res)) ? 2 : 0;
-          }
-        else if constexpr (STAGE == 2)
-          {
-            if (status0[v] == 0)
-              st.alpha = st.rho / tot[3 * v];
-          }
-        else
-          {
-            if (status0[v] != 0)
-              continue;
-            st.res = sqrt(fabs(tot[2 * v]));
-            st.its = it;
-            if (st.res <= st.tol)
-              st.status = 1;
-            else if (it >= W.max_its || isnan(st.res))
-              st.status = 2;
-            st.rho_prev = st.rho;
-            st.rho      = tot[2 * v + 1];
-            st.beta     = st.rho / st.rho_prev;
-          }
+        st.res      = sqrt(tot[3 * v]);
+        st.tol      = W.tol_rel * sqrt(tot[3 * v + 1]);
+        st.rho      = tot[3 * v + 2];
+        st.rho_prev = 0.;
+        st.its      = 0;
+        // SolverControl::check(0, res): success first, then failure at the cap
+        st.status = st.res <= st.tol ? 1 : (W.max_its <= 0 || isnan(st.res)) ? 2 : 0;
       }
+    else if constexpr (STAGE == 2)
+      st.alpha = st.rho / tot[3 * v];
+    else
+      {
+        st.res = sqrt(fabs(tot[2 * v]));
+        st.its = it;
+        if (st.res <= st.tol)
+          st.status = 1;
+        else if (it >= W.max_its || isnan(st.res))
+          st.status = 2;
+        st.rho_prev = st.rho;
+        st.rho      = tot[2 * v + 1];
+        st.beta     = st.rho / st.rho_prev;
+      }
+    W.state[v] = st;
   }
 
   // x += alpha p; r += alpha q, with the partials of |r|^2 and r.D^-1 r
@@ -498,6 +493,7 @@ namespace lbfem_amd
               const V *__restrict__ q)
   {
     __shared__ double red[2 * NM][kThreads / 64];
+    __shared__ double alpha_s[NM];
     const int64_t     n = int64_t(G.np) * G.np;
     double            part[2 * NM];
 #pragma unroll
@@ -509,10 +505,33 @@ namespace lbfem_amd
 #pragma unroll
     for (int v = 0; v < NM; ++v)
       {
-        act[v]   = v < W.nv && W.state[v].status == 0;
-        alpha[v] = act[v] ? V(W.state[v].alpha) : V(0);
-        off[v]   = v < W.nv ? int64_t(W.block[v]) * G.stride : 0;
+        act[v] = v < W.nv && W.state[v].status == 0;
+        off[v] = v < W.nv ? int64_t(W.block[v]) * G.stride : 0;
       }
+    if (W.alpha_rows > 0)
+      {
+        // alpha = rho / p.q from the direction stage's partials, summed as k_cg_finalize<2>
+        // sums them (thread t takes rows t, t + 256, ..., then the block), so every block
+        // holds the same value, bit for bit, without a launch of its own
+        double pq[NM];
+#pragma unroll
+        for (int v = 0; v < NM; ++v)
+          pq[v] = 0.;
+        for (int bb = threadIdx.x; bb < W.alpha_rows; bb += kThreads)
+#pragma unroll
+          for (int v = 0; v < NM; ++v)
+            if (act[v])
+              pq[v] += W.partial[size_t(bb) * kPart + 3 * v];
+        block_sum<NM>(pq, red);
+        if (threadIdx.x == 0)
+#pragma unroll
+          for (int v = 0; v < NM; ++v)
+            alpha_s[v] = act[v] ? W.state[v].rho / pq[v] : 0.;
+        __syncthreads();
+      }
+#pragma unroll
+    for (int v = 0; v < NM; ++v)
+      alpha[v] = act[v] ? V(W.alpha_rows > 0 ? alpha_s[v] : W.state[v].alpha) : V(0);
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
       {
         const int    i = int(k % G.np), j = int(k / G.np);
@@ -540,9 +559,10 @@ namespace lbfem_amd
             }
       }
     block_sum<2 * NM>(part, red);
+    // after the direction stage's rows (other blocks may still be reading them for alpha)
     if (threadIdx.x == 0)
       for (int k = 0; k < 2 * NM; ++k)
-        W.partial[size_t(blockIdx.x) * kPart + k] = part[k];
+        W.partial[size_t(W.alpha_rows + blockIdx.x) * kPart + k] = part[k];
   }
 
   // ------------------------------------------------------------------ diagnostics
@@ -671,8 +691,9 @@ namespace lbfem_amd
             const hipError_t e = quad_operator_launch<KIND>(G, W, it, x, b, r, y, y2, s);
             if (e != hipSuccess)
               return e;
-            if (KIND != 0)
-              hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, quad_tile_count(G), it);
+            if (KIND != 0 && !(KIND == 2 && W.alpha_rows > 0))
+              hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, quad_tile_count(G), it,
+                                 0);
             return hipGetLastError();
           }
         else
@@ -700,9 +721,18 @@ namespace lbfem_amd
         default:
           return hipErrorInvalidValue;
       }
-    if (KIND != 0)
-      hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, nt, it);
+    if (KIND != 0 && !(KIND == 2 && W.alpha_rows > 0))
+      hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, nt, it, 0);
     return hipGetLastError();
+  }
+
+  int
+  cg_direction_rows(const Geo &G)
+  {
+    if (G.gen.vert != nullptr)
+      return quad_tile_count(G);
+    constexpr int TX = 32, TY = 16; // the tensor operator's tile (operator_launch)
+    return ((G.P + 1 + TX - 1) / TX) * ((G.P + 1 + TY - 1) / TY);
   }
 
   template <typename V>
@@ -710,9 +740,9 @@ namespace lbfem_amd
   update_launch(const Geo &G, const CgWork &W, const int it, V *x, V *r, const V *p, const V *q, hipStream_t s)
   {
     int nb = nodal_blocks(G);
-    nb     = std::min(nb, std::min(W.max_blocks, 1024));
+    nb     = std::min(nb, std::min(W.max_blocks - W.alpha_rows, 1024));
     hipLaunchKernelGGL(k_cg_update<V>, dim3(nb), dim3(kThreads), 0, s, G, W, x, r, p, q);
-    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kThreads), 0, s, W, nb, it);
+    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kThreads), 0, s, W, nb, it, W.alpha_rows);
     return hipGetLastError();
   }
 
