@@ -474,7 +474,7 @@ namespace lbfem_amd
     for (int al = 1; al < Q; ++al)
       {
         const double ex = S.e[al][0], ey = S.e[al][1];
-        double      *ra = rhs + (al - 1) * G.stride;
+        double      *ra = STEP ? nullptr : rhs + (al - 1) * G.stride; // the step writes no right-hand side
         if (ex == 0. && ey == 0.) // at rest in this sweep: r = 0 (streaming.h:118-122)
           {
             for (int k = tid; k < nown; k += kQT)
