@@ -28,7 +28,7 @@ Q = 9
 # kernel names of the tile variants (Cfg<T, p, TX, TY, RO, PS, mode>), fused Lee-Lin step, FP64, Q1
 TILE_KERNEL = {v: f"k_tile<Cfg<double, 1, {tx}, {ty}, {ps}, 1> >"
                for v, (tx, ty, ps) in enumerate(((32, 8, 1), (32, 16, 1), (32, 8, 2), (32, 8, 4), (16, 8, 4),
-                                                 (40, 8, 1), (48, 8, 1)))}
+                                                 (40, 8, 1), (48, 8, 1), (40, 8, 2)))}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_tile_cavity_q1_256.txt")
 
 

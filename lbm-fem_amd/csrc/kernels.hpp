@@ -37,7 +37,8 @@ namespace lbfem_amd
     kTileSmall4   = 4, // 16 x 8 nodes, 512 threads, split over 4
     kTileWide40   = 5, // 40 x 8 nodes, 320 threads: 231 tiles (one per CU) on a 257^2 node grid
     kTileWide48   = 6, // 48 x 8 nodes, 384 threads
-    kTileVariants = 7
+    kTileSplit2W40 = 7, // 40 x 8 nodes, 640 threads, split over 2 (about 1.5 node-waves per SIMD at 257^2)
+    kTileVariants = 8
   };
 
   template <typename T, int p>
@@ -80,7 +81,10 @@ namespace lbfem_amd
   inline void
   tile_dims(const int variant, int &tx, int &ty)
   {
-    tx = variant == kTileSmall4 ? 16 : variant == kTileWide40 ? 40 : variant == kTileWide48 ? 48 : 32;
+    tx = variant == kTileSmall4                                  ? 16 :
+         (variant == kTileWide40 || variant == kTileSplit2W40) ? 40 :
+         variant == kTileWide48                                  ? 48 :
+                                                                   32;
     ty = variant == kTileTall ? 16 : 8;
   }
 

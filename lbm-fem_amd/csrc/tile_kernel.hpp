@@ -707,6 +707,8 @@ namespace lbfem_amd
           return launch_tile<tile::Cfg<T, p, 40, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileWide48:
           return launch_tile<tile::Cfg<T, p, 48, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
+        case kTileSplit2W40:
+          return launch_tile<tile::Cfg<T, p, 40, 8, 2, MODE>>(G, S, fin, feq, out, rhs, stream);
         default:
           return hipErrorInvalidValue;
       }
@@ -733,6 +735,8 @@ namespace lbfem_amd
           return launch_batch<tile::Cfg<T, p, 40, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
         case kTileWide48:
           return launch_batch<tile::Cfg<T, p, 48, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+        case kTileSplit2W40:
+          return launch_batch<tile::Cfg<T, p, 40, 8, 2, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }

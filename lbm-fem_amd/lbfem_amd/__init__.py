@@ -20,8 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "liblbfem_hip.so")
 
 OK, EINVAL, ENOCONV, EHIP = 0, 1, 2, 3
-TILE_WIDE, TILE_TALL, TILE_SPLIT2, TILE_SPLIT4, TILE_SMALL4, TILE_WIDE40, TILE_WIDE48 = 0, 1, 2, 3, 4, 5, 6
-TILE_VARIANTS = 7
+TILE_WIDE, TILE_TALL, TILE_SPLIT2, TILE_SPLIT4, TILE_SMALL4, TILE_WIDE40, TILE_WIDE48, TILE_SPLIT2_W40 = range(8)
+TILE_VARIANTS = 8
 LEELIN, BARDOW = 0, 1
 TG2, TG3, TG3_SPLIT = 0, 1, 2
 MASS_CG, MASS_LUMPED, MASS_RICHARDSON = 0, 1, 2
