@@ -2039,8 +2039,8 @@ extern "C"
       if (desc->n_cells < 2 || desc->degree < 1 || desc->degree > kMaxDegree || !(desc->L > 0.) ||
           desc->stretch < 0.)
         return fail(LBFEM_EINVAL, "invalid mesh description (N >= 2, degree 1..3, L > 0)");
-      if (desc->n_periodic != 0)
-        return fail(LBFEM_EINVAL, "the stream function is computed for walled boxes (n_periodic = 0)");
+      if (desc->n_periodic != 0 || desc->distort != 0.)
+        return fail(LBFEM_EINVAL, "the stream function is computed for walled boxes (n_periodic = 0, no distortion)");
       const Basis1D B(desc->degree);
       const Axis1D  ax(B, box_vertices(desc->n_cells, desc->L, desc->stretch), false);
       const int     p = desc->degree, np = ax.P + 1, w = 2 * p + 1;

@@ -40,3 +40,10 @@ def test_stream_function_rejects_periodic():
     z = np.zeros(81)
     with pytest.raises(A.LbfemError):
         A.check(A.lib().lbfem_stream_function(A.C.byref(desc), A._p(z), A._p(z), A._p(z), None))
+
+
+def test_stream_function_rejects_a_distorted_mesh():
+    desc = A.MeshDesc(8, 1, 0, 1.0, 0.0, 0.02)
+    z = np.zeros(81)
+    with pytest.raises(A.LbfemError):
+        A.check(A.lib().lbfem_stream_function(A.C.byref(desc), A._p(z), A._p(z), A._p(z), None))
