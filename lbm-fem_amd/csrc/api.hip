@@ -1832,7 +1832,7 @@ extern "C"
           int64_t nodes = 0;
           for (const int k : g.members)
             nodes += int64_t(b->members[k]->d->G.np) * b->members[k]->d->G.np;
-          g.variant = nodes < 64LL * 4 * 1024 ? kTileSplit4 : kTileWide;
+          g.variant = size_default_variant(nodes);
         }
       HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       rebuild_tables(*b);

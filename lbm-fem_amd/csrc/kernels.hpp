@@ -131,16 +131,20 @@ namespace lbfem_amd
       }
   }
 
-  // The variant used until a measurement picks one: below about four wide-tile
-  // wavefronts per SIMD of the device (256 CUs x 4 SIMDs) a node's populations
-  // are split over four threads, so that the device fills.
+  // The variant used until a measurement picks one, by the node count of the launch: below
+  // about four wide-tile wavefronts per SIMD of the device (256 CUs x 4 SIMDs) the 40 x 8 tile
+  // split over two threads per node (the fastest measured shape at 257^2 nodes, Q1 and Q2,
+  // FP64 and FP32), else the 32 x 16 tile (the batch autotuner's pick for large groups).
+  inline int
+  size_default_variant(const long long nodes)
+  {
+    return nodes < 64LL * 4 * 1024 ? kTileSplit2W40 : kTileTall;
+  }
   inline int
   tile_default_variant(const Geo &G)
   {
-    const long long nodes = (long long)(G.P + 1) * (G.P + 1);
-    return nodes < 64LL * 4 * 1024 ? kTileSplit4 : kTileWide;
+    return size_default_variant((long long)(G.P + 1) * (G.P + 1));
   }
-
   // nodal operations over every node (collision.h:78-168)
   hipError_t collide_launch(const Geo &G, double *g, double omega, hipStream_t stream);
   hipError_t equilibrium_launch(const Geo &G, const double *f, double *feq, hipStream_t stream);
