@@ -141,7 +141,7 @@ def cpu_oracle_rate(threads):
     """The oracle's headline-workload rate on `threads` host threads, in a child process (no GPU there)."""
     import subprocess
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), LBFEM_ORACLE_OMP="1" if threads > 1 else "0")
-    steps = 40 if threads == 1 else 200
+    steps = 200 if threads == 1 else 600  # about 7 s on one core, 30 core-seconds on 16 threads
     r = subprocess.run([sys.executable, "-c", CPU_CHILD, ROOT, str(steps)], env=env, capture_output=True,
                        text=True, timeout=300, check=True)
     return json.loads(r.stdout.strip().splitlines()[-1])
