@@ -170,6 +170,7 @@ class Discretization:
         h = _vp()
         check(lib().lbfem_disc_create(C.byref(desc), C.byref(h)))
         self.h = h
+        self._destroy = lib().lbfem_disc_destroy  # held, so that teardown at interpreter exit finds it
         info = DiscInfo()
         check(lib().lbfem_disc_get_info(self.h, C.byref(info)))
         self.N, self.p = n_cells, degree
@@ -187,7 +188,7 @@ class Discretization:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().lbfem_disc_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def node_coordinates(self):
@@ -293,11 +294,12 @@ class Scheme:
         h = _vp()
         check(lib().lbfem_scheme_create(disc.h, C.byref(st), C.byref(h)))
         self.h = h
+        self._destroy = lib().lbfem_scheme_destroy
         self.dt = self.lam = None
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().lbfem_scheme_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def set_time_step(self, dt, lam):
@@ -407,12 +409,13 @@ class Batch:
         h = _vp()
         check(lib().lbfem_batch_create(arr, len(self.schemes), C.byref(h)))
         self.h = h
+        self._destroy = lib().lbfem_batch_destroy
         if graph_steps != 8:
             check(lib().lbfem_batch_set_graph_steps(self.h, int(graph_steps)))
 
     def close(self):
         if getattr(self, "h", None):
-            lib().lbfem_batch_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
