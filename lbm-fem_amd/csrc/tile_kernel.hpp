@@ -42,7 +42,8 @@ namespace lbfem_amd
     __device__ __forceinline__ double
     band(const double *B, const int i, const int o)
     {
-      return B[i * (2 * p + 1) + o + p];
+      // device (global) memory, also where the pointer comes from a batch table
+      return ((const __attribute__((address_space(1))) double *)B)[i * (2 * p + 1) + o + p];
     }
 
     // global node line of a tile position, -1 outside a non-periodic axis. A
@@ -136,6 +137,13 @@ namespace lbfem_amd
       // small differences instead of cancelling large values
       constexpr bool CEN        = sizeof(T) == 4;
       T(&F)[NF][HY][HX]         = sm.F;
+      // the population buffers are device (global) memory: say so, so that the loads and
+      // stores are global ones even where the pointers come from a batch table
+      using GT                   = __attribute__((address_space(1))) T;
+      const GT *const gfin       = (const GT *)fin;
+      const GT *const gfeq       = (const GT *)feq_in;
+      GT *const       gout       = (GT *)out;
+      GT *const       grhs       = (GT *)rhs_out;
       const int i0 = (tile % S.tiles_x) * TX, j0 = (tile / S.tiles_x) * C::TY;
       const int P = G.P, tid = threadIdx.x;
       const T   a_c = T(S.a), b_c = T(S.b), c_c = T(S.c), hdt2 = T(0.5 * S.dt * S.dt);
@@ -157,7 +165,7 @@ namespace lbfem_amd
               const int64_t idx = node_index(G, gis[r], gjs[r]);
 #pragma unroll
               for (int a = 0; a < Q; ++a)
-                v[r][a] = fin[a * G.stride + idx];
+                v[r][a] = gfin[a * G.stride + idx];
             }
           else
 #pragma unroll
@@ -187,7 +195,7 @@ namespace lbfem_amd
               else
 #pragma unroll
                 for (int a = 0; a < Q; ++a)
-                  fe[a] = feq_in[a * G.stride + node_index(G, gis[r], gjs[r])];
+                  fe[a] = gfeq[a * G.stride + node_index(G, gis[r], gjs[r])];
 #pragma unroll
               for (int a = 1; a < Q; ++a)
                 F[Q + a - 1][ky][kx] = a_c * v[r][a] + b_c * fe[a];
@@ -561,7 +569,7 @@ namespace lbfem_amd
           // a slave's own entries (the staged tile holds its master's); only nodal work applies
 #pragma unroll
           for (int a = 0; a < Q; ++a)
-            own[a] = fin[a * G.stride + idx];
+            own[a] = gfin[a * G.stride + idx];
           if constexpr (!LL)
             if (S.collide)
               collide_node(own, S.omega, wall, wx, wy);
@@ -590,18 +598,18 @@ namespace lbfem_amd
               own[a] += xinc[a - 1];
 #pragma unroll
           for (int a = 0; a < Q; ++a)
-            out[a * G.stride + idx] = own[a];
+            gout[a * G.stride + idx] = own[a];
         }
       else
         {
           // right-hand side (constrained entries get nothing) and, fused, g*
 #pragma unroll
           for (int a = 0; a < NM; ++a)
-            rhs_out[a * G.stride + idx] = slave ? T(0) : xinc[a];
+            grhs[a * G.stride + idx] = slave ? T(0) : xinc[a];
           if (out != nullptr)
 #pragma unroll
             for (int a = 0; a < Q; ++a)
-              out[a * G.stride + idx] = own[a];
+              gout[a * G.stride + idx] = own[a];
         }
     }
   } // namespace tile
