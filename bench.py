@@ -215,6 +215,21 @@ def main():
     log(f"cavity Q1 256^2 leelin lumped: {1e3 * sec / K:.4f} ms/step, {value:.1f} MLUPS, kernel {1e6 * t_kernel:.2f} us, "
         f"{achieved:.0f} GB/s algorithmic")
 
+    # ---- the stream function of the headline flow's velocity (include/lbfem/io.h:31-93): Laplace CG on the device
+    _, u_h, v_h = A.moments(s.f)
+    u_d = torch.tensor(u_h, device="cuda", dtype=torch.float64)
+    v_d = torch.tensor(v_h, device="cuda", dtype=torch.float64)
+    psi_d = torch.empty_like(u_d)
+    sf_its = A.disc_stream_function(disc, u_d, v_d, psi_d)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        A.disc_stream_function(disc, u_d, v_d, psi_d)
+    sf_ms = 1e3 * (time.perf_counter() - t0) / 3
+    stream_fn = {"workload": f"cavity Q1 256^2 velocity after {K + W} headline steps, CG 1e-12 |b|",
+                 "cg_iterations": sf_its, "ms": round(sf_ms, 3), "us_per_iteration": round(1e3 * sf_ms / sf_its, 2)}
+    log("stream function", stream_fn)
+    del u_d, v_d, psi_d
+
     variants = {}
     # ---- consistent mass (CG tol 1e-10), and Q2 128^2
     for name, N, p, mass in (("q1_256_leelin_cg", 256, 1, A.MASS_CG), ("q2_128_leelin_lumped", 128, 2, A.MASS_LUMPED),
@@ -420,6 +435,7 @@ def main():
         "cpu_baseline": cpu,
         "tile_variant": variant, "autotune_us": [round(1e6 * t, 3) for t in tune_s],
         "variants": variants,
+        "stream_function": stream_fn,
         "hbm_sweep": sweep,
         "precision": precision,
         "general_quadrilaterals": general,

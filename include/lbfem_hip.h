@@ -202,10 +202,16 @@ int lbfem_diagnostics(const lbfem_disc *disc, const double *f, const double *u_e
                       void *stream);
 
 /* The stream function of a nodal velocity field on a walled box mesh
- * (include/lbfem/io.h:31-93, compute_stream_function): (grad phi, grad psi) =
- * (dphi/dx, v) - (dphi/dy, u), psi = 0 on the walls, CG with Jacobi to 1e-12 |b|.
- * u, v, psi: host arrays of n_nodes (lexicographic); n_periodic must be 0.
- * Host computation (post-processing), no device needed. */
+ * (replaces lbfem::compute_stream_function, include/lbfem/io.h:31-93):
+ * (grad phi, grad psi) = (dphi/dx, v) - (dphi/dy, u), psi = 0 on the walls,
+ * SolverCG with the Jacobi preconditioner to 1e-12 |b| from zero, at most
+ * 10000 iterations (LBFEM_ENOCONV beyond). u, v, psi: device arrays of
+ * n_nodes (lexicographic) on the stream; the disc must be walled (n_periodic
+ * 0) and undistorted (LBFEM_EINVAL otherwise). Synchronises the stream. */
+int lbfem_disc_stream_function(const lbfem_disc *disc, const double *u, const double *v, double *psi,
+                               int *iterations, void *stream);
+/* The same from host arrays and a mesh description (the computation runs on
+ * the device, on a discretization of its own). */
 int lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const double *v, double *psi,
                           int *iterations);
 

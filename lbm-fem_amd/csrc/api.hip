@@ -449,13 +449,15 @@ namespace
 
   // CG on the vectors of W (include/lbfem/mass.h:204-211): returns LBFEM_OK or
   // LBFEM_ENOCONV; adds the iterations of all vectors to *its.
+  // The host reads the stopping state after iteration check_from and then every
+  // check_every iterations; iterations of converged vectors are no-ops.
   template <typename V>
   int
-  run_cg(const lbfem_disc &d, const CgWork &W, V *x, const V *b, V *r, V *p0, V *p1, V *q, CgState *host_state,
-         hipStream_t stream, int64_t *its, const int check_from, const bool fail_at_cap = true)
+  run_cg(const Geo &G, const CgWork &W, V *x, const V *b, V *r, V *p0, V *p1, V *q, CgState *host_state,
+         hipStream_t stream, int64_t *its, const int check_from, const bool fail_at_cap = true,
+         const int check_every = 1)
   {
-    const Geo &G  = d.G;
-    CgWork     Wf = W;
+    CgWork Wf = W;
     // small grids: the update kernel reduces the direction stage's partials itself (one
     // launch less per iteration); large ones keep the separate one-block finalisation
     const int rows = cg_direction_rows(G);
@@ -486,7 +488,7 @@ namespace
       {
         HIP_CHECK(cg_direction_launch(G, Wf, it, pb[it % 2], r, pb[(it + 1) % 2], q, stream));
         HIP_CHECK(cg_update_launch(G, Wf, it, x, r, pb[(it + 1) % 2], q, stream));
-        if (it >= check_from)
+        if (it >= check_from && (it - check_from) % check_every == 0)
           {
             bool      finished = false;
             const int st       = done(finished);
@@ -563,7 +565,7 @@ namespace
               ++W.nv;
             }
         int64_t   its = 0;
-        const int st  = run_cg(d, W, xs, s.rhs.p, s.r.p, s.pb[0].p, s.pb[1].p, s.q.p, s.host_state, s.stream, &its,
+        const int st  = run_cg(d.G, W, xs, s.rhs.p, s.r.p, s.pb[0].p, s.pb[1].p, s.q.p, s.host_state, s.stream, &its,
                                std::max(1, s.last_its - 1));
         s.cg_iterations += its;
         s.last_its = int((its + W.nv - 1) / std::max(1, W.nv));
@@ -617,7 +619,7 @@ namespace
                                        " refinements (SolverControl::NoConvergence)");
         HIP_CHECK(negate_to_float_launch(G, s.r.p, s.g32r.p, s.stream));
         int64_t   inner = 0;
-        const int st    = run_cg<float>(*s.d, W32, s.d32.p, s.g32r.p, s.r32.p, s.p32[0].p, s.p32[1].p, s.q32.p,
+        const int st    = run_cg<float>(s.d->G, W32, s.d32.p, s.g32r.p, s.r32.p, s.p32[0].p, s.p32[1].p, s.q32.p,
                                      s.host_state32, s.stream, &inner, 1, false);
         if (st != LBFEM_OK)
           return st;
@@ -1350,7 +1352,7 @@ extern "C"
       int       rc  = LBFEM_OK;
       try
         {
-          rc = run_cg(*d, W, x, r, d->s_r.p, d->s_p0.p, d->s_p1.p, d->s_q.p, hs, st, &its, 1);
+          rc = run_cg(d->G, W, x, r, d->s_r.p, d->s_p0.p, d->s_p1.p, d->s_q.p, hs, st, &its, 1);
         }
       catch (...)
         {
@@ -2025,10 +2027,64 @@ extern "C"
   // Stream function of a nodal velocity field (include/lbfem/io.h:31-93): with
   // u = -dpsi/dy, v = dpsi/dx, (grad phi, grad psi) = (dphi/dx, v) - (dphi/dy, u)
   // and psi = 0 on the walls, solved by deal.II's SolverCG sequence with the
-  // Jacobi preconditioner to 1e-12 |b|, at most 10000 iterations. On the tensor
-  // mesh the Laplacian is Kx (x) My + Mx (x) Ky and the right-hand side
-  // (Dx^T (x) My) v - (Mx (x) Dy^T) u, from the same assembled 1-D bands as the
-  // streaming operator. Host arithmetic: post-processing, not the time step.
+  // Jacobi preconditioner to 1e-12 |b| from zero, at most 10000 iterations. On
+  // the tensor mesh the Laplacian is Kx (x) My + Mx (x) Ky and the right-hand
+  // side (Dx^T (x) My) v - (Mx (x) Dy^T) u, from the same assembled 1-D bands as
+  // the streaming operator; the CG is the mass solves' (one vector, the operator
+  // tile in its Laplacian mode), its stopping state read every 16 iterations.
+  int
+  lbfem_disc_stream_function(const lbfem_disc *dc, const double *u, const double *v, double *psi, int *iterations,
+                             void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!dc || !u || !v || !psi)
+        return fail(LBFEM_EINVAL, "null argument");
+      if (dc->G.ax.periodic || dc->G.ay.periodic || dc->G.gen.vert != nullptr)
+        return fail(LBFEM_EINVAL, "the stream function is computed for walled boxes (n_periodic = 0, no distortion)");
+      auto       *d  = const_cast<lbfem_disc *>(dc); // scratch only
+      hipStream_t st = as_stream(stream);
+      if (iterations)
+        *iterations = 0;
+      d->ensure_scratch();
+      DevBuf<double> b, dinv, r, p0, p1, q;
+      for (DevBuf<double> *bb : {&b, &dinv, &r, &p0, &p1, &q})
+        bb->alloc(d->stride);
+      Geo G          = d->G;
+      G.gen.diag_inv = dinv.p; // the Jacobi preconditioner of the Laplacian
+      HIP_CHECK(psi_setup_launch(G, u, v, b.p, dinv.p, st));
+      HIP_CHECK(hipMemsetAsync(psi, 0, sizeof(double) * size_t(d->n), st));
+      CgWork W{};
+      W.nv         = 1;
+      W.block[0]   = 0;
+      W.max_its    = 10000;
+      W.tol_rel    = 1e-12;
+      W.state      = d->cg_state.p;
+      W.partial    = d->partial.p;
+      W.counter    = d->counter.p;
+      W.max_blocks = kMaxCgBlocks;
+      W.laplace    = 1;
+      CgState *hs  = nullptr;
+      HIP_CHECK(hipHostMalloc(&hs, sizeof(CgState) * NM));
+      int64_t its = 0;
+      int     rc  = LBFEM_OK;
+      try
+        {
+          rc = run_cg(G, W, psi, b.p, r.p, p0.p, p1.p, q.p, hs, st, &its, 16, true, 16);
+        }
+      catch (...)
+        {
+          (void)hipHostFree(hs);
+          throw;
+        }
+      (void)hipHostFree(hs);
+      if (iterations)
+        *iterations = int(its);
+      return rc;
+    });
+  }
+
+  // The same from host arrays and a mesh description: a discretization of its own,
+  // the fields copied to and from the device.
   int
   lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const double *v, double *psi,
                         int *iterations)
@@ -2036,97 +2092,23 @@ extern "C"
     return guarded([&]() -> int {
       if (!desc || !u || !v || !psi)
         return fail(LBFEM_EINVAL, "null argument");
-      if (desc->n_cells < 2 || desc->degree < 1 || desc->degree > kMaxDegree || !(desc->L > 0.) ||
-          desc->stretch < 0.)
-        return fail(LBFEM_EINVAL, "invalid mesh description (N >= 2, degree 1..3, L > 0)");
       if (desc->n_periodic != 0 || desc->distort != 0.)
         return fail(LBFEM_EINVAL, "the stream function is computed for walled boxes (n_periodic = 0, no distortion)");
-      const Basis1D B(desc->degree);
-      const Axis1D  ax(B, box_vertices(desc->n_cells, desc->L, desc->stretch), false);
-      const int     p = desc->degree, np = ax.P + 1, w = 2 * p + 1;
-      const size_t  n = size_t(np) * np;
-      auto          boundary = [&](const int i, const int j) { return i == 0 || j == 0 || i == ax.P || j == ax.P; };
-      // y = A x over the interior rows (constrained rows: identity)
-      auto apply = [&](const std::vector<double> &x, std::vector<double> &y, const std::vector<double> *Bx,
-                       const std::vector<double> *By, const std::vector<double> *Cx, const std::vector<double> *Cy) {
-        for (int j = 0; j < np; ++j)
-          for (int i = 0; i < np; ++i)
-            {
-              double s = 0.;
-              for (int oy = 0; oy < w; ++oy)
-                {
-                  const int jj = j + oy - p;
-                  if (jj < 0 || jj >= np)
-                    continue;
-                  const double by = (*By)[size_t(j) * w + oy], cy = (*Cy)[size_t(j) * w + oy];
-                  for (int ox = 0; ox < w; ++ox)
-                    {
-                      const int ii = i + ox - p;
-                      if (ii < 0 || ii >= np)
-                        continue;
-                      s += ((*Bx)[size_t(i) * w + ox] * by + (*Cx)[size_t(i) * w + ox] * cy) * x[size_t(jj) * np + ii];
-                    }
-                }
-              y[size_t(j) * np + i] = s;
-            }
-      };
-      std::vector<double> b(n), zero(n, 0.), uu(u, u + n), vv(v, v + n), tmp(n);
-      // b = (Dx^T (x) My) v - (Mx (x) Dy^T) u
-      apply(vv, b, &ax.DT, &ax.M, &zero, &zero);
-      apply(uu, tmp, &zero, &zero, &ax.M, &ax.DT);
-      std::vector<double> diag(n);
-      for (int j = 0; j < np; ++j)
-        for (int i = 0; i < np; ++i)
-          {
-            const size_t k = size_t(j) * np + i;
-            b[k]           = boundary(i, j) ? 0. : b[k] - tmp[k];
-            diag[k]        = boundary(i, j) ? 1. :
-                                              ax.K[size_t(i) * w + p] * ax.M[size_t(j) * w + p] +
-                                         ax.M[size_t(i) * w + p] * ax.K[size_t(j) * w + p];
-          }
-      auto laplace = [&](const std::vector<double> &x, std::vector<double> &y) {
-        apply(x, y, &ax.K, &ax.M, &ax.M, &ax.K);
-        for (int j = 0; j < np; ++j)
-          for (int i = 0; i < np; ++i)
-            if (boundary(i, j))
-              y[size_t(j) * np + i] = x[size_t(j) * np + i];
-      };
-      auto dot = [&](const std::vector<double> &a, const std::vector<double> &c) {
-        double s = 0.;
-        for (size_t k = 0; k < n; ++k)
-          s += a[k] * c[k];
-        return s;
-      };
-      // SolverCG (deal.II): x = 0, r = A x - b = -b
-      std::vector<double> x(n, 0.), r(n), pp(n, 0.), vq(n);
-      for (size_t k = 0; k < n; ++k)
-        r[k] = -b[k];
-      const double tol = 1e-12 * std::sqrt(dot(b, b));
-      double       res = std::sqrt(dot(r, r)), rho = 0., rho_prev = 1.;
-      int          it = 0;
-      for (; res > tol; ++it)
-        {
-          if (it >= 10000)
-            return fail(LBFEM_ENOCONV, "stream function: CG did not converge in 10000 iterations");
-          for (size_t k = 0; k < n; ++k)
-            vq[k] = r[k] / diag[k];
-          rho               = dot(r, vq);
-          const double beta = it == 0 ? 0. : rho / rho_prev;
-          for (size_t k = 0; k < n; ++k)
-            pp[k] = beta * pp[k] - vq[k];
-          laplace(pp, vq);
-          const double alpha = rho / dot(pp, vq);
-          for (size_t k = 0; k < n; ++k)
-            {
-              x[k] += alpha * pp[k];
-              r[k] += alpha * vq[k];
-            }
-          rho_prev = rho;
-          res      = std::sqrt(std::abs(dot(r, r)));
-        }
-      std::copy(x.begin(), x.end(), psi);
-      if (iterations)
-        *iterations = it;
+      lbfem_disc *raw = nullptr;
+      if (const int rc = lbfem_disc_create(desc, &raw); rc != LBFEM_OK)
+        return rc;
+      std::unique_ptr<lbfem_disc> d(raw);
+      const size_t                n = size_t(d->n);
+      DevBuf<double>              du, dv, dpsi;
+      du.alloc(n);
+      dv.alloc(n);
+      dpsi.alloc(n);
+      HIP_CHECK(hipMemcpy(du.p, u, n * sizeof(double), hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(dv.p, v, n * sizeof(double), hipMemcpyHostToDevice));
+      const int rc = lbfem_disc_stream_function(d.get(), du.p, dv.p, dpsi.p, iterations, nullptr);
+      if (rc != LBFEM_OK)
+        return rc;
+      HIP_CHECK(hipMemcpy(psi, dpsi.p, n * sizeof(double), hipMemcpyDeviceToHost));
       return LBFEM_OK;
     });
   }

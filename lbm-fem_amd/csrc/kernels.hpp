@@ -186,6 +186,10 @@ namespace lbfem_amd
     // > 0: the direction stage's partial rows, which the update kernel reduces itself
     // (every block, in the order the finalisation uses) instead of a separate launch
     int       alpha_rows;
+    // 1: the stream function's Laplacian Kx (x) My + Mx (x) Ky on a walled box instead of
+    // M + c K_e: wall entries read as zero, wall rows the identity; the Jacobi diagonal
+    // comes from Geo::gen.diag_inv
+    int       laplace;
   };
 
   // partial rows the CG direction stage writes (its blocks per vector)
@@ -239,6 +243,11 @@ namespace lbfem_amd
   hipError_t lumped_solve_launch(const Geo &G, double *x, const double *r, int nv, hipStream_t stream);
   hipError_t richardson_update_launch(const Geo &G, double *x, const double *r, const double *Ax, int nv,
                                       hipStream_t stream);
+
+  // the stream function's right-hand side b = (Dx^T (x) My) v - (Mx (x) Dy^T) u and inverse
+  // Jacobi diagonal 1 / (Kx My + Mx Ky)_ii, with b = 0 and 1 on the walls (io.h:55-85)
+  hipError_t psi_setup_launch(const Geo &G, const double *u, const double *v, double *b, double *dinv,
+                              hipStream_t stream);
 
   // diagnostics sums over non-slave nodes (cgdbe.cc:353-368)
   hipError_t stream_copy_launch(void *dst, const void *src, size_t bytes, hipStream_t s);

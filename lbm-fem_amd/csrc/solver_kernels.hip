@@ -266,7 +266,8 @@ namespace lbfem_amd
       {
         const int kx = k % HX, ky = k / HX;
         const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
-        const bool    ok  = gi >= 0 && gj >= 0;
+        // the Laplacian's wall entries are constrained to zero (read as zero by the cells)
+        const bool    ok  = gi >= 0 && gj >= 0 && !(W.laplace && (gi == 0 || gj == 0 || gi == P || gj == P));
         const int64_t idx = ok ? node_index(G, gi, gj) : 0;
         const V       d   = ok ? V(mass_diag_inv(G, gi, gj)) : V(1);
         V             xv[NM], rv[NM];
@@ -333,7 +334,25 @@ namespace lbfem_amd
 #pragma unroll
         for (int r = 0; r < RO; ++r)
           acc[r] = V(0);
-        if (c == V(0) || (ex == V(0) && ey == V(0))) // the mass matrix: Mx (x) My
+        if (W.laplace) // the stream function's Laplacian: Kx (x) My + Mx (x) Ky
+          {
+#pragma unroll
+            for (int cc = 0; cc < W_; ++cc)
+#pragma unroll
+              for (int r = 0; r < RO; ++r)
+                {
+                  V My = V(0), Ky = V(0);
+#pragma unroll
+                  for (int o = 0; o < W_; ++o)
+                    {
+                      const V xv = Xv[jr + r + o][ii + cc];
+                      My += yc[r][0][o] * xv;
+                      Ky += yc[r][2][o] * xv;
+                    }
+                  acc[r] += xc[2][cc] * My + xc[0][cc] * Ky;
+                }
+          }
+        else if (c == V(0) || (ex == V(0) && ey == V(0))) // the mass matrix: Mx (x) My
           {
 #pragma unroll
             for (int cc = 0; cc < W_; ++cc)
@@ -376,11 +395,11 @@ namespace lbfem_amd
             if (gi > P || gj > P)
               continue;
             const int64_t idx   = node_index(G, gi, gj);
-            const bool    slave = is_slave(G, gi, gj);
+            const bool    slave = is_slave(G, gi, gj) || (W.laplace && (gi == 0 || gj == 0 || gi == P || gj == P));
             V             own   = Xv[jr + r + p][ii + p];
             V             a     = acc[r];
-            if (slave) // constrained rows are the identity (MatrixFreeOperators::Base::vmult),
-              {        // on the slave's own entry (the staged tile holds its master's)
+            if (slave) // constrained rows are the identity (MatrixFreeOperators::Base::vmult), on the
+              {        // slave's own entry (the staged tile holds its master's, or zero on a wall)
                 if constexpr (KIND == 2)
                   own = it > 1 ? beta[v] * x[vb + idx] - r_in[vb + idx] : -r_in[vb + idx]; // D^-1 = 1
                 else
@@ -565,6 +584,50 @@ namespace lbfem_amd
         W.partial[size_t(W.alpha_rows + blockIdx.x) * kPart + k] = part[k];
   }
 
+  // ------------------------------------------------------------------ stream function
+  // The right-hand side of include/lbfem/io.h:61-81, (dphi/dx, v) - (dphi/dy, u), on the
+  // tensor mesh: b = (Dx^T (x) My) v - (Mx (x) Dy^T) u, with the 1-D bands D_io = int phi_i
+  // phi'_(i+o) and Dx^T its transpose; and the inverse of the Laplacian's diagonal
+  // Kx_ii My_jj + Mx_ii Ky_jj (LaplaceOperator::compute_diagonal). Wall rows: b = 0, 1.
+  template <int p>
+  __global__ void __launch_bounds__(kThreads)
+  k_psi_setup(const Geo G, const double *__restrict__ u, const double *__restrict__ v, double *__restrict__ b,
+              double *__restrict__ dinv)
+  {
+    const int     P = G.P;
+    const int64_t n = int64_t(G.np) * G.np;
+    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        if (i == 0 || j == 0 || i == P || j == P)
+          {
+            b[k]    = 0.;
+            dinv[k] = 1.;
+            continue;
+          }
+        double s = 0.;
+#pragma unroll
+        for (int oy = -p; oy <= p; ++oy)
+          {
+            const int jj = j + oy;
+            if (jj < 0 || jj > P)
+              continue;
+            const double my = band<p>(G.ay.M, j, oy), dty = band<p>(G.ay.DT, j, oy);
+#pragma unroll
+            for (int ox = -p; ox <= p; ++ox)
+              {
+                const int ii = i + ox;
+                if (ii < 0 || ii > P)
+                  continue;
+                const int64_t q = node_index(G, ii, jj);
+                s += band<p>(G.ax.DT, i, ox) * my * v[q] - band<p>(G.ax.M, i, ox) * dty * u[q];
+              }
+          }
+        b[k]    = s;
+        dinv[k] = 1. / (band<p>(G.ax.K, i, 0) * band<p>(G.ay.M, j, 0) + band<p>(G.ax.M, i, 0) * band<p>(G.ay.K, j, 0));
+      }
+  }
+
   // ------------------------------------------------------------------ diagnostics
   __global__ void __launch_bounds__(kThreads)
   k_diagnostics(const Geo G, const double *f, const double *u_exact, double *u_prev, double *partial,
@@ -664,6 +727,25 @@ namespace lbfem_amd
   extrapolate_launch(const Geo &G, double *x, double *x_prev, const unsigned mask, hipStream_t s)
   {
     hipLaunchKernelGGL(k_extrapolate, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, x_prev, mask);
+    return hipGetLastError();
+  }
+  hipError_t
+  psi_setup_launch(const Geo &G, const double *u, const double *v, double *b, double *dinv, hipStream_t s)
+  {
+    switch (G.p)
+      {
+        case 1:
+          hipLaunchKernelGGL(k_psi_setup<1>, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, u, v, b, dinv);
+          break;
+        case 2:
+          hipLaunchKernelGGL(k_psi_setup<2>, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, u, v, b, dinv);
+          break;
+        case 3:
+          hipLaunchKernelGGL(k_psi_setup<3>, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, u, v, b, dinv);
+          break;
+        default:
+          return hipErrorInvalidValue;
+      }
     return hipGetLastError();
   }
   hipError_t

@@ -106,6 +106,7 @@ def lib():
             "lbfem_diagnostics": (i, [_vp, _vp, _vp, _vp, p, _vp]),
             "lbfem_hbm_copy": (i, [_vp, _vp, C.c_size_t, _vp]),
             "lbfem_stream_function": (i, [C.POINTER(MeshDesc), p, p, p, C.POINTER(C.c_int)]),
+            "lbfem_disc_stream_function": (i, [_vp, _vp, _vp, _vp, C.POINTER(C.c_int), _vp]),
             "lbfem_batch_create": (i, [C.POINTER(_vp), i, C.POINTER(_vp)]),
             "lbfem_batch_destroy": (i, [_vp]),
             "lbfem_batch_set_graph_steps": (i, [_vp, i]),
@@ -390,6 +391,15 @@ def diagnostics(disc, f_dev_ptr, u_exact=None, u_prev=None, stream=None):
     return s
 
 
+def disc_stream_function(disc, u, v, psi, stream=None):
+    """lbfem_disc_stream_function: psi (device) of the device nodal velocity u, v on the walled box of disc
+    (include/lbfem/io.h:31-93); returns the CG iterations."""
+    its = C.c_int(0)
+    check(lib().lbfem_disc_stream_function(disc.h, C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()),
+                                            C.c_void_p(psi.data_ptr()), C.byref(its), _stream(stream)))
+    return its.value
+
+
 def hbm_copy(dst, src, stream=None):
     """lbfem_hbm_copy: device copy of src into dst (torch tensors of equal byte size, on the GPU)."""
     nbytes = src.numel() * src.element_size()
@@ -449,7 +459,7 @@ class Batch:
 
 def stream_function(n_cells, degree, u, v, L=1.0, stretch=0.0):
     """include/lbfem/io.h:31-93: psi with u = -dpsi/dy, v = dpsi/dx, psi = 0 on the walls of the box
-    (lexicographic nodal arrays); returns (psi, CG iterations). Host computation."""
+    (lexicographic nodal host arrays); returns (psi, CG iterations). Computed on the device."""
     u = np.ascontiguousarray(u, dtype=np.float64)
     v = np.ascontiguousarray(v, dtype=np.float64)
     psi = np.zeros_like(u)

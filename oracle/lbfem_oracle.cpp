@@ -1074,11 +1074,13 @@ namespace
   //   r = A x - b (r = -b if x == 0); res = |r|; check(0)
   //   loop: v = P r; rho = r.v; p = (it > 1 ? beta p : 0) - v; v = A p;
   //         alpha = rho / p.v; x += alpha p; r += alpha v; res = |r|; check(it)
+  // A(y, x) is the matrix-vector product, diag the matrix diagonal (1 on
+  // constrained rows), applied as DiagonalMatrix::vmult of the inverse entries.
+  template <class Vmult>
   int
-  cg_solve(const ora_disc &d, double *x, const double *b, const double tol_rel, const int max_its, const double c,
-           const double ex, const double ey, int *its)
+  cg_core(const int n, const double *diag, Vmult &&A, double *x, const double *b, const double tol_rel,
+          const int max_its, int *its)
   {
-    const int           n   = d.n;
     const double        tol = tol_rel * std::sqrt(dot(b, b, n));
     std::vector<double> r(n), v(n), p(n);
     bool                zero = true;
@@ -1086,7 +1088,7 @@ namespace
       zero = x[i] == 0.;
     if (!zero)
       {
-        streaming_vmult(d, r.data(), x, c, ex, ey);
+        A(r.data(), x);
         for (int i = 0; i < n; ++i)
           r[i] -= b[i];
       }
@@ -1112,7 +1114,7 @@ namespace
         ++it;
         rho_prev = rho;
         for (int i = 0; i < n; ++i) // DiagonalMatrix::vmult with the inverse entries 1/diag
-          v[i] = (1. / d.diag[i]) * r[i];
+          v[i] = (1. / diag[i]) * r[i];
         rho = dot(r.data(), v.data(), n);
         if (it > 1)
           {
@@ -1123,7 +1125,7 @@ namespace
         else
           for (int i = 0; i < n; ++i)
             p[i] = -v[i];
-        streaming_vmult(d, v.data(), p.data(), c, ex, ey);
+        A(v.data(), p.data());
         const double pAp   = dot(p.data(), v.data(), n);
         const double alpha = rho / pAp;
         for (int i = 0; i < n; ++i)
@@ -1146,6 +1148,15 @@ namespace
             return ORA_ENOCONV;
           }
       }
+  }
+
+  int
+  cg_solve(const ora_disc &d, double *x, const double *b, const double tol_rel, const int max_its, const double c,
+           const double ex, const double ey, int *its)
+  {
+    return cg_core(
+      d.n, d.diag.data(), [&](double *y, const double *v) { streaming_vmult(d, y, v, c, ex, ey); }, x, b, tol_rel,
+      max_its, its);
   }
 
   // include/lbfem/mass.h:184-213
@@ -1176,6 +1187,141 @@ namespace
         return ORA_OK;
       }
     return cg_solve(d, x, r, cg_tol, cg_max, tg3, ex, ey, its);
+  }
+
+  // ---------------------------------------------------------------- stream function
+  // include/lbfem/io.h:31-93 (compute_stream_function) on a walled box: the
+  // zero boundary constraints make the wall nodes constrained (read as zero by
+  // the cells, identity rows, nothing distributed to them).
+  bool
+  on_wall(const ora_disc &d, const int g)
+  {
+    const int i = g % d.np, j = g / d.np, P = d.p * d.N;
+    return i == 0 || j == 0 || i == P || j == P;
+  }
+
+  // values of a cell field at the Gauss points (FEEvaluation::evaluate(values))
+  void
+  cell_values(const ora_disc &d, const double *G, double *u)
+  {
+    const int      nn = d.p + 1;
+    const Basis1D &B  = d.B;
+    double         T[25];
+    for (int ky = 0; ky < nn; ++ky)
+      for (int qx = 0; qx < nn; ++qx)
+        {
+          double s = 0.;
+          for (int kx = 0; kx < nn; ++kx)
+            s += B.S[qx * nn + kx] * G[ky * nn + kx];
+          T[ky * nn + qx] = s;
+        }
+    for (int qy = 0; qy < nn; ++qy)
+      for (int qx = 0; qx < nn; ++qx)
+        {
+          double s = 0.;
+          for (int ky = 0; ky < nn; ++ky)
+            s += B.S[qy * nn + ky] * T[ky * nn + qx];
+          u[qy * nn + qx] = s;
+        }
+  }
+
+  // the local Laplace operator of cell (cx, cy) on the cell values G
+  // (MatrixFreeOperators::LaplaceOperator: integrate(gradients) of the gradients times JxW)
+  void
+  cell_laplace(const ora_disc &d, const int cx, const int cy, const double *G, double *R)
+  {
+    const int    nn = d.p + 1, nq = nn * nn;
+    const double hx = d.hx(cx), hy = d.hy(cy);
+    double       gx[25], gy[25], z[25];
+    cell_gradients(d, G, hx, hy, gx, gy);
+    for (int qy = 0; qy < nn; ++qy)
+      for (int qx = 0; qx < nn; ++qx)
+        {
+          const int    q   = qy * nn + qx;
+          const double jxw = hx * hy * d.B.qw[qx] * d.B.qw[qy];
+          gx[q] *= jxw;
+          gy[q] *= jxw;
+        }
+    for (int q = 0; q < nq; ++q)
+      z[q] = 0.;
+    cell_integrate(d, z, gx, gy, hx, hy, R);
+  }
+
+  // y = A x: the wall entries of x read as zero, wall rows the identity
+  void
+  laplace_vmult(const ora_disc &d, double *y, const double *x)
+  {
+    const int           N = d.N, nn = d.p + 1, nq = nn * nn;
+    std::vector<double> Rc((size_t)N * N * nq);
+    for (int cy = 0; cy < N; ++cy)
+      for (int cx = 0; cx < N; ++cx)
+        {
+          double G[25];
+          for (int ky = 0; ky < nn; ++ky)
+            for (int kx = 0; kx < nn; ++kx)
+              {
+                const int g          = d.cell_node(cx, cy, kx, ky);
+                G[ky * nn + kx]      = on_wall(d, g) ? 0. : x[g];
+              }
+          cell_laplace(d, cx, cy, G, Rc.data() + (size_t)(cy * N + cx) * nq);
+        }
+    d.assemble(Rc.data(), y);
+    for (int g = 0; g < d.n; ++g)
+      if (on_wall(d, g))
+        y[g] = x[g];
+  }
+
+  int
+  stream_function(const ora_disc &d, const double *u, const double *v, double *psi, int *its)
+  {
+    const int           N = d.N, nn = d.p + 1, nq = nn * nn, n = d.n;
+    std::vector<double> Rc((size_t)N * N * nq), b(n), diag(n);
+    // right-hand side (dphi/dx, v) - (dphi/dy, u) (io.h:61-81)
+    for (int cy = 0; cy < N; ++cy)
+      for (int cx = 0; cx < N; ++cx)
+        {
+          const double hx = d.hx(cx), hy = d.hy(cy);
+          double       G[25], uq[25], vq[25], Fx[25], Fy[25], z[25];
+          d.gather(u, cx, cy, G);
+          cell_values(d, G, uq);
+          d.gather(v, cx, cy, G);
+          cell_values(d, G, vq);
+          for (int qy = 0; qy < nn; ++qy)
+            for (int qx = 0; qx < nn; ++qx)
+              {
+                const int    q   = qy * nn + qx;
+                const double jxw = hx * hy * d.B.qw[qx] * d.B.qw[qy];
+                Fx[q]            = vq[q] * jxw;
+                Fy[q]            = -uq[q] * jxw;
+                z[q]             = 0.;
+              }
+          cell_integrate(d, z, Fx, Fy, hx, hy, Rc.data() + (size_t)(cy * N + cx) * nq);
+        }
+    d.assemble(Rc.data(), b.data());
+    // the diagonal, cell by cell on unit vectors (LaplaceOperator::compute_diagonal)
+    for (int cy = 0; cy < N; ++cy)
+      for (int cx = 0; cx < N; ++cx)
+        for (int k = 0; k < nq; ++k)
+          {
+            double G[25], R[25];
+            for (int m = 0; m < nq; ++m)
+              G[m] = m == k ? 1. : 0.;
+            cell_laplace(d, cx, cy, G, R);
+            Rc[(size_t)(cy * N + cx) * nq + k] = R[k];
+          }
+    d.assemble(Rc.data(), diag.data());
+    for (int g = 0; g < n; ++g)
+      if (on_wall(d, g))
+        {
+          b[g]    = 0.;
+          diag[g] = 1.;
+        }
+    // SolverCG(10000, 1e-12 |b|) from psi = 0 with the Jacobi preconditioner (io.h:83-85)
+    for (int g = 0; g < n; ++g)
+      psi[g] = 0.;
+    return cg_core(
+      n, diag.data(), [&](double *y, const double *x) { laplace_vmult(d, y, x); }, psi, b.data(), 1e-12, 10000,
+      its);
   }
 } // namespace
 
@@ -1646,6 +1792,18 @@ extern "C"
           }
       }
     return ORA_OK;
+  }
+
+  int
+  ora_stream_function(const ora_disc *d, const double *u, const double *v, double *psi, int *its)
+  {
+    if (d->per[0] || d->per[1] || d->general)
+      return ORA_EINVAL;
+    int       it = 0;
+    const int st = stream_function(*d, u, v, psi, &it);
+    if (its)
+      *its = it;
+    return st;
   }
 
   void

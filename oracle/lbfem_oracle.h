@@ -93,6 +93,13 @@ void ora_scheme_get_increments(const ora_scheme *s, int which, double *x);
  * |u-u_ex|^2, |u_ex|^2, |u|^2, rho, area, |u-u_prev| (u_prev given, 2 n). */
 void ora_diagnostics(const ora_disc *d, const double *f, double t, const double *u_prev, double *sums);
 
+/* Stream function of a nodal velocity field u, v (n each) on a walled,
+ * undistorted box (include/lbfem/io.h:31-93): (grad phi, grad psi) =
+ * (dphi/dx, v) - (dphi/dy, u), psi = 0 on the walls, SolverCG to 1e-12 |b|
+ * from zero with the Jacobi preconditioner. ORA_OK, ORA_ENOCONV, or ORA_EINVAL
+ * on a periodic or general mesh. */
+int ora_stream_function(const ora_disc *d, const double *u, const double *v, double *psi, int *its);
+
 #ifdef __cplusplus
 }
 #endif

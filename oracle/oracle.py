@@ -85,6 +85,7 @@ def lib():
             "ora_scheme_cg_iterations": (C.c_longlong, [vp]),
             "ora_scheme_get_increments": (None, [vp, C.c_int, _P]),
             "ora_diagnostics": (None, [vp, _P, C.c_double, _P, _P]),
+            "ora_stream_function": (C.c_int, [vp, _P, _P, _P, C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -260,6 +261,16 @@ class Disc:
         s = np.zeros(6)
         lib().ora_diagnostics(self.h, _p(f), t, _p(up), _p(s))
         return s
+
+    def stream_function(self, u, v):
+        """psi of the nodal velocity (u, v) on a walled box (include/lbfem/io.h:31-93);
+        returns (psi, CG iterations, status)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        psi = np.zeros(self.n)
+        its = C.c_int(0)
+        st = lib().ora_stream_function(self.h, _p(u), _p(v), _p(psi), C.byref(its))
+        return psi, its.value, st
 
 
 class Scheme:
