@@ -340,7 +340,7 @@ struct lbfem_scheme
   std::vector<std::array<double, 2 * Q>> sweeps;
   double                tg3         = 0.;
   bool                  extrapolate = false;
-  int                   last_its    = 1;
+  int                   last_its    = 1; // the largest CG count of the last solve
   int                   tile_variant = -1; // TileVariant of the tile kernel; -1: tile_default_variant
   // stage timers (include/lbfem/timers.h): collision, advection, mass, total
   bool                                   timing = false;
@@ -564,11 +564,15 @@ namespace
               W.vec[W.nv].coef   = s.tg3 * s.dt * s.dt;
               ++W.nv;
             }
+        // the host first reads the stopping state at the last solve's largest count (a vector
+        // that converges earlier idles through no-op iterations, cheaper than a host round trip)
         int64_t   its = 0;
         const int st  = run_cg(d.G, W, xs, s.rhs.p, s.r.p, s.pb[0].p, s.pb[1].p, s.q.p, s.host_state, s.stream, &its,
-                               std::max(1, s.last_its - 1));
+                               std::max(1, s.last_its));
         s.cg_iterations += its;
-        s.last_its = int((its + W.nv - 1) / std::max(1, W.nv));
+        s.last_its = 0;
+        for (int v = 0; v < W.nv; ++v)
+          s.last_its = std::max(s.last_its, s.host_state[v].its);
         if (st != LBFEM_OK)
           return st;
       }

@@ -222,20 +222,50 @@ namespace lbfem_amd
   // ------------------------------------------------------------------ operator tile
   // KIND 0: y = A x.  KIND 1 (CG start): r = A x - b, partials |r|^2, |b|^2, r.D^-1 r.
   // KIND 2 (CG direction): p_new = beta p_old - D^-1 r, q = A p_new, partial p_new.q.
-  // A = M + c K_e = Mx (My + c ey^2 Ky) + Kx (c ex^2 My) + Dx^T (c ex ey Dy) + Dx (c ex ey Dy^T).
-  // The operands of all active vectors are staged in LDS at once (one barrier), then
+  // OP kOpMass: A = Mx (x) My (TG2 mass solves, Richardson, the vectors without a K_e term).
+  // OP kOpStreaming: A = M + c K_e = Mx (My + c ey^2 Ky) + Kx (c ex^2 My) + Dx^T (c ex ey Dy)
+  //   + Dx (c ex ey Dy^T) (mass.h:105-144; a vector with c = 0 or e = 0 takes the mass branch).
+  // OP kOpLaplace: the stream function's A = Kx (x) My + Mx (x) Ky on a walled box (io.h:55-85):
+  //   wall entries read as zero, wall rows the identity, the Jacobi diagonal from gen.diag_inv.
+  // A thread keeps only the 1-D bands its OP uses (registers set the occupancy).
+  // One vector per block (blockIdx.y): its operand tile is staged in LDS (one barrier), then
   // each thread applies the bands at its RO owned nodes of one column.
   // V: the vector type, double (the FP64 solves and the refinement residuals) or
   // float (the inner solves of the mixed form); scalars and partial sums in FP64.
-  template <typename V, int p, int TX, int TY, int KIND>
+  enum
+  {
+    kOpMass      = 0,
+    kOpStreaming = 1,
+    kOpLaplace   = 2
+  };
+
+  template <int OP>
+  struct OpBands
+  {
+    // bands a thread holds: M; M, K; or M, D, K, D^T (slot of each, -1 when absent)
+    static constexpr int NB = OP == kOpMass ? 1 : OP == kOpLaplace ? 2 : 4;
+    static constexpr int M = 0, K = OP == kOpMass ? -1 : OP == kOpLaplace ? 1 : 2;
+    static constexpr int D = OP == kOpStreaming ? 1 : -1, DT = OP == kOpStreaming ? 3 : -1;
+  };
+
+  __device__ __forceinline__ const double *
+  band_array(const AxisDev &a, const int which) // 0 M, 1 D, 2 K, 3 D^T
+  {
+    return which == 0 ? a.M : which == 1 ? a.D : which == 2 ? a.K : a.DT;
+  }
+
+  template <typename V, int p, int TX, int TY, int KIND, int OP>
   __global__ void __launch_bounds__(kThreads)
   k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const V *x,
              const V *b, const V *r_in, V *y, V *y2)
   {
-    constexpr int W_ = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
-    constexpr int TR = kThreads / TX, RO = TY / TR;
+    using B                  = OpBands<OP>;
+    constexpr int  W_        = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
+    constexpr int  TR        = kThreads / TX, RO = TY / TR;
+    constexpr bool LAP       = OP == kOpLaplace;
+    constexpr int  which[4]  = {0, OP == kOpLaplace ? 2 : 1, 2, 3}; // band array of each slot
     static_assert(TY % TR == 0, "tile shape");
-    __shared__ V X[1][HY][HX];
+    __shared__ V      X[HY][HX];
     __shared__ double red[3][kThreads / 64];
 
     const int nt   = tiles_x * tiles_y;
@@ -246,113 +276,94 @@ namespace lbfem_amd
       return;
     const int i0 = (tile % tiles_x) * TX, j0 = (tile / tiles_x) * TY;
     const int P = G.P, tid = threadIdx.x;
+    const auto wall = [P](const int i, const int j) { return i == 0 || j == 0 || i == P || j == P; };
 
-    // active vectors (uniform): the start and plain products take every vector
-    // this block's vector (blockIdx.y); a converged one is skipped (uniform)
-    const int vsel = blockIdx.y;
-    if (KIND == 2 && W.state[vsel].status != 0)
+    // this block's vector (blockIdx.y, uniform); a converged one is skipped
+    const int v = blockIdx.y;
+    if (KIND == 2 && W.state[v].status != 0)
       {
         if (tid == 0)
-          W.partial[size_t(tile) * kPart + 3 * vsel] = 0.;
+          W.partial[size_t(tile) * kPart + 3 * v] = 0.;
         return;
       }
-    const unsigned act = 1u << vsel;
-    V beta[NM];
-    for (int v = 0; v < NM; ++v)
-      beta[v] = (v == vsel && KIND == 2 && it > 1) ? V(W.state[v].beta) : V(0);
+    const V       beta = (KIND == 2 && it > 1) ? V(W.state[v].beta) : V(0);
+    const int64_t vb   = int64_t(W.block[v]) * G.stride;
 
-    // stage the operands (KIND 2: the new direction) of the halo'd block
+    // stage the operand (KIND 2: the new direction) of the halo'd block
     for (int k = tid; k < NH; k += kThreads)
       {
         const int kx = k % HX, ky = k / HX;
         const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
         // the Laplacian's wall entries are constrained to zero (read as zero by the cells)
-        const bool    ok  = gi >= 0 && gj >= 0 && !(W.laplace && (gi == 0 || gj == 0 || gi == P || gj == P));
-        const int64_t idx = ok ? node_index(G, gi, gj) : 0;
-        const V       d   = ok ? V(mass_diag_inv(G, gi, gj)) : V(1);
-        V             xv[NM], rv[NM];
-#pragma unroll
-        for (int v = 0; v < NM; ++v)
+        const bool ok  = gi >= 0 && gj >= 0 && !(LAP && wall(gi, gj));
+        V          val = V(0);
+        if (ok)
           {
-            xv[v] = rv[v] = V(0);
-            if (ok && (act & (1u << v)))
+            const int64_t idx = node_index(G, gi, gj);
+            const V       xv  = (KIND != 2 || it > 1) ? x[vb + idx] : V(0);
+            if constexpr (KIND == 2)
               {
-                const int64_t vb = int64_t(W.block[v]) * G.stride;
-                if (KIND != 2 || it > 1)
-                  xv[v] = x[vb + idx];
-                if constexpr (KIND == 2)
-                  rv[v] = r_in[vb + idx];
+                const V z = V(mass_diag_inv(G, gi, gj)) * r_in[vb + idx];
+                val       = it > 1 ? beta * xv - z : -z;
               }
+            else
+              val = xv;
           }
-#pragma unroll
-        for (int v = 0; v < NM; ++v)
-          if (act & (1u << v))
-            {
-              V val = xv[v];
-              if constexpr (KIND == 2)
-                {
-                  const V z = d * rv[v];
-                  val            = it > 1 ? beta[v] * xv[v] - z : -z;
-                }
-              X[0][ky][kx] = val;
-            }
+        X[ky][kx] = val;
       }
     const int ii = tid % TX, jr = (tid / TX) * RO;
     const int gi = i0 + ii;
-    V         xc[4][W_], yc[RO][4][W_];
+    V         xc[B::NB][W_], yc[RO][B::NB][W_];
 #pragma unroll
     for (int o = 0; o < W_; ++o)
       {
         const bool in = gi <= P;
-        xc[0][o]      = in ? V(band<p>(G.ax.M, gi, o - p)) : V(0);
-        xc[1][o]      = in ? V(band<p>(G.ax.D, gi, o - p)) : V(0);
-        xc[2][o]      = in ? V(band<p>(G.ax.K, gi, o - p)) : V(0);
-        xc[3][o]      = in ? V(band<p>(G.ax.DT, gi, o - p)) : V(0);
+#pragma unroll
+        for (int t = 0; t < B::NB; ++t)
+          xc[t][o] = in ? V(band<p>(band_array(G.ax, which[t]), gi, o - p)) : V(0);
 #pragma unroll
         for (int r = 0; r < RO; ++r)
           {
             const int  gj = j0 + jr + r;
             const bool iy = gj <= P;
-            yc[r][0][o]   = iy ? V(band<p>(G.ay.M, gj, o - p)) : V(0);
-            yc[r][1][o]   = iy ? V(band<p>(G.ay.D, gj, o - p)) : V(0);
-            yc[r][2][o]   = iy ? V(band<p>(G.ay.K, gj, o - p)) : V(0);
-            yc[r][3][o]   = iy ? V(band<p>(G.ay.DT, gj, o - p)) : V(0);
+#pragma unroll
+            for (int t = 0; t < B::NB; ++t)
+              yc[r][t][o] = iy ? V(band<p>(band_array(G.ay, which[t]), gj, o - p)) : V(0);
           }
       }
     __syncthreads();
 
-    // this block's vector only; its partial sums in scalars (an array indexed by a run-time
-    // vector number would live in scratch memory)
+    // partial sums in scalars (an array indexed by a run-time vector number would live in scratch memory)
     double part0 = 0., part1 = 0., part2 = 0.;
-    {
+    V      acc[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r)
+      acc[r] = V(0);
+    const V c = V(W.vec[v].coef), ex = V(W.vec[v].e[0]), ey = V(W.vec[v].e[1]);
+    if constexpr (OP == kOpLaplace) // Kx (x) My + Mx (x) Ky
       {
-        const int     v = vsel;
-        const V       c = V(W.vec[v].coef), ex = V(W.vec[v].e[0]), ey = V(W.vec[v].e[1]);
-        const int64_t vb = int64_t(W.block[v]) * G.stride;
-        const V(*Xv)[HX] = X[0];
-        V acc[RO];
 #pragma unroll
-        for (int r = 0; r < RO; ++r)
-          acc[r] = V(0);
-        if (W.laplace) // the stream function's Laplacian: Kx (x) My + Mx (x) Ky
-          {
+        for (int cc = 0; cc < W_; ++cc)
 #pragma unroll
-            for (int cc = 0; cc < W_; ++cc)
+          for (int r = 0; r < RO; ++r)
+            {
+              V My = V(0), Ky = V(0);
 #pragma unroll
-              for (int r = 0; r < RO; ++r)
+              for (int o = 0; o < W_; ++o)
                 {
-                  V My = V(0), Ky = V(0);
-#pragma unroll
-                  for (int o = 0; o < W_; ++o)
-                    {
-                      const V xv = Xv[jr + r + o][ii + cc];
-                      My += yc[r][0][o] * xv;
-                      Ky += yc[r][2][o] * xv;
-                    }
-                  acc[r] += xc[2][cc] * My + xc[0][cc] * Ky;
+                  const V xv = X[jr + r + o][ii + cc];
+                  My += yc[r][B::M][o] * xv;
+                  Ky += yc[r][B::K][o] * xv;
                 }
-          }
-        else if (c == V(0) || (ex == V(0) && ey == V(0))) // the mass matrix: Mx (x) My
+              acc[r] += xc[B::K][cc] * My + xc[B::M][cc] * Ky;
+            }
+      }
+    else
+      {
+        bool mass_only = true;
+        if constexpr (OP == kOpStreaming)
+          mass_only = c == V(0) || (ex == V(0) && ey == V(0));
+        if (mass_only) // Mx (x) My
           {
 #pragma unroll
             for (int cc = 0; cc < W_; ++cc)
@@ -362,11 +373,11 @@ namespace lbfem_amd
                   V s = V(0);
 #pragma unroll
                   for (int o = 0; o < W_; ++o)
-                    s += yc[r][0][o] * Xv[jr + r + o][ii + cc];
-                  acc[r] += xc[0][cc] * s;
+                    s += yc[r][B::M][o] * X[jr + r + o][ii + cc];
+                  acc[r] += xc[B::M][cc] * s;
                 }
           }
-        else
+        else if constexpr (OP == kOpStreaming)
           {
             const V kyy = c * ey * ey, kxx = c * ex * ex, kxy = c * ex * ey;
 #pragma unroll
@@ -378,62 +389,61 @@ namespace lbfem_amd
 #pragma unroll
                   for (int o = 0; o < W_; ++o)
                     {
-                      const V xv = Xv[jr + r + o][ii + cc];
-                      My += yc[r][0][o] * xv;
-                      Ky += yc[r][2][o] * xv;
-                      Dy += yc[r][1][o] * xv;
-                      DTy += yc[r][3][o] * xv;
+                      const V xv = X[jr + r + o][ii + cc];
+                      My += yc[r][B::M][o] * xv;
+                      Ky += yc[r][B::K][o] * xv;
+                      Dy += yc[r][B::D][o] * xv;
+                      DTy += yc[r][B::DT][o] * xv;
                     }
-                  acc[r] += xc[0][cc] * (My + kyy * Ky) + xc[2][cc] * (kxx * My) + xc[3][cc] * (kxy * Dy) +
-                            xc[1][cc] * (kxy * DTy);
+                  acc[r] += xc[B::M][cc] * (My + kyy * Ky) + xc[B::K][cc] * (kxx * My) + xc[B::DT][cc] * (kxy * Dy) +
+                            xc[B::D][cc] * (kxy * DTy);
                 }
           }
+      }
 #pragma unroll
-        for (int r = 0; r < RO; ++r)
-          {
-            const int gj = j0 + jr + r;
-            if (gi > P || gj > P)
-              continue;
-            const int64_t idx   = node_index(G, gi, gj);
-            const bool    slave = is_slave(G, gi, gj) || (W.laplace && (gi == 0 || gj == 0 || gi == P || gj == P));
-            V             own   = Xv[jr + r + p][ii + p];
-            V             a     = acc[r];
-            if (slave) // constrained rows are the identity (MatrixFreeOperators::Base::vmult), on the
-              {        // slave's own entry (the staged tile holds its master's, or zero on a wall)
-                if constexpr (KIND == 2)
-                  own = it > 1 ? beta[v] * x[vb + idx] - r_in[vb + idx] : -r_in[vb + idx]; // D^-1 = 1
-                else
-                  own = x[vb + idx];
-                a = own;
-              }
-            if constexpr (KIND == 0)
-              y[vb + idx] = a;
-            else if constexpr (KIND == 1)
-              {
-                const V bb  = b[vb + idx];
-                const V rr  = a - bb;
-                y[vb + idx] = rr;
-                const double r8 = double(rr), b8 = double(bb);
-                part0 += r8 * r8;
-                part1 += b8 * b8;
-                part2 += r8 * (mass_diag_inv(G, gi, gj) * r8);
-              }
+    for (int r = 0; r < RO; ++r)
+      {
+        const int gj = j0 + jr + r;
+        if (gi > P || gj > P)
+          continue;
+        const int64_t idx   = node_index(G, gi, gj);
+        const bool    slave = is_slave(G, gi, gj) || (LAP && wall(gi, gj));
+        V             own   = X[jr + r + p][ii + p];
+        V             a     = acc[r];
+        if (slave) // constrained rows are the identity (MatrixFreeOperators::Base::vmult), on the
+          {        // slave's own entry (the staged tile holds its master's, or zero on a wall)
+            if constexpr (KIND == 2)
+              own = it > 1 ? beta * x[vb + idx] - r_in[vb + idx] : -r_in[vb + idx]; // D^-1 = 1
             else
-              {
-                y[vb + idx]  = own; // p_new
-                y2[vb + idx] = a;   // q = A p_new
-                part0 += double(own) * double(a);
-              }
+              own = x[vb + idx];
+            a = own;
+          }
+        if constexpr (KIND == 0)
+          y[vb + idx] = a;
+        else if constexpr (KIND == 1)
+          {
+            const V bb  = b[vb + idx];
+            const V rr  = a - bb;
+            y[vb + idx] = rr;
+            const double r8 = double(rr), b8 = double(bb);
+            part0 += r8 * r8;
+            part1 += b8 * b8;
+            part2 += r8 * (mass_diag_inv(G, gi, gj) * r8);
+          }
+        else
+          {
+            y[vb + idx]  = own; // p_new
+            y2[vb + idx] = a;   // q = A p_new
+            part0 += double(own) * double(a);
           }
       }
-    }
     if constexpr (KIND != 0)
       {
         double pv[3] = {part0, part1, part2};
         block_sum<3>(pv, red);
         if (tid == 0)
           for (int k = 0; k < (KIND == 1 ? 3 : 1); ++k)
-            W.partial[size_t(tile) * kPart + 3 * vsel + k] = pv[k];
+            W.partial[size_t(tile) * kPart + 3 * v + k] = pv[k];
       }
   }
 
@@ -441,35 +451,54 @@ namespace lbfem_amd
   // fixed summation order (a kernel boundary, not a device-wide fence, orders it
   // after the partials). STAGE 1: start (|r|, |b|, r.D^-1 r); 2: direction
   // (alpha = rho / p.q); 3: update (|r|, stopping test, rho, beta).
+  // One block (1024 threads measured no faster at a million nodes, and slower on small meshes).
+  constexpr int kFinThreads = kThreads;
+
   template <int STAGE>
-  __global__ void __launch_bounds__(kThreads)
+  __global__ void __launch_bounds__(kFinThreads)
   k_cg_finalize(const CgWork W, const int nparts, const int it, const int row0)
   {
     constexpr int stride = STAGE == 3 ? 2 : 3, nk = STAGE == 1 ? 3 : STAGE == 2 ? 1 : 2;
-    constexpr int K      = stride * NM;
-    __shared__ double red[K][kThreads / 64];
+    constexpr int K = stride * NM, NW = kFinThreads / 64;
+    __shared__ double red[K][NW];
     __shared__ double tot[K];
     __shared__ int    status0[NM];
     if (threadIdx.x < NM)
       status0[threadIdx.x] = int(threadIdx.x) < W.nv ? W.state[threadIdx.x].status : 1;
     __syncthreads();
-    // every sum in one pass: thread t takes partials t, t + 256, ... (a fixed order)
+    // every sum in one pass: thread t takes partials t, t + 256, ..., then the wavefront
+    // (shuffles), then the wavefronts in order, one sum per thread (a fixed order)
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
       acc[k] = 0.;
-    for (int bb = threadIdx.x; bb < nparts; bb += kThreads)
+    for (int bb = threadIdx.x; bb < nparts; bb += kFinThreads)
 #pragma unroll
       for (int v = 0; v < NM; ++v)
         if (v < W.nv && (STAGE == 1 || status0[v] == 0))
 #pragma unroll
           for (int k = 0; k < nk; ++k)
             acc[stride * v + k] += W.partial[size_t(row0 + bb) * kPart + stride * v + k];
-    block_sum<K>(acc, red);
-    if (threadIdx.x == 0)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-      for (int k = 0; k < K; ++k)
-        tot[k] = acc[k];
+    for (int k = 0; k < K; ++k)
+      {
+        double t = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+          t += __shfl_down(t, off, 64);
+        if (lane == 0)
+          red[k][wave] = t;
+      }
+    __syncthreads();
+    if (threadIdx.x < K)
+      {
+        double t = 0.;
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+          t += red[threadIdx.x][w];
+        tot[threadIdx.x] = t;
+      }
     __syncthreads();
     // the scalars of vector v in thread v: one load and one store of its state, in parallel
     // (a loop over the vectors in one thread serialises their dependent global accesses)
@@ -529,9 +558,9 @@ namespace lbfem_amd
       }
     if (W.alpha_rows > 0)
       {
-        // alpha = rho / p.q from the direction stage's partials, summed as k_cg_finalize<2>
-        // sums them (thread t takes rows t, t + 256, ..., then the block), so every block
-        // holds the same value, bit for bit, without a launch of its own
+        // alpha = rho / p.q from the direction stage's partials, summed in a fixed order
+        // (thread t takes rows t, t + 256, ..., then the block), so every block holds the
+        // same value, bit for bit, without a launch of its own
         double pq[NM];
 #pragma unroll
         for (int v = 0; v < NM; ++v)
@@ -774,7 +803,7 @@ namespace lbfem_amd
             if (e != hipSuccess)
               return e;
             if (KIND != 0 && !(KIND == 2 && W.alpha_rows > 0))
-              hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, quad_tile_count(G), it,
+              hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kFinThreads), 0, s, W, quad_tile_count(G), it,
                                  0);
             return hipGetLastError();
           }
@@ -786,25 +815,47 @@ namespace lbfem_amd
     const int     nt = tx * ty, nb8 = (nt + 7) & ~7;
     if (KIND != 0 && nt > W.max_blocks)
       return hipErrorInvalidValue;
+    // the band set: the Laplacian's, or the streaming matrix's when a vector has a K_e term
+    int op = W.laplace ? int(kOpLaplace) : int(kOpMass);
+    for (int v = 0; v < W.nv && !W.laplace; ++v)
+      if (W.vec[v].coef != 0. && (W.vec[v].e[0] != 0. || W.vec[v].e[1] != 0.))
+        op = kOpStreaming;
+    const dim3 grid(nb8, W.nv), block(kThreads);
+    const auto go = [&](auto pc, auto oc) {
+      constexpr int P_ = decltype(pc)::value, OP_ = decltype(oc)::value;
+      hipLaunchKernelGGL((k_operator<V, P_, TX, TY, KIND, OP_>), grid, block, 0, s, G, W, it, tx, ty, x, b, r, y,
+                         y2);
+    };
+    const auto by_op = [&](auto pc) {
+      if (op == kOpMass)
+        go(pc, std::integral_constant<int, kOpMass>{});
+      else if (op == kOpStreaming)
+        go(pc, std::integral_constant<int, kOpStreaming>{});
+      else if constexpr (std::is_same_v<V, double>)
+        go(pc, std::integral_constant<int, kOpLaplace>{});
+      else
+        return false; // the stream function is FP64 only
+      return true;
+    };
+    bool ok = false;
     switch (G.p)
       {
         case 1:
-          hipLaunchKernelGGL((k_operator<V, 1, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty,
-                             x, b, r, y, y2);
+          ok = by_op(std::integral_constant<int, 1>{});
           break;
         case 2:
-          hipLaunchKernelGGL((k_operator<V, 2, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty,
-                             x, b, r, y, y2);
+          ok = by_op(std::integral_constant<int, 2>{});
           break;
         case 3:
-          hipLaunchKernelGGL((k_operator<V, 3, TX, TY, KIND>), dim3(nb8, W.nv), dim3(kThreads), 0, s, G, W, it, tx, ty,
-                             x, b, r, y, y2);
+          ok = by_op(std::integral_constant<int, 3>{});
           break;
         default:
-          return hipErrorInvalidValue;
+          break;
       }
+    if (!ok)
+      return hipErrorInvalidValue;
     if (KIND != 0 && !(KIND == 2 && W.alpha_rows > 0))
-      hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kThreads), 0, s, W, nt, it, 0);
+      hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kFinThreads), 0, s, W, nt, it, 0);
     return hipGetLastError();
   }
 
@@ -824,7 +875,7 @@ namespace lbfem_amd
     int nb = nodal_blocks(G);
     nb     = std::min(nb, std::min(W.max_blocks - W.alpha_rows, 1024));
     hipLaunchKernelGGL(k_cg_update<V>, dim3(nb), dim3(kThreads), 0, s, G, W, x, r, p, q);
-    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kThreads), 0, s, W, nb, it, W.alpha_rows);
+    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kFinThreads), 0, s, W, nb, it, W.alpha_rows);
     return hipGetLastError();
   }
 
