@@ -611,8 +611,10 @@ namespace lbfem_amd
   // |b|^2, r.D^-1 r; KIND 2 (CG direction): p_new = beta p_old - D^-1 r, q = A
   // p_new and the partial p_new.q. A = M + c K_e with constrained rows the
   // identity (mass.h:105-144). One vector per block (blockIdx.y), the same
-  // partial-sum layout as the tensor-mesh operator.
-  template <int p, int KIND>
+  // partial-sum layout as the tensor-mesh operator. STIFF: some vector has a
+  // K_e term (TG3); without one the gradient arrays are not instantiated
+  // (registers set the occupancy).
+  template <int p, int KIND, bool STIFF>
   __global__ void __launch_bounds__(kQT)
   k_quad_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const double *x, const double *b,
                   const double *r_in, double *y, double *y2)
@@ -641,7 +643,7 @@ namespace lbfem_amd
     const double  beta = (KIND == 2 && it > 1) ? W.state[vsel].beta : 0.;
     const int64_t vb   = int64_t(W.block[vsel]) * G.stride;
     const double  c = W.vec[vsel].coef, ex = W.vec[vsel].e[0], ey = W.vec[vsel].e[1];
-    const bool    stiff = !(c == 0. || (ex == 0. && ey == 0.));
+    const bool    stiff = STIFF && !(c == 0. || (ex == 0. && ey == 0.));
     stage_basis(G, B);
 
     const int wi = p * bx.wcx + 1, wj = p * bx.wcy + 1;
@@ -667,15 +669,17 @@ namespace lbfem_amd
 
     if (has_cell)
       {
-        double u[n1][n1], uq[n1][n1], uxi[n1][n1], ueta[n1][n1];
+        constexpr int ng = STIFF ? n1 : 1; // the gradient arrays of the K_e term
+        double        u[n1][n1], uq[n1][n1], uxi[ng][ng], ueta[ng][ng];
 #pragma unroll
         for (int ky = 0; ky < n1; ++ky)
 #pragma unroll
           for (int kx = 0; kx < n1; ++kx)
             u[ky][kx] = X[p * lcy + ky][p * lcx + kx];
         ref_values<p>(B, u, uq);
-        if (stiff)
-          ref_gradients<p>(B, u, uxi, ueta);
+        if constexpr (STIFF)
+          if (stiff)
+            ref_gradients<p>(B, u, uxi, ueta);
         double v[n1][n1], fxi[n1][n1], feta[n1][n1];
 #pragma unroll
         for (int qy = 0; qy < n1; ++qy)
@@ -686,15 +690,15 @@ namespace lbfem_amd
               inverse_jacobian(cv, B.qx[qx], B.qx[qy], k00, k01, k10, k11, det);
               const double jxw = det * B.qw[qx] * B.qw[qy];
               v[qy][qx]        = uq[qy][qx] * jxw;
-              if (stiff)
-                {
-                  const double w0 = k00 * ex + k01 * ey, w1 = k10 * ex + k11 * ey; // K e
-                  const double t  = c * (w0 * uxi[qy][qx] + w1 * ueta[qy][qx]) * jxw;
-                  fxi[qy][qx]     = t * w0;
-                  feta[qy][qx]    = t * w1;
-                }
-              else
-                fxi[qy][qx] = feta[qy][qx] = 0.;
+              fxi[qy][qx] = feta[qy][qx] = 0.;
+              if constexpr (STIFF)
+                if (stiff)
+                  {
+                    const double w0 = k00 * ex + k01 * ey, w1 = k10 * ex + k11 * ey; // K e
+                    const double t  = c * (w0 * uxi[qy][qx] + w1 * ueta[qy][qx]) * jxw;
+                    fxi[qy][qx]     = t * w0;
+                    feta[qy][qx]    = t * w1;
+                  }
             }
         integrate<p>(B, v, fxi, feta, &R[0][tid], C::WC);
       }
@@ -835,7 +839,15 @@ namespace lbfem_amd
     if (KIND != 0 && nt > W.max_blocks)
       return hipErrorInvalidValue;
     const int nb8 = (nt + 7) & ~7;
-    hipLaunchKernelGGL((k_quad_operator<p, KIND>), dim3(nb8, W.nv), dim3(kQT), 0, s, G, W, it, tx, x, b, r, y, y2);
+    bool      stiff = false;
+    for (int v = 0; v < W.nv; ++v)
+      stiff |= W.vec[v].coef != 0. && (W.vec[v].e[0] != 0. || W.vec[v].e[1] != 0.);
+    if (stiff)
+      hipLaunchKernelGGL((k_quad_operator<p, KIND, true>), dim3(nb8, W.nv), dim3(kQT), 0, s, G, W, it, tx, x, b, r, y,
+                         y2);
+    else
+      hipLaunchKernelGGL((k_quad_operator<p, KIND, false>), dim3(nb8, W.nv), dim3(kQT), 0, s, G, W, it, tx, x, b, r,
+                         y, y2);
     return hipGetLastError();
   }
 
