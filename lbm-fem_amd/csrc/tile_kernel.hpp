@@ -77,6 +77,14 @@ namespace lbfem_amd
       static constexpr int  NF   = LL ? Q + NM : Q;      // f (or g*) and, for Lee-Lin, h of the moving populations
       static constexpr int  R    = (NH + NT - 1) / NT;   // prologue rounds
       static constexpr bool WGT  = p == 1;               // the operator as 3 x 3 node weights
+      // LDS row pitch of the staged planes. The LDS serves 128 bytes a cycle (LB lanes of T);
+      // where a row of TX threads does not end on a multiple of LB lanes, a group of LB lanes
+      // spans two rows and hits the same banks twice unless the pitch is congruent to TX
+      // modulo LB (then the two rows' words follow on in the banks). Padding is skipped where
+      // it would more than double the plane (16-wide FP32 tiles).
+      static constexpr int  LB   = 128 / int(sizeof(T));
+      static constexpr int  PXP  = HX + ((TX - HX) % LB + LB) % LB;
+      static constexpr int  PX   = (TX % LB == 0 || PXP > 2 * HX) ? HX : PXP;
       static_assert(TPP % 64 == 0 && NT <= 1024, "whole wavefronts, at most 1024 threads");
       static_assert(NM % PS == 0 && PPP % 2 == 0, "parts of whole population pairs");
     };
@@ -84,7 +92,7 @@ namespace lbfem_amd
     template <class C>
     struct Smem
     {
-      typename C::T F[C::NF][C::HY][C::HX];
+      typename C::T F[C::NF][C::HY][C::PX];
       typename C::T X[C::PS > 1 ? NM : 1][C::TY][C::TX];      // increments of the parts
       typename C::T MP[C::PS > 1 ? NM / 2 : 1][C::TY][C::TX]; // wall mass flux of population pairs
     };
@@ -136,7 +144,8 @@ namespace lbfem_amd
       // constants (its bands' row or column sums vanish), and FP32 then carries the
       // small differences instead of cancelling large values
       constexpr bool CEN        = sizeof(T) == 4;
-      T(&F)[NF][HY][HX]         = sm.F;
+      constexpr int  PX         = C::PX;
+      T(&F)[NF][HY][PX]         = sm.F;
       // the population buffers are device (global) memory: say so, so that the loads and
       // stores are global ones even where the pointers come from a batch table
       using GT                   = __attribute__((address_space(1))) T;
@@ -268,8 +277,8 @@ namespace lbfem_amd
         const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
         const bool nx = ex != T(0), ny = ey != T(0);
         res           = T(0);
-        const T(*Ff)[HX] = F[al];
-        const T(*Fh)[HX] = F[LL ? Q + al - 1 : al];
+        const T(*Ff)[PX] = F[al];
+        const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
         const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
         const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
         if constexpr (WGT)
@@ -403,8 +412,8 @@ namespace lbfem_amd
             const T ey = T(S.e[al][1]);
             if (ey == T(0))
               return;
-            const T(*Ff)[HX] = F[al];
-            const T(*Fh)[HX] = F[LL ? Q + al - 1 : al];
+            const T(*Ff)[PX] = F[al];
+            const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
             const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
             const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
             T       hy = 0, fy = 0;
@@ -438,7 +447,7 @@ namespace lbfem_amd
       auto face = [&](const int al) -> T {
         const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
         const bool nx = ex != T(0), ny = ey != T(0);
-        const T(*Ff)[HX] = F[al];
+        const T(*Ff)[PX] = F[al];
         T fc             = T(0);
         // surface term (dt^2/2) oint phi (n.e)(e.grad f) on walls (advection.h:177-199), as exact
         // 1-D integrals along the wall: Dx of the tangential derivative, Mx of the normal one.
