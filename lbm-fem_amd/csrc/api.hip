@@ -362,7 +362,7 @@ struct lbfem_scheme
   int
   variant() const
   {
-    return tile_variant >= 0 ? tile_variant : tile_default_variant(d->G);
+    return tile_variant >= 0 ? tile_variant : tile_default_variant(d->G, set.precision == LBFEM_MIXED);
   }
 
   void
@@ -1248,7 +1248,7 @@ extern "C"
         HIP_CHECK(quad_advect_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, d->G, S, f,
                                      kind == LBFEM_LEELIN ? feq : nullptr, nullptr, rhs, as_stream(stream)));
       else
-        HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, tile_default_variant(d->G), d->G,
+        HIP_CHECK(tile_launch(kind == LBFEM_LEELIN ? kAdvectLeeLin : kAdvectBardow, tile_default_variant(d->G, false), d->G,
                               S, f, kind == LBFEM_LEELIN ? feq : nullptr, (double *)nullptr, rhs,
                               as_stream(stream)));
       return LBFEM_OK;
@@ -1747,7 +1747,7 @@ extern "C"
           HIP_CHECK(hipEventCreate(&e0));
           HIP_CHECK(hipEventCreate(&e1));
           double best = 1e300;
-          int    pick = tile_default_variant(s->d->G);
+          int    pick = s->variant();
           for (int v = 0; v < kTileVariants; ++v)
             {
               // two launches to load the code, three to size the sample, then about 2 ms of launches
@@ -1838,7 +1838,7 @@ extern "C"
           int64_t nodes = 0;
           for (const int k : g.members)
             nodes += int64_t(b->members[k]->d->G.np) * b->members[k]->d->G.np;
-          g.variant = size_default_variant(nodes);
+          g.variant = size_default_variant(nodes, g.mixed);
         }
       HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       rebuild_tables(*b);

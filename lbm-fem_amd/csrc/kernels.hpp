@@ -134,16 +134,17 @@ namespace lbfem_amd
   // The variant used until a measurement picks one, by the node count of the launch: below
   // about four wide-tile wavefronts per SIMD of the device (256 CUs x 4 SIMDs) the 40 x 8 tile
   // split over two threads per node (the fastest measured shape at 257^2 nodes, Q1 and Q2,
-  // FP64 and FP32), else the 32 x 16 tile (the batch autotuner's pick for large groups).
+  // FP64 and FP32); above, the 32 x 8 tile in FP64 (20-30 % ahead of 32 x 16 at 1025^2 and
+  // 2049^2 nodes, Q1 and Q2, both schemes) and the 32 x 16 tile in FP32 (2 % ahead).
   inline int
-  size_default_variant(const long long nodes)
+  size_default_variant(const long long nodes, const bool fp32)
   {
-    return nodes < 64LL * 4 * 1024 ? kTileSplit2W40 : kTileTall;
+    return nodes < 64LL * 4 * 1024 ? kTileSplit2W40 : fp32 ? kTileTall : kTileWide;
   }
   inline int
-  tile_default_variant(const Geo &G)
+  tile_default_variant(const Geo &G, const bool fp32)
   {
-    return size_default_variant((long long)(G.P + 1) * (G.P + 1));
+    return size_default_variant((long long)(G.P + 1) * (G.P + 1), fp32);
   }
   // nodal operations over every node (collision.h:78-168)
   hipError_t collide_launch(const Geo &G, double *g, double omega, hipStream_t stream);
