@@ -689,12 +689,15 @@ namespace
       const Work   solve_work     = {per_solve.passes * solves, per_solve.flops * solves};
       const double seconds        = std::max(wall, 1e-300);
       const auto   stage          = [&](const char *name, const double sec, const Work &w) {
-        const double s_ = std::max(sec, 1e-300);
         std::cout << "  " << std::left << std::setw(12) << name << std::right << std::setw(9) << std::fixed
                   << std::setprecision(3) << sec << " s " << std::setw(5) << std::setprecision(1)
-                  << 100 * sec / seconds << " % " << std::setw(6) << std::setprecision(2) << w.passes * gb / s_
-                  << " GB/s " << std::setw(6) << std::setprecision(2) << 1e-9 * w.flops * double(n) * total_steps / s_
-                  << " GFlop/s  (" << std::setprecision(0) << w.passes << " vector passes, " << w.flops
+                  << 100 * sec / seconds << " % ";
+        if (sec > 0.) // a stage fused into another has no time of its own
+          std::cout << std::setw(6) << std::setprecision(2) << w.passes * gb / sec << " GB/s " << std::setw(6)
+                    << std::setprecision(2) << 1e-9 * w.flops * double(n) * total_steps / sec << " GFlop/s  (";
+        else
+          std::cout << "(fused; ";
+        std::cout << std::setprecision(0) << w.passes << " vector passes, " << w.flops
                   << " flops per node and step, intensity " << std::setprecision(2) << w.flops / (w.passes * 8.)
                   << " flop/byte)" << std::defaultfloat << "\n";
       };
