@@ -100,6 +100,45 @@ def test_cavity_startup_cg_iterations_match_published(tmp_path):
     assert len(its) == 1 and abs(its[0] - 10.1) <= 0.05, its
 
 
+# docs/results.md:97-160: relative L2 velocity error of the vortex (1,4), Re 100, t = 0.25 t_ref,
+# Bardow, CG 1e-8: (streaming, CFL, Ma = 0.05 or 1.6/N) -> the published value per N (3 digits);
+# and the CG iterations per solve of TG3 at CFL 1 (1 decimal)
+TG3_TABLE = {
+    24: {("tg2", 0.25, 0): 1.41e-1, ("tg3", 0.25, 0): 1.49e-1, ("tg3", 1.0, 0): 1.04e-1,
+         ("tg3-split", 1.0, 0): 3.26e-2, ("tg3-split", 1.0, 1): 5.38e-2, ("tg2", 0.25, 1): 1.08e-1},
+    32: {("tg2", 0.25, 0): 5.82e-2, ("tg3", 0.25, 0): 6.16e-2, ("tg3", 1.0, 0): 5.00e-2,
+         ("tg3-split", 1.0, 0): 2.15e-2, ("tg3-split", 1.0, 1): 2.15e-2, ("tg2", 0.25, 1): 5.82e-2},
+    48: {("tg2", 0.25, 0): 1.73e-2, ("tg3", 0.25, 0): 1.80e-2, ("tg3", 1.0, 0): 1.40e-2,
+         ("tg3-split", 1.0, 0): 5.75e-3, ("tg3-split", 1.0, 1): 5.58e-3, ("tg2", 0.25, 1): 2.51e-2},
+    64: {("tg2", 0.25, 0): 7.58e-3, ("tg3", 0.25, 0): 7.66e-3, ("tg3", 1.0, 0): 6.76e-3,
+         ("tg3-split", 1.0, 0): 3.56e-3, ("tg3-split", 1.0, 1): 3.43e-3, ("tg2", 0.25, 1): 1.39e-2},
+    96: {("tg2", 0.25, 0): 2.89e-3, ("tg3", 0.25, 0): 2.83e-3, ("tg3", 1.0, 0): 2.84e-3,
+         ("tg3-split", 1.0, 0): 2.16e-3, ("tg3-split", 1.0, 1): 1.28e-3, ("tg2", 0.25, 1): 6.15e-3},
+}
+TG3_CG_ITS = {24: 5.1, 32: 4.9, 48: 3.8, 64: 2.9, 96: 2.1}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", sorted(TG3_TABLE))
+def test_tg3_and_split_tables_published(tmp_path, N):
+    """The TG3 and split-TG3 tables of docs/results.md:97-160 (the reference's deal.II runs) through
+    the driver: every error within 0.55 units of its third published digit (the published values
+    are rounded), and the CG iterations per solve of TG3 at CFL 1 within 0.05 of the published."""
+    for (streaming, cfl, small_mach), want in TG3_TABLE[N].items():
+        mach = 1.6 / N if small_mach else 0.05
+        r = run(["--case", "tgv", "--cells", str(N), "--mach", repr(mach), "--modes", "1,4", "--reynolds", "100",
+                 "--tend", "0.25", "--streaming", streaming, "--cfl", str(cfl), "--no-output"], tmp_path)
+        assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+        rows = [x for x in (ln.split() for ln in r.stdout.splitlines()) if len(x) == 6 and x[0].isdigit()]
+        err = float(rows[-1][4])
+        unit = 10.0 ** (np.floor(np.log10(want)) - 2)
+        print(f"N {N} {streaming} CFL {cfl} Ma {mach:.4f}: {err:.6e} (published {want:.2e})", flush=True)
+        assert abs(err - want) <= 0.55 * unit, (N, streaming, cfl, mach, err, want)
+        if streaming == "tg3" and cfl == 1.0:
+            its = [float(ln.split(":")[1].split()[0]) for ln in r.stdout.splitlines() if "CG iterations" in ln]
+            assert abs(its[0] - TG3_CG_ITS[N]) <= 0.05, (N, its, TG3_CG_ITS[N])
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("args", [
     ["--case", "tgv", "--cells", "24", "--mass", "lumped"],
