@@ -212,3 +212,36 @@ def test_distorted_vortex_published_error(tmp_path):
             sec[m.group(1)] = float(m.group(2))
     assert set(sec) == {"collision", "advection", "mass solves"}, r.stdout[-1500:]
     assert sec["mass solves"] > sec["collision"] > 0 and sec["advection"] > 0, sec
+
+
+# docs/results.md:222-233: the vortex on the distorted mesh, 48^2 Q1, modes (1,4), Re 100, Ma 0.05,
+# t = 0.25 t_ref, Bardow, CG 1e-8: (streaming, nominal CFL) -> errors at eps 0, 0.02, 0.04 (None:
+# the published "unstable")
+DISTORTION_TABLE = {
+    ("tg2", 0.25): (1.73e-2, 1.81e-2, 2.07e-2),
+    ("tg2", 0.45): (2.11e-2, 2.28e-2, None),
+    ("tg3", 0.5): (3.01e-2, 3.17e-2, 3.68e-2),
+    ("tg3", 1.0): (1.40e-2, 2.01e-2, 3.38e-2),
+    ("tg3-split", 0.5): (2.90e-2, 3.04e-2, 3.49e-2),
+    ("tg3-split", 1.0): (5.75e-3, 1.23e-2, 2.47e-2),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("streaming,cfl", sorted(DISTORTION_TABLE))
+def test_distortion_table_published(tmp_path, streaming, cfl):
+    """The whole distortion table of docs/results.md:222-233 through the driver (general
+    quadrilaterals for eps > 0): every error within 0.55 units of its third published digit, and the
+    published "unstable" run blows up."""
+    for eps, want in zip((0.0, 0.02, 0.04), DISTORTION_TABLE[(streaming, cfl)]):
+        r = run(["--case", "tgv", "--cells", "48", "--mach", "0.05", "--modes", "1,4", "--reynolds", "100",
+                 "--tend", "0.25", "--distort", str(eps), "--streaming", streaming, "--cfl", str(cfl), "--no-output"],
+                tmp_path)
+        assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+        rows = [x for x in (ln.split() for ln in r.stdout.splitlines()) if len(x) == 6 and x[0].isdigit()]
+        err = float(rows[-1][4])
+        print(f"{streaming} CFL {cfl} eps {eps}: {err:.6e} (published {want})", flush=True)
+        if want is None:
+            assert not err < 1.0, (streaming, cfl, eps, err)
+        else:
+            assert abs(err - want) <= 0.55 * 10.0 ** (np.floor(np.log10(want)) - 2), (streaming, cfl, eps, err, want)
