@@ -245,3 +245,54 @@ def test_distortion_table_published(tmp_path, streaming, cfl):
             assert not err < 1.0, (streaming, cfl, eps, err)
         else:
             assert abs(err - want) <= 0.55 * 10.0 ** (np.floor(np.log10(want)) - 2), (streaming, cfl, eps, err, want)
+
+
+def _vortex_cases():
+    """The Taylor-Green tables of docs/results.md:27-95 as (name, driver arguments, published value,
+    published significant digits)."""
+    cases = []
+    # diffusive scaling (results.md:27-35): modes (1,1), Ma ~ h, Re 20, t = t_ref/2, CFL 0.25
+    for N, mach, vals in ((16, 0.2, (9.2e-3, 3.4e-2, 9.2e-3, 3.4e-2)), (32, 0.1, (3.2e-3, 9.0e-3, 3.2e-3, 9.0e-3)),
+                          (64, 0.05, (4.3e-4, 2.2e-3, 4.4e-4, 2.1e-3))):
+        for (scheme, mass), want in zip((("leelin", "cg"), ("leelin", "lumped"), ("bardow", "cg"),
+                                         ("bardow", "lumped")), vals):
+            cases.append((f"diffusive-{N}-{scheme}-{mass}",
+                          ["--cells", str(N), "--mach", str(mach), "--modes", "1,1", "--reynolds", "20", "--tend", "0.5",
+                           "--cfl", "0.25", "--scheme", scheme, "--mass", mass], want, 2))
+    # the lumped scheme's error against the time step (results.md:44-46): 32^2, modes (1,1), Re 100
+    for cfl, want in ((0.25, 1.6e-2), (0.5, 5.0e-2)):
+        cases.append((f"lumped-32-cfl{cfl}", ["--cells", "32", "--modes", "1,1", "--reynolds", "100", "--mass", "lumped",
+                                              "--cfl", str(cfl)], want, 2))
+    # modes (1,4) (results.md:55-60): Re 100, CFL 0.25, t = 0.25 t_ref, Ma 0.05; Q2 on N/2 elements
+    table = {32: (5.8e-2, 6.4e-2, 1.0e-1, 1.7e-2), 64: (7.6e-3, 7.8e-3, 8.8e-3, 2.5e-3),
+             128: (2.1e-3, 2.0e-3, 1.9e-3, 1.8e-3), 256: (1.8e-3, None, 1.9e-3, 1.8e-3)}
+    for N, vals in table.items():
+        for (scheme, mass, deg), want in zip((("bardow", "cg", 1), ("leelin", "cg", 1), ("bardow", "lumped", 1),
+                                              ("bardow", "cg", 2)), vals):
+            if want is not None:
+                cases.append((f"modes14-{N}-{scheme}-{mass}-q{deg}",
+                              ["--cells", str(N // deg), "--degree", str(deg), "--mach", "0.05", "--modes", "1,4",
+                               "--reynolds", "100", "--tend", "0.25", "--cfl", "0.25", "--scheme", scheme, "--mass", mass],
+                              want, 2))
+    # the non-doubling sequence (results.md:78-86), the meshes the TG3 table does not hold
+    for N, fixed, scaled in ((40, 2.97e-2, 3.65e-2), (80, 4.24e-3, 8.87e-3)):
+        for mach, want in ((0.05, fixed), (1.6 / N, scaled)):
+            cases.append((f"sequence-{N}-ma{mach:.4f}",
+                          ["--cells", str(N), "--mach", repr(mach), "--modes", "1,4", "--reynolds", "100", "--tend", "0.25",
+                           "--cfl", "0.25"], want, 3))
+    return cases
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,args,want,digits", [pytest.param(*c, id=c[0]) for c in _vortex_cases()])
+def test_vortex_tables_published(tmp_path, name, args, want, digits):
+    """The Taylor-Green tables of docs/results.md:27-95 (the reference's runs: diffusive scaling, the
+    lumped scheme against the time step, the modes (1,4) convergence for both schemes, lumped and
+    consistent, Q1 and Q2, and the non-doubling mesh sequence) through the driver: each relative L2
+    velocity error within half a unit of its last published digit."""
+    r = run(["--case", "tgv", *args, "--no-output"], tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    rows = [x for x in (ln.split() for ln in r.stdout.splitlines()) if len(x) == 6 and x[0].isdigit()]
+    err = float(rows[-1][4])
+    unit = 10.0 ** (np.floor(np.log10(want)) - (digits - 1))
+    assert abs(err - want) <= 0.5 * unit, (name, err, want)
