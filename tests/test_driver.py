@@ -2,8 +2,10 @@
 line on the CPU; on the GPU the reference's CI check (cavity, 16^2 Q1, Re 100
 then 400 by continuation, lumped mass, each to steady state; psi_max within 10 %
 of Ghia, Ghia & Shin 1982, .github/workflows/ci.yml:50-58), the reference's own cavity
-result files reproduced to their printed digits, and a checkpoint / restart that continues
-bitwise."""
+result files reproduced to their printed digits, the published tables of
+docs/results.md (Taylor-Green convergence, TG3 and split TG3, iterated lumping,
+distorted meshes, start-up Couette) and the published CG iteration counts reproduced
+through the driver, and a checkpoint / restart that continues bitwise."""
 import os
 import subprocess
 
