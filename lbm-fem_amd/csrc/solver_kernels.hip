@@ -447,11 +447,13 @@ namespace lbfem_amd
       }
   }
 
-  // The scalars of one CG stage from the per-block partials, in one block and a
-  // fixed summation order (a kernel boundary, not a device-wide fence, orders it
-  // after the partials). STAGE 1: start (|r|, |b|, r.D^-1 r); 2: direction
-  // (alpha = rho / p.q); 3: update (|r|, stopping test, rho, beta).
-  // One block (1024 threads measured no faster at a million nodes, and slower on small meshes).
+  // The scalars of one CG stage from the per-block partials, one block per vector and a
+  // fixed summation order (a kernel boundary, not a device-wide fence, orders it after the
+  // partials): thread t takes rows t, t + 256, ..., then the wavefront (shuffles), then the
+  // wavefronts in order. STAGE 1: start (|r|, |b|, r.D^-1 r); 2: direction (alpha = rho /
+  // p.q); 3: update (|r|, stopping test, rho, beta). The vectors' blocks read their partials
+  // side by side on separate CUs (one block for all of them spent most of its time in the
+  // partials' loads at a million nodes; 1024 threads in it measured no faster).
   constexpr int kFinThreads = kThreads;
 
   template <int STAGE>
@@ -459,29 +461,26 @@ namespace lbfem_amd
   k_cg_finalize(const CgWork W, const int nparts, const int it, const int row0)
   {
     constexpr int stride = STAGE == 3 ? 2 : 3, nk = STAGE == 1 ? 3 : STAGE == 2 ? 1 : 2;
-    constexpr int K = stride * NM, NW = kFinThreads / 64;
-    __shared__ double red[K][NW];
-    __shared__ double tot[K];
-    __shared__ int    status0[NM];
-    if (threadIdx.x < NM)
-      status0[threadIdx.x] = int(threadIdx.x) < W.nv ? W.state[threadIdx.x].status : 1;
-    __syncthreads();
-    // every sum in one pass: thread t takes partials t, t + 256, ..., then the wavefront
-    // (shuffles), then the wavefronts in order, one sum per thread (a fixed order)
-    double acc[K];
+    constexpr int NW = kFinThreads / 64;
+    __shared__ double red[nk][NW];
+    __shared__ double tot[nk];
+    const int         v = blockIdx.x; // the vector (uniform)
+    if (v >= W.nv)
+      return;
+    CgState st = W.state[v];
+    if (STAGE != 1 && st.status != 0)
+      return;
+    double acc[nk];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+    for (int k = 0; k < nk; ++k)
       acc[k] = 0.;
     for (int bb = threadIdx.x; bb < nparts; bb += kFinThreads)
 #pragma unroll
-      for (int v = 0; v < NM; ++v)
-        if (v < W.nv && (STAGE == 1 || status0[v] == 0))
-#pragma unroll
-          for (int k = 0; k < nk; ++k)
-            acc[stride * v + k] += W.partial[size_t(row0 + bb) * kPart + stride * v + k];
+      for (int k = 0; k < nk; ++k)
+        acc[k] += W.partial[size_t(row0 + bb) * kPart + stride * v + k];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+    for (int k = 0; k < nk; ++k)
       {
         double t = acc[k];
 #pragma unroll
@@ -491,7 +490,7 @@ namespace lbfem_amd
           red[k][wave] = t;
       }
     __syncthreads();
-    if (threadIdx.x < K)
+    if (threadIdx.x < nk)
       {
         double t = 0.;
 #pragma unroll
@@ -500,34 +499,30 @@ namespace lbfem_amd
         tot[threadIdx.x] = t;
       }
     __syncthreads();
-    // the scalars of vector v in thread v: one load and one store of its state, in parallel
-    // (a loop over the vectors in one thread serialises their dependent global accesses)
-    const int v = threadIdx.x;
-    if (v >= W.nv || (STAGE != 1 && status0[v] != 0))
+    if (threadIdx.x != 0)
       return;
-    CgState st = W.state[v];
     if constexpr (STAGE == 1)
       {
-        st.res      = sqrt(tot[3 * v]);
-        st.tol      = W.tol_rel * sqrt(tot[3 * v + 1]);
-        st.rho      = tot[3 * v + 2];
+        st.res      = sqrt(tot[0]);
+        st.tol      = W.tol_rel * sqrt(tot[1]);
+        st.rho      = tot[2];
         st.rho_prev = 0.;
         st.its      = 0;
         // SolverControl::check(0, res): success first, then failure at the cap
         st.status = st.res <= st.tol ? 1 : (W.max_its <= 0 || isnan(st.res)) ? 2 : 0;
       }
     else if constexpr (STAGE == 2)
-      st.alpha = st.rho / tot[3 * v];
+      st.alpha = st.rho / tot[0];
     else
       {
-        st.res = sqrt(fabs(tot[2 * v]));
+        st.res = sqrt(fabs(tot[0]));
         st.its = it;
         if (st.res <= st.tol)
           st.status = 1;
         else if (it >= W.max_its || isnan(st.res))
           st.status = 2;
         st.rho_prev = st.rho;
-        st.rho      = tot[2 * v + 1];
+        st.rho      = tot[1];
         st.beta     = st.rho / st.rho_prev;
       }
     W.state[v] = st;
@@ -803,7 +798,7 @@ namespace lbfem_amd
             if (e != hipSuccess)
               return e;
             if (KIND != 0 && !(KIND == 2 && W.alpha_rows > 0))
-              hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kFinThreads), 0, s, W, quad_tile_count(G), it,
+              hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(std::max(W.nv, 1)), dim3(kFinThreads), 0, s, W, quad_tile_count(G), it,
                                  0);
             return hipGetLastError();
           }
@@ -855,7 +850,7 @@ namespace lbfem_amd
     if (!ok)
       return hipErrorInvalidValue;
     if (KIND != 0 && !(KIND == 2 && W.alpha_rows > 0))
-      hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(1), dim3(kFinThreads), 0, s, W, nt, it, 0);
+      hipLaunchKernelGGL((k_cg_finalize<KIND>), dim3(std::max(W.nv, 1)), dim3(kFinThreads), 0, s, W, nt, it, 0);
     return hipGetLastError();
   }
 
@@ -875,7 +870,7 @@ namespace lbfem_amd
     int nb = nodal_blocks(G);
     nb     = std::min(nb, std::min(W.max_blocks - W.alpha_rows, 1024));
     hipLaunchKernelGGL(k_cg_update<V>, dim3(nb), dim3(kThreads), 0, s, G, W, x, r, p, q);
-    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(1), dim3(kFinThreads), 0, s, W, nb, it, W.alpha_rows);
+    hipLaunchKernelGGL((k_cg_finalize<3>), dim3(std::max(W.nv, 1)), dim3(kFinThreads), 0, s, W, nb, it, W.alpha_rows);
     return hipGetLastError();
   }
 
