@@ -528,84 +528,59 @@ namespace lbfem_amd
     W.state[v] = st;
   }
 
-  // x += alpha p; r += alpha q, with the partials of |r|^2 and r.D^-1 r
-  // (the stopping test and beta follow in k_cg_finalize<3>)
+  // x += alpha p; r += alpha q, with the partials of |r|^2 and r.D^-1 r (the stopping
+  // test and beta follow in k_cg_finalize<3>). blockIdx.y is the vector: each block updates
+  // one vector over the nodes its blockIdx.x strides through, so a batch of nv vectors has
+  // nv times the wavefronts in flight, and each vector's sums keep the order they have with
+  // one vector per launch.
   template <typename V>
   __global__ void __launch_bounds__(kThreads)
   k_cg_update(const Geo G, const CgWork W, V *__restrict__ x, V *__restrict__ r, const V *__restrict__ p,
               const V *__restrict__ q)
   {
-    __shared__ double red[2 * NM][kThreads / 64];
-    __shared__ double alpha_s[NM];
-    const int64_t     n = int64_t(G.np) * G.np;
-    double            part[2 * NM];
-#pragma unroll
-    for (int k = 0; k < 2 * NM; ++k)
-      part[k] = 0.;
-    V       alpha[NM];
-    int64_t off[NM];
-    bool    act[NM];
-#pragma unroll
-    for (int v = 0; v < NM; ++v)
-      {
-        act[v] = v < W.nv && W.state[v].status == 0;
-        off[v] = v < W.nv ? int64_t(W.block[v]) * G.stride : 0;
-      }
+    __shared__ double red[2][kThreads / 64];
+    __shared__ double alpha_s;
+    const int         v = blockIdx.y; // uniform over the block
+    if (v >= W.nv || W.state[v].status != 0)
+      return;
+    const int64_t n   = int64_t(G.np) * G.np;
+    const int64_t off = int64_t(W.block[v]) * G.stride;
+    double        a   = W.state[v].alpha;
     if (W.alpha_rows > 0)
       {
         // alpha = rho / p.q from the direction stage's partials, summed in a fixed order
         // (thread t takes rows t, t + 256, ..., then the block), so every block holds the
         // same value, bit for bit, without a launch of its own
-        double pq[NM];
-#pragma unroll
-        for (int v = 0; v < NM; ++v)
-          pq[v] = 0.;
+        double pq[1] = {0.};
         for (int bb = threadIdx.x; bb < W.alpha_rows; bb += kThreads)
-#pragma unroll
-          for (int v = 0; v < NM; ++v)
-            if (act[v])
-              pq[v] += W.partial[size_t(bb) * kPart + 3 * v];
-        block_sum<NM>(pq, red);
+          pq[0] += W.partial[size_t(bb) * kPart + 3 * v];
+        block_sum<1>(pq, red);
         if (threadIdx.x == 0)
-#pragma unroll
-          for (int v = 0; v < NM; ++v)
-            alpha_s[v] = act[v] ? W.state[v].rho / pq[v] : 0.;
+          alpha_s = W.state[v].rho / pq[0];
         __syncthreads();
+        a = alpha_s;
       }
-#pragma unroll
-    for (int v = 0; v < NM; ++v)
-      alpha[v] = act[v] ? V(W.alpha_rows > 0 ? alpha_s[v] : W.state[v].alpha) : V(0);
+    const V alpha   = V(a);
+    double  part[2] = {0., 0.};
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
       {
         const int    i = int(k % G.np), j = int(k / G.np);
-        const double d = mass_diag_inv(G, i, j);
-        V            xv[NM], pv[NM], qv[NM], rv[NM];
-#pragma unroll
-        for (int v = 0; v < NM; ++v)
-          if (act[v])
-            {
-              xv[v] = x[off[v] + k];
-              pv[v] = p[off[v] + k];
-              qv[v] = q[off[v] + k];
-              rv[v] = r[off[v] + k];
-            }
-#pragma unroll
-        for (int v = 0; v < NM; ++v)
-          if (act[v])
-            {
-              x[off[v] + k]   = xv[v] + alpha[v] * pv[v];
-              const V rr    = rv[v] + alpha[v] * qv[v];
-              r[off[v] + k] = rr;
-              const double r8 = double(rr);
-              part[2 * v] += r8 * r8;
-              part[2 * v + 1] += r8 * (d * r8);
-            }
+        const double d  = mass_diag_inv(G, i, j);
+        const V      xv = x[off + k], pv = p[off + k], qv = q[off + k], rv = r[off + k];
+        x[off + k]      = xv + alpha * pv;
+        const V rr      = rv + alpha * qv;
+        r[off + k]      = rr;
+        const double r8 = double(rr);
+        part[0] += r8 * r8;
+        part[1] += r8 * (d * r8);
       }
-    block_sum<2 * NM>(part, red);
+    block_sum<2>(part, red);
     // after the direction stage's rows (other blocks may still be reading them for alpha)
     if (threadIdx.x == 0)
-      for (int k = 0; k < 2 * NM; ++k)
-        W.partial[size_t(W.alpha_rows + blockIdx.x) * kPart + k] = part[k];
+      {
+        W.partial[size_t(W.alpha_rows + blockIdx.x) * kPart + 2 * v]     = part[0];
+        W.partial[size_t(W.alpha_rows + blockIdx.x) * kPart + 2 * v + 1] = part[1];
+      }
   }
 
   // ------------------------------------------------------------------ stream function
@@ -869,7 +844,7 @@ namespace lbfem_amd
   {
     int nb = nodal_blocks(G);
     nb     = std::min(nb, std::min(W.max_blocks - W.alpha_rows, 1024));
-    hipLaunchKernelGGL(k_cg_update<V>, dim3(nb), dim3(kThreads), 0, s, G, W, x, r, p, q);
+    hipLaunchKernelGGL(k_cg_update<V>, dim3(nb, std::max(W.nv, 1)), dim3(kThreads), 0, s, G, W, x, r, p, q);
     hipLaunchKernelGGL((k_cg_finalize<3>), dim3(std::max(W.nv, 1)), dim3(kFinThreads), 0, s, W, nb, it, W.alpha_rows);
     return hipGetLastError();
   }
