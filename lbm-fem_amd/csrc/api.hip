@@ -341,6 +341,7 @@ struct lbfem_scheme
   double                tg3         = 0.;
   bool                  extrapolate = false;
   int                   last_its    = 1; // the largest CG count of the last solve
+  int                   last_inner[4] = {1, 1, 1, 1}; // the same for the mixed form's first inner solves
   int                   tile_variant = -1; // TileVariant of the tile kernel; -1: tile_default_variant
   // stage timers (include/lbfem/timers.h): collision, advection, mass, total
   bool                                   timing = false;
@@ -587,7 +588,9 @@ namespace
   // most 30 refinements, else NoConvergence. An inner solve stops at half the
   // gap to the outer tolerance, or at 1e-5 |r| (what FP32 reaches), whichever
   // is looser: from the extrapolated start a few inner iterations then finish
-  // the solve. The iterations counted are the inner ones.
+  // the solve. The iterations counted are the inner ones. As for the FP64 solve,
+  // the host first reads an inner solve's stopping state at the largest count of
+  // the previous sweep's inner solve of the same refinement.
   int
   run_mixed_cg(lbfem_scheme &s, const CgWork &W64, double *x64, const double *b64, int64_t *its)
   {
@@ -623,11 +626,15 @@ namespace
                                        " refinements (SolverControl::NoConvergence)");
         HIP_CHECK(negate_to_float_launch(G, s.r.p, s.g32r.p, s.stream));
         int64_t   inner = 0;
+        const int slot  = std::min(outer, 3);
         const int st    = run_cg<float>(s.d->G, W32, s.d32.p, s.g32r.p, s.r32.p, s.p32[0].p, s.p32[1].p, s.q32.p,
-                                     s.host_state32, s.stream, &inner, 1, false);
+                                     s.host_state32, s.stream, &inner, std::max(1, s.last_inner[slot]), false);
         if (st != LBFEM_OK)
           return st;
         *its += inner;
+        s.last_inner[slot] = 0;
+        for (int v = 0; v < W32.nv; ++v)
+          s.last_inner[slot] = std::max(s.last_inner[slot], s.host_state32[v].its);
         HIP_CHECK(refine_add_launch(G, x64, s.d32.p, mask, s.stream));
       }
   }
