@@ -122,10 +122,21 @@ struct lbfem_disc
   bool                     general = false;
   std::vector<double>      gen_xy, gen_ml, gen_diag;
   DevBuf<double>           gen_buf;
-  // scratch of the operator-level entry points
+  // scratch of the operator-level entry points (the mass solve and the stream function),
+  // allocated on first use and kept: one call at a time per discretization
   DevBuf<CgState>  cg_state;
-  DevBuf<double>   partial, s_r, s_p0, s_p1, s_q, s_ax;
+  DevBuf<double>   partial, s_r, s_p0, s_p1, s_q, s_ax, s_b, s_dinv;
   DevBuf<unsigned> counter;
+  CgState         *host_cg = nullptr; // pinned: the CG stopping states read by the host
+
+  lbfem_disc() = default;
+  lbfem_disc(const lbfem_disc &) = delete;
+  lbfem_disc &operator=(const lbfem_disc &) = delete;
+  ~lbfem_disc()
+  {
+    if (host_cg)
+      (void)hipHostFree(host_cg);
+  }
 
   void
   upload_axis(const Axis1D &a, DevBuf<double> &buf, AxisDev &dev)
@@ -314,6 +325,18 @@ struct lbfem_disc
     partial.alloc(size_t(3 * NM) * kMaxCgBlocks);
     counter.alloc(1);
     HIP_CHECK(hipMemset(counter.p, 0, sizeof(unsigned)));
+    HIP_CHECK(hipHostMalloc(&host_cg, sizeof(CgState) * NM));
+  }
+
+  // the node vectors of the operator-level CG (r, two directions, A p, A x) and of the
+  // stream function (its right-hand side and inverse diagonal)
+  void
+  ensure_vectors()
+  {
+    if (s_r.n >= size_t(stride))
+      return;
+    for (DevBuf<double> *bb : {&s_r, &s_p0, &s_p1, &s_q, &s_ax, &s_b, &s_dinv})
+      bb->alloc(stride);
   }
 };
 
@@ -1333,14 +1356,7 @@ extern "C"
       if (m->type != LBFEM_MASS_CG && m->type != LBFEM_MASS_RICHARDSON)
         return fail(LBFEM_EINVAL, "unknown mass type");
       d->ensure_scratch();
-      if (d->s_r.n < size_t(d->stride))
-        {
-          d->s_r.alloc(d->stride);
-          d->s_p0.alloc(d->stride);
-          d->s_p1.alloc(d->stride);
-          d->s_q.alloc(d->stride);
-          d->s_ax.alloc(d->stride);
-        }
+      d->ensure_vectors();
       CgWork W      = cg_work(*d, d->cg_state.p, d->partial.p, d->counter.p, *m);
       W.nv          = 1;
       W.block[0]    = 0;
@@ -1357,20 +1373,8 @@ extern "C"
             }
           return LBFEM_OK;
         }
-      CgState *hs = nullptr;
-      HIP_CHECK(hipHostMalloc(&hs, sizeof(CgState) * NM));
       int64_t   its = 0;
-      int       rc  = LBFEM_OK;
-      try
-        {
-          rc = run_cg(d->G, W, x, r, d->s_r.p, d->s_p0.p, d->s_p1.p, d->s_q.p, hs, st, &its, 1);
-        }
-      catch (...)
-        {
-          (void)hipHostFree(hs);
-          throw;
-        }
-      (void)hipHostFree(hs);
+      const int rc  = run_cg(d->G, W, x, r, d->s_r.p, d->s_p0.p, d->s_p1.p, d->s_q.p, d->host_cg, st, &its, 1);
       if (iterations)
         *iterations = int(its);
       return rc;
@@ -2057,12 +2061,10 @@ extern "C"
       if (iterations)
         *iterations = 0;
       d->ensure_scratch();
-      DevBuf<double> b, dinv, r, p0, p1, q;
-      for (DevBuf<double> *bb : {&b, &dinv, &r, &p0, &p1, &q})
-        bb->alloc(d->stride);
+      d->ensure_vectors();
       Geo G          = d->G;
-      G.gen.diag_inv = dinv.p; // the Jacobi preconditioner of the Laplacian
-      HIP_CHECK(psi_setup_launch(G, u, v, b.p, dinv.p, st));
+      G.gen.diag_inv = d->s_dinv.p; // the Jacobi preconditioner of the Laplacian
+      HIP_CHECK(psi_setup_launch(G, u, v, d->s_b.p, d->s_dinv.p, st));
       HIP_CHECK(hipMemsetAsync(psi, 0, sizeof(double) * size_t(d->n), st));
       CgWork W{};
       W.nv         = 1;
@@ -2074,20 +2076,9 @@ extern "C"
       W.counter    = d->counter.p;
       W.max_blocks = kMaxCgBlocks;
       W.laplace    = 1;
-      CgState *hs  = nullptr;
-      HIP_CHECK(hipHostMalloc(&hs, sizeof(CgState) * NM));
-      int64_t its = 0;
-      int     rc  = LBFEM_OK;
-      try
-        {
-          rc = run_cg(G, W, psi, b.p, r.p, p0.p, p1.p, q.p, hs, st, &its, 16, true, 16);
-        }
-      catch (...)
-        {
-          (void)hipHostFree(hs);
-          throw;
-        }
-      (void)hipHostFree(hs);
+      int64_t   its = 0;
+      const int rc  = run_cg(G, W, psi, d->s_b.p, d->s_r.p, d->s_p0.p, d->s_p1.p, d->s_q.p, d->host_cg, st, &its, 16,
+                             true, 16);
       if (iterations)
         *iterations = int(its);
       return rc;
