@@ -85,9 +85,10 @@ namespace
       if (p)
         (void)hipFree(p);
       p = nullptr;
-      n = count;
+      n = 0;
       if (count)
         HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+      n = count; // only once the memory is there
     }
     void
     upload(const std::vector<T> &v)
@@ -125,7 +126,7 @@ struct lbfem_disc
   // scratch of the operator-level entry points (the mass solve and the stream function),
   // allocated on first use and kept: one call at a time per discretization
   DevBuf<CgState>  cg_state;
-  DevBuf<double>   partial, s_r, s_p0, s_p1, s_q, s_ax, s_b, s_dinv;
+  DevBuf<double>   partial, s_r, s_p0, s_p1, s_q, s_ax, s_b, s_dinv, s_sums;
   DevBuf<unsigned> counter;
   CgState         *host_cg = nullptr; // pinned: the CG stopping states read by the host
 
@@ -319,13 +320,15 @@ struct lbfem_disc
   void
   ensure_scratch()
   {
-    if (cg_state.p)
+    if (!host_cg)
+      HIP_CHECK(hipHostMalloc(&host_cg, sizeof(CgState) * NM));
+    if (counter.p) // allocated last: the rest is there
       return;
     cg_state.alloc(NM);
     partial.alloc(size_t(3 * NM) * kMaxCgBlocks);
+    s_sums.alloc(6); // the diagnostics' sums
     counter.alloc(1);
     HIP_CHECK(hipMemset(counter.p, 0, sizeof(unsigned)));
-    HIP_CHECK(hipHostMalloc(&host_cg, sizeof(CgState) * NM));
   }
 
   // the node vectors of the operator-level CG (r, two directions, A p, A x) and of the
@@ -333,7 +336,7 @@ struct lbfem_disc
   void
   ensure_vectors()
   {
-    if (s_r.n >= size_t(stride))
+    if (s_dinv.n >= size_t(stride)) // allocated last: the rest is there
       return;
     for (DevBuf<double> *bb : {&s_r, &s_p0, &s_p1, &s_q, &s_ax, &s_b, &s_dinv})
       bb->alloc(stride);
@@ -1683,11 +1686,10 @@ extern "C"
         return fail(LBFEM_EINVAL, "null argument");
       auto *d = const_cast<lbfem_disc *>(dc);
       d->ensure_scratch();
-      DevBuf<double> dsum;
-      dsum.alloc(6);
       hipStream_t st = as_stream(stream);
-      HIP_CHECK(diagnostics_launch(d->G, f, u_exact, u_prev, dsum.p, d->partial.p, d->counter.p, kMaxCgBlocks, st));
-      HIP_CHECK(hipMemcpyAsync(sums, dsum.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(
+        diagnostics_launch(d->G, f, u_exact, u_prev, d->s_sums.p, d->partial.p, d->counter.p, kMaxCgBlocks, st));
+      HIP_CHECK(hipMemcpyAsync(sums, d->s_sums.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipStreamSynchronize(st));
       return LBFEM_OK;
     });
