@@ -251,3 +251,12 @@ def test_diagnostics_sums():
     s = A.diagnostics(gd, gs.device_populations(), ue, up)
     so = od.diagnostics(gs.f, 10 * dt)
     assert np.allclose(s[:5], so[:5], rtol=1e-12)
+    # a second call on the same discretization (its sums buffer is reused) sees the new state
+    gs.step(10)
+    gs.synchronize()
+    _, ux, uy = tc.exact(xy[:, 0], xy[:, 1], 20 * dt)
+    ue = torch.from_numpy(np.stack([ux, uy], 1).reshape(-1).copy()).cuda()
+    s2 = A.diagnostics(gd, gs.device_populations(), ue, up)
+    so2 = od.diagnostics(gs.f, 20 * dt)
+    assert np.allclose(s2[:5], so2[:5], rtol=1e-12)
+    assert not np.allclose(s2[:5], s[:5], rtol=1e-12)
