@@ -2,12 +2,16 @@
 
 Workload (the plain-method GPU configuration): lid-driven cavity on the unit
 square, U0 = 0.1 (Ma = 0.1732), tanh clustering 1.2, Q1 on 256 x 256 cells,
-Lee-Lin predictor-corrector, Re 400, CFL 0.5, FP64. The headline value is the
-lumped-mass step (one fused kernel per step); the consistent-mass (CG, tol
-1e-10) step, the Q2 128 x 128 mesh, the mixed-precision form and, with
---sweep, an HBM-resident Q1 sweep (2048^2, 4096^2 cells, beyond the 256 MB
-Infinity Cache) are reported beside it. A STREAM-like copy measures the
-achievable HBM bandwidth of the device.
+Lee-Lin predictor-corrector, Re 1000, CFL 0.5 (dt / lambda = 2.88), FP64. Of the
+configuration's two Reynolds numbers, Lee-Lin holds on this mesh at Re 1000 only
+(at Re 400, dt / lambda = 1.15, it diverges within a few hundred steps with either
+mass, in the oracle as on the GPU). The headline value is the lumped-mass step
+(one fused kernel per step, 66 049 nodes: a cache-resident, latency-bound launch);
+the consistent-mass (CG, tol 1e-10) step, Bardow on Q1 256^2 and Q2 128^2 at
+Re 400, the mixed-precision form and the HBM-resident Q1 2048^2 step (0.6 GB of
+populations per step, beyond the 256 MB Infinity Cache; Re 8000 keeps
+dt / lambda = 2.88) with its own roofline are reported beside it; --sweep adds
+4096^2. A STREAM-like copy measures the achievable HBM bandwidth of the device.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--sweep]
        python bench.py --profile-headline   # the headline steps and the copy only (profiler passes)
@@ -19,24 +23,33 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lbm-fem_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 Q = 9
-# kernel names of the tile variants (Cfg<T, p, TX, TY, RO, PS, mode>), fused Lee-Lin step, FP64, Q1
-TILE_KERNEL = {v: f"k_tile<Cfg<double, 1, {tx}, {ty}, {ps}, 1> >"
-               for v, (tx, ty, ps) in enumerate(((32, 8, 1), (32, 16, 1), (32, 8, 2), (32, 8, 4), (16, 8, 4),
-                                                 (40, 8, 1), (48, 8, 1), (40, 8, 2)))}
+# kernel names of the tile variants (Cfg<T, p, TX, TY, PS, mode>), fused Lee-Lin step, Q1
+TILE_SHAPES = ((32, 8, 1), (32, 16, 1), (32, 8, 2), (32, 8, 4), (16, 8, 4), (40, 8, 1), (48, 8, 1), (40, 8, 2))
+
+
+def tile_kernel(variant, t="double"):
+    tx, ty, ps = TILE_SHAPES[variant]
+    return f"k_tile<Cfg<{t}, 1, {tx}, {ty}, {ps}, 1> >"
+
+
+TILE_KERNEL = {v: tile_kernel(v) for v in range(len(TILE_SHAPES))}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_tile_cavity_q1_256.txt")
+PMC_SUMMARY_HBM = os.path.join(ROOT, "profiles", "pmc_k_tile_cavity_q1_2048.txt")
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=400.0, cfl=0.5, mixed=False, tune=True):
+def make_scheme(A, N, p, leelin=True, mass=None, cg_tol=1e-10, Re=1000.0, cfl=0.5, mixed=False, tune=True):
     tc = A.test_cases.make("cavity", mach=0.1 / (1 / 3) ** 0.5, stretch=1.2)  # U0 = Ma c_s = 0.1
     tc.set_reynolds(Re)
     disc = A.Discretization(tc, N, p)
@@ -94,12 +107,13 @@ def hbm_probe(torch, A, nbytes=1 << 30, reps=10):
     return gbs, ok
 
 
-def pmc_traffic(kernel):
-    """HBM bytes per launch of the headline kernel (variant) from the committed profiler summary
-    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy kernel's reading of its known size), or None."""
+def pmc_traffic(kernel, path=PMC_SUMMARY):
+    """HBM bytes per launch of a tile kernel (variant) read from a committed profiler summary (FETCH_SIZE +
+    WRITE_SIZE per dispatch, scaled by the copy kernel's reading of its known size), or None. A lookup of an
+    earlier profiler run of the same command, not a measurement of this run."""
     key = f"traffic_bytes_per_launch[{kernel}]:"
     try:
-        with open(PMC_SUMMARY) as fh:
+        with open(path) as fh:
             for line in fh:
                 if line.startswith(key):
                     return int(float(line.split(":", 1)[1]))
@@ -113,7 +127,7 @@ import json, os, sys, time
 sys.path.insert(0, os.path.join(sys.argv[1], "oracle"))
 import oracle as O
 steps = int(sys.argv[2])
-od = O.Disc(256, 1, O.CAVITY, mach=0.1 / (1 / 3) ** 0.5, stretch=1.2, reynolds=400.0)
+od = O.Disc(256, 1, O.CAVITY, mach=0.1 / (1 / 3) ** 0.5, stretch=1.2, reynolds=1000.0)
 s = O.Scheme(od, O.LEELIN, O.TG2, O.MASS_LUMPED)
 s.set_time_step(od.time_step(0.5), od.relaxation_time)
 s.set_initial_populations()
@@ -154,6 +168,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true")
+    ap.add_argument("--no-hbm", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -177,7 +192,7 @@ def main():
     if args.profile_headline:
         # profiler passes: the headline steps, then the copy probe over 256 MiB (known size)
         N = args.profile_cells
-        disc, s = make_scheme(A, N, 1)
+        disc, s = make_scheme(A, N, 1, Re=1000.0 * N / 256)  # dt / lambda 2.88 on every mesh size
         log(f"profile-headline: tile variant {s.tile_variant} ({TILE_KERNEL[s.tile_variant]})")
         timed(torch, s, K, W)
         gbs, ok = hbm_probe(torch, A, nbytes=256 << 20, reps=3)
@@ -212,8 +227,9 @@ def main():
     t_kernel = kernel_time(torch, s, K)
     alg_bytes = 18 * 8 * n
     achieved = alg_bytes / t_kernel / 1e9
-    log(f"cavity Q1 256^2 leelin lumped: {1e3 * sec / K:.4f} ms/step, {value:.1f} MLUPS, kernel {1e6 * t_kernel:.2f} us, "
-        f"{achieved:.0f} GB/s algorithmic")
+    head_finite = bool(np.isfinite(s.f).all())
+    log(f"cavity Q1 256^2 leelin lumped Re 1000: {1e3 * sec / K:.4f} ms/step, {value:.1f} MLUPS, kernel "
+        f"{1e6 * t_kernel:.2f} us, {achieved:.0f} GB/s algorithmic, finite={head_finite}")
 
     # ---- the stream function of the headline flow's velocity (include/lbfem/io.h:31-93): Laplace CG on the device
     _, u_h, v_h = A.moments(s.f)
@@ -231,19 +247,26 @@ def main():
     del u_d, v_d, psi_d
 
     variants = {}
-    # ---- consistent mass (CG tol 1e-10), and Q2 128^2
-    for name, N, p, mass in (("q1_256_leelin_cg", 256, 1, A.MASS_CG), ("q2_128_leelin_lumped", 128, 2, A.MASS_LUMPED),
-                             ("q2_128_leelin_cg", 128, 2, A.MASS_CG), ("q1_256_bardow_lumped", 256, 1, None),
-                             ("q1_256_leelin_lumped_mixed", 256, 1, None), ("q2_128_leelin_lumped_mixed", 128, 2, None),
-                             ("q1_256_leelin_cg_mixed", 256, 1, A.MASS_CG)):
-        leelin = "bardow" not in name
-        d2, s2 = make_scheme(A, N, p, leelin=leelin, mass=mass, mixed=name.endswith("mixed"))
+    # ---- the consistent mass (CG tol 1e-10), and the points where Lee-Lin diverges (Re 400, and Q2 128^2 at
+    # either Reynolds number) with Bardow, the reference's default scheme, which holds there
+    for name, N, p, mass, Re in (("q1_256_leelin_cg_re1000", 256, 1, A.MASS_CG, 1000.0),
+                                 ("q1_256_bardow_lumped_re400", 256, 1, A.MASS_LUMPED, 400.0),
+                                 ("q1_256_bardow_cg_re400", 256, 1, A.MASS_CG, 400.0),
+                                 ("q2_128_bardow_lumped_re400", 128, 2, A.MASS_LUMPED, 400.0),
+                                 ("q2_128_bardow_cg_re400", 128, 2, A.MASS_CG, 400.0),
+                                 ("q1_256_leelin_lumped_re1000_mixed", 256, 1, A.MASS_LUMPED, 1000.0),
+                                 ("q2_128_bardow_lumped_re400_mixed", 128, 2, A.MASS_LUMPED, 400.0),
+                                 ("q1_256_leelin_cg_re1000_mixed", 256, 1, A.MASS_CG, 1000.0)):
+        leelin = "leelin" in name
+        d2, s2 = make_scheme(A, N, p, leelin=leelin, mass=mass, mixed=name.endswith("mixed"), Re=Re)
         steps = max(10, K // 4) if mass == A.MASS_CG else K
         its0 = s2.cg_iterations
         sec2 = timed(torch, s2, steps, W)
         its = (s2.cg_iterations - its0) / (8 * (steps + W)) if mass == A.MASS_CG else 0.0
-        variants[name] = {"ms_per_step": 1e3 * sec2 / steps, "MLUPS": 9 * d2.n_nodes * steps / sec2 / 1e6,
-                          "cg_its_per_solve": round(its, 2), "tile_variant": s2.tile_variant}
+        variants[name] = {"ms_per_step": round(1e3 * sec2 / steps, 5),
+                          "MLUPS": round(9 * d2.n_nodes * steps / sec2 / 1e6, 1),
+                          "cg_its_per_solve": round(its, 2), "tile_variant": s2.tile_variant,
+                          "finite": bool(np.isfinite(s2.f).all())}
         log(name, variants[name])
         del s2, d2
 
@@ -278,26 +301,41 @@ def main():
             log("general", name, row)
             del sg, dg
 
-    # ---- HBM-resident sweep (working set > 256 MB Infinity Cache)
-    sweep = []
+    # ---- HBM-resident step (working set > 256 MB Infinity Cache): Q1 2048^2 (0.6 GB of FP64 populations read
+    # and written per step), FP64 and mixed; with --sweep also 4096^2. Re scaled with N (Re 1000 N / 256) keeps
+    # dt / lambda = 2.88, the headline's stable Lee-Lin point.
+    sweep, roofline_hbm = [], None
+    legs = [(2048, 40, False), (2048, 40, True)]
     if args.sweep:
-        for N, steps, mixed in ((2048, 40, False), (4096, 12, False), (2048, 40, True), (4096, 12, True)):
-            d3, s3 = make_scheme(A, N, 1, mixed=mixed)
+        legs += [(4096, 12, False), (4096, 12, True)]
+    if not args.no_hbm:
+        for N, steps, mixed in legs:
+            d3, s3 = make_scheme(A, N, 1, mixed=mixed, Re=1000.0 * N / 256)
             sec3 = timed(torch, s3, steps, 3)
             tk = kernel_time(torch, s3, steps)
-            gbs = 18 * (4 if mixed else 8) * d3.n_nodes / tk / 1e9
+            alg3 = 18 * (4 if mixed else 8) * d3.n_nodes
+            gbs = alg3 / tk / 1e9
+            kern = tile_kernel(s3.tile_variant, "float" if mixed else "double")
             sweep.append({"cells": f"{N}x{N}", "degree": 1, "form": "mixed" if mixed else "fp64",
-                          "n_nodes": d3.n_nodes, "ms_per_step": 1e3 * sec3 / steps,
-                          "MLUPS": 9 * d3.n_nodes * steps / sec3 / 1e6, "kernel_us": 1e6 * tk,
-                          "achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)})
-            log("sweep", sweep[-1])
+                          "Re": 1000.0 * N / 256, "n_nodes": d3.n_nodes, "ms_per_step": round(1e3 * sec3 / steps, 4),
+                          "MLUPS": round(9 * d3.n_nodes * steps / sec3 / 1e6, 1), "kernel": kern,
+                          "kernel_us": round(1e6 * tk, 2), "achieved_GBs": round(gbs, 1),
+                          "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "finite": bool(np.isfinite(s3.f).all())})
+            log("hbm", sweep[-1])
+            if N == 2048 and not mixed:
+                roofline_hbm = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                "traffic": pmc_traffic(kern, PMC_SUMMARY_HBM),
+                                "traffic_source": "lookup: profiles/pmc_k_tile_cavity_q1_2048.txt (an earlier "
+                                                  "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE run of this leg)",
+                                "kernel": kern, "kernel_us": round(1e6 * tk, 2), "algorithmic_bytes_per_launch": alg3,
+                                "workload": "cavity Q1 2048^2 cells (4 198 401 nodes), Lee-Lin, lumped, Re 8000, FP64"}
             del s3, d3
 
     # ---- precision: TGV (1,1) on Q1 1024^2, Bardow, consistent mass (CG 1e-10), FP64 and mixed (FP32 inner CG
     # refined with FP64 residuals); L2 velocity error of the mixed run vs FP64 and both vs the analytic decay
     precision = None
     if not args.no_precision:
-        import numpy as np
         steps_p = args.precision_steps
         precision = {"workload": "Taylor-Green (1,1), Bardow, consistent mass CG 1e-10, CFL 0.25, Re 100, "
                                  f"{steps_p} steps after 5; Q1 1024^2 and Q2 512^2 cells"}
@@ -374,9 +412,14 @@ def main():
             for x in ss:
                 x.synchronize()
             t1 = time.perf_counter() - t0
-            batched[name] = {"ms_per_step": round(1e3 * tb / kb, 4), "MLUPS": round(9 * tot * kb / tb / 1e6, 1),
+            nonfinite = sum(0 if np.isfinite(x.f).all() else 1 for x in ss)
+            mlups_b = 9 * tot * kb / tb / 1e6
+            batched[name] = {"ms_per_step": round(1e3 * tb / kb, 4), "MLUPS": round(mlups_b, 1),
                              "one_by_one_MLUPS": round(9 * tot * k1 / t1 / 1e6, 1),
-                             "vs_plain_headline": round(9 * tot * kb / tb / 1e6 / value, 2)}
+                             "vs_plain_headline": round(mlups_b / value, 2),
+                             # against the plain FP64 step where it too is throughput-bound (Q1 2048^2, from HBM)
+                             "vs_plain_hbm_fp64": round(mlups_b / sweep[0]["MLUPS"], 2) if sweep else None,
+                             "nonfinite_members": nonfinite}
             log("batched", name, batched[name])
             del ss, b
 
@@ -406,7 +449,7 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
         many = cpu_oracle_rate(threads=threads)
         cpu = {"value": many["MLUPS"], "unit": "MLUPS", "cores": threads, "kind": "port",
-               "sample": f"{many['steps']} steps of the same workload (cavity Q1 256^2, Lee-Lin, lumped) on "
+               "sample": f"{many['steps']} steps of the same workload (cavity Q1 256^2, Lee-Lin, lumped, Re 1000) on "
                          f"{threads} host threads (OpenMP build of the oracle), {many['seconds']:.1f} s; one core: "
                          f"{one['MLUPS']} MLUPS ({one['steps']} steps, {one['seconds']:.1f} s)",
                "single_core_MLUPS": one["MLUPS"], "host_cpu": host_cpu(), "host_nproc": os.cpu_count()}
@@ -426,12 +469,17 @@ def main():
         "dtype": "f64",
         "data": "synthetic (lid-driven cavity from rest, seeded by the flow definition)",
         "config": {"workload": "lid-driven cavity, Q1 256x256 cells (66049 nodes), Lee-Lin, lumped mass, "
-                               "Re 400, CFL 0.5, U0 0.1, stretch 1.2",
-                   "n_nodes": n, "mnups": round(value / 9, 3)},
+                               "Re 1000, CFL 0.5 (dt/lambda 2.88), U0 0.1, stretch 1.2",
+                   "n_nodes": n, "mnups": round(value / 9, 3), "finite": head_finite},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(TILE_KERNEL[variant]),
+                     "traffic_source": "lookup: profiles/pmc_k_tile_cavity_q1_256.txt (an earlier rocprofv3 --pmc "
+                                       "FETCH_SIZE / WRITE_SIZE run of this workload)",
+                     "regime": "latency-bound: 9.5 MB per step is cache-resident and 66 049 nodes give about one "
+                               "wavefront per SIMD; roofline_hbm is the same kernel from HBM",
                      "kernel": TILE_KERNEL[variant], "kernel_us": round(1e6 * t_kernel, 3),
                      "algorithmic_bytes_per_launch": alg_bytes, "measured_copy_GBs": round(hbm_gbs, 1)},
+        "roofline_hbm": roofline_hbm,
         "cpu_baseline": cpu,
         "tile_variant": variant, "autotune_us": [round(1e6 * t, 3) for t in tune_s],
         "variants": variants,

@@ -93,7 +93,11 @@ int lbfem_disc_set_walls(lbfem_disc *disc, int32_t n, const int32_t *nodes, cons
 int lbfem_disc_time_step(const lbfem_disc *disc, double cfl, double *dt);
 
 /* ---------------------------------------------------------- operators
- * Device pointers, stream as void* (hipStream_t). */
+ * Device pointers, stream as void* (hipStream_t). lbfem_mass_solve (CG and
+ * Richardson), lbfem_diagnostics and lbfem_disc_stream_function keep their
+ * work vectors in the discretization (allocated on first use) and return
+ * after their work on it is done; calls of these on one discretization
+ * from several host threads must be serialized by the caller. */
 
 /* lbfem::AdvectionOperator::apply (include/lbfem/advection.h:49-58) with the
  * two implementations of the reference: kind LBFEM_BARDOW is
@@ -130,8 +134,9 @@ typedef struct
 
 /* lbfem::MassSolver::solve (include/lbfem/mass.h:184-213): x <- A^{-1} r for
  * one population, A = M + tg3_coefficient K_e(e); CG starts from x.
- * *iterations receives the CG steps (0 for the lumped variants). Synchronizes
- * the stream (the stopping test reads the residual). */
+ * *iterations receives the CG steps (0 for the lumped variants). CG and
+ * Richardson synchronize the stream (the stopping test reads the residual;
+ * the Richardson passes use the discretization's scratch). */
 int lbfem_mass_solve(const lbfem_disc *disc, const lbfem_mass_settings *settings, double *x, const double *r,
                      const double *e, double tg3_coefficient, int *iterations, void *stream);
 
@@ -173,6 +178,12 @@ int lbfem_scheme_synchronize(lbfem_scheme *scheme);
 int lbfem_scheme_get_stream(const lbfem_scheme *scheme, void **stream);
 /* MassSolver::cg_iterations (mass.h:215), summed over all solves. */
 int lbfem_scheme_cg_iterations(const lbfem_scheme *scheme, int64_t *iterations);
+/* The CG steps of each solve of the last step (the SolverControl::last_step()
+ * of the reference's per-population solves, mass.h:204-211): its[sw * 8 + a - 1]
+ * for sweep sw (0, and 1 for split TG3) and population a, 0 where no CG ran
+ * (lumped, Richardson, a population at rest in the sweep). The mixed form
+ * reports the inner FP32 iterations summed over the refinements. */
+int lbfem_scheme_last_cg_iterations(const lbfem_scheme *scheme, int32_t *its);
 /* StageTimers (include/lbfem/timers.h:17-23): seconds in collision,
  * advection, mass, and the total of the timed steps; timing adds events. */
 int lbfem_scheme_enable_stage_timers(lbfem_scheme *scheme, int enable);

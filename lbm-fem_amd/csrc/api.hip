@@ -363,6 +363,7 @@ struct lbfem_scheme
   DevBuf<unsigned>      counter;
   CgState              *host_state = nullptr; // pinned
   int64_t               cg_iterations = 0;
+  int                   last_step_its[2][NM] = {}; // the last step's CG steps per sweep and moving population
   std::vector<std::array<double, 2 * Q>> sweeps;
   double                tg3         = 0.;
   bool                  extrapolate = false;
@@ -599,7 +600,10 @@ namespace
         s.cg_iterations += its;
         s.last_its = 0;
         for (int v = 0; v < W.nv; ++v)
-          s.last_its = std::max(s.last_its, s.host_state[v].its);
+          {
+            s.last_its                      = std::max(s.last_its, s.host_state[v].its);
+            s.last_step_its[sw][W.block[v]] = s.host_state[v].its;
+          }
         if (st != LBFEM_OK)
           return st;
       }
@@ -618,7 +622,7 @@ namespace
   // the host first reads an inner solve's stopping state at the largest count of
   // the previous sweep's inner solve of the same refinement.
   int
-  run_mixed_cg(lbfem_scheme &s, const CgWork &W64, double *x64, const double *b64, int64_t *its)
+  run_mixed_cg(lbfem_scheme &s, const CgWork &W64, double *x64, const double *b64, int64_t *its, int *per_pop)
   {
     const Geo &G         = s.d->G;
     const int  max_outer = 30;
@@ -660,7 +664,10 @@ namespace
         *its += inner;
         s.last_inner[slot] = 0;
         for (int v = 0; v < W32.nv; ++v)
-          s.last_inner[slot] = std::max(s.last_inner[slot], s.host_state32[v].its);
+          {
+            s.last_inner[slot] = std::max(s.last_inner[slot], s.host_state32[v].its);
+            per_pop[W32.block[v]] += s.host_state32[v].its;
+          }
         HIP_CHECK(refine_add_launch(G, x64, s.d32.p, mask, s.stream));
       }
   }
@@ -703,7 +710,7 @@ namespace
           ++W.nv;
         }
     int64_t   its = 0;
-    const int st  = run_mixed_cg(s, W, xs, s.rhs.p, &its);
+    const int st  = run_mixed_cg(s, W, xs, s.rhs.p, &its, s.last_step_its[sw]);
     s.cg_iterations += its;
     if (st != LBFEM_OK)
       return st;
@@ -774,6 +781,9 @@ namespace
   int
   step_once(lbfem_scheme &s)
   {
+    for (auto &row : s.last_step_its)
+      for (int &v : row)
+        v = 0;
     const lbfem_disc &d = *s.d;
     const Geo        &G = d.G;
     const bool        ll = s.set.scheme == LBFEM_LEELIN;
@@ -1012,27 +1022,47 @@ namespace
     return LBFEM_OK;
   }
 
-  // graph[k]: graph_steps steps starting at parity k (graph_steps even, so parity returns to k)
+  // graph[k]: graph_steps steps starting at parity k (graph_steps even, so parity returns to k).
+  // Both graphs or neither: any failure drops what was captured.
   int
   capture_graphs(lbfem_batch &b)
   {
-    for (int k = 0; k < 2; ++k)
+    b.drop_graphs();
+    try
       {
-        HIP_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-        int st = LBFEM_OK;
-        for (int n = 0; n < b.graph_steps && st == LBFEM_OK; ++n)
-          st = batch_step_eager(b, (k + n) & 1);
-        hipGraph_t       gr = nullptr;
-        const hipError_t ec = hipStreamEndCapture(b.stream, &gr);
-        if (st != LBFEM_OK)
+        for (int k = 0; k < 2; ++k)
           {
-            if (gr)
-              (void)hipGraphDestroy(gr);
-            return st;
+            HIP_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+            int st = LBFEM_OK;
+            try
+              {
+                for (int n = 0; n < b.graph_steps && st == LBFEM_OK; ++n)
+                  st = batch_step_eager(b, (k + n) & 1);
+              }
+            catch (...)
+              {
+                hipGraph_t gr = nullptr;
+                (void)hipStreamEndCapture(b.stream, &gr); // leave the stream out of capture mode
+                if (gr)
+                  (void)hipGraphDestroy(gr);
+                throw;
+              }
+            hipGraph_t       gr = nullptr;
+            const hipError_t ec = hipStreamEndCapture(b.stream, &gr);
+            b.graph[k]          = gr; // owned by the batch from here (drop_graphs frees it)
+            if (st != LBFEM_OK)
+              {
+                b.drop_graphs();
+                return st;
+              }
+            HIP_CHECK(ec);
+            HIP_CHECK(hipGraphInstantiate(&b.exec[k], gr, nullptr, nullptr, 0));
           }
-        HIP_CHECK(ec);
-        b.graph[k] = gr;
-        HIP_CHECK(hipGraphInstantiate(&b.exec[k], gr, nullptr, nullptr, 0));
+      }
+    catch (...)
+      {
+        b.drop_graphs();
+        throw;
       }
     return LBFEM_OK;
   }
@@ -1374,6 +1404,8 @@ extern "C"
               HIP_CHECK(vmult_launch(d->G, W, x, d->s_ax.p, st));
               HIP_CHECK(richardson_update_launch(d->G, x, r, d->s_ax.p, 1, st));
             }
+          // the passes use the discretization's scratch (A x): done before the next call may reuse it
+          HIP_CHECK(hipStreamSynchronize(st));
           return LBFEM_OK;
         }
       int64_t   its = 0;
@@ -1516,14 +1548,15 @@ extern "C"
       const int64_t n = s->d->n;
       if (s->mixed) // deviations from the rest state, rounded to FP32
         {
-          std::vector<float> dv(static_cast<size_t>(n));
+          // on the scheme's stream, after the steps still queued there (a step writes g32[cur])
+          std::vector<float> dv(static_cast<size_t>(Q * n));
           for (int a = 0; a < Q; ++a)
-            {
-              for (int64_t i = 0; i < n; ++i)
-                dv[i] = float(f[a * n + i] - w_of(a));
-              HIP_CHECK(hipMemcpy(s->g32[s->cur].p + a * s->d->stride, dv.data(), n * sizeof(float),
-                                  hipMemcpyHostToDevice));
-            }
+            for (int64_t i = 0; i < n; ++i)
+              dv[a * n + i] = float(f[a * n + i] - w_of(a));
+          for (int a = 0; a < Q; ++a)
+            HIP_CHECK(hipMemcpyAsync(s->g32[s->cur].p + a * s->d->stride, dv.data() + a * n, n * sizeof(float),
+                                     hipMemcpyHostToDevice, s->stream));
+          HIP_CHECK(hipStreamSynchronize(s->stream));
           return LBFEM_OK;
         }
       for (int a = 0; a < Q; ++a)
@@ -1648,6 +1681,17 @@ extern "C"
     if (!s || !its)
       return fail(LBFEM_EINVAL, "null argument");
     *its = s->cg_iterations;
+    return LBFEM_OK;
+  }
+
+  int
+  lbfem_scheme_last_cg_iterations(const lbfem_scheme *s, int32_t *its)
+  {
+    if (!s || !its)
+      return fail(LBFEM_EINVAL, "null argument");
+    for (int sw = 0; sw < 2; ++sw)
+      for (int a = 0; a < NM; ++a)
+        its[sw * NM + a] = s->last_step_its[sw][a];
     return LBFEM_OK;
   }
 
@@ -1895,7 +1939,7 @@ extern "C"
       int left = n_steps;
       if (b->graph_steps > 0 && left >= b->graph_steps)
         {
-          if (!b->exec[0])
+          if (!b->exec[0] || !b->exec[1])
             if (const int st = capture_graphs(*b); st != LBFEM_OK)
               return st;
           for (; left >= b->graph_steps; left -= b->graph_steps)

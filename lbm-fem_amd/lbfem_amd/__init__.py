@@ -101,6 +101,7 @@ def lib():
             "lbfem_scheme_step": (i, [_vp, i]),
             "lbfem_scheme_synchronize": (i, [_vp]),
             "lbfem_scheme_cg_iterations": (i, [_vp, i64]),
+            "lbfem_scheme_last_cg_iterations": (i, [_vp, i32]),
             "lbfem_scheme_enable_stage_timers": (i, [_vp, i]),
             "lbfem_scheme_stage_times": (i, [_vp, p]),
             "lbfem_diagnostics": (i, [_vp, _vp, _vp, _vp, p, _vp]),
@@ -339,6 +340,13 @@ class Scheme:
         v = C.c_int64(0)
         check(lib().lbfem_scheme_cg_iterations(self.h, C.byref(v)))
         return v.value
+
+    def last_cg_iterations(self):
+        """The last step's CG steps per solve (SolverControl::last_step(), include/lbfem/mass.h:204-211),
+        shape (2 sweeps, 8 populations); 0 where no CG ran."""
+        v = np.zeros(16, dtype=np.int32)
+        check(lib().lbfem_scheme_last_cg_iterations(self.h, v.ctypes.data_as(C.POINTER(C.c_int32))))
+        return v.reshape(2, 8)
 
     def enable_stage_timers(self, on=True):
         check(lib().lbfem_scheme_enable_stage_timers(self.h, int(on)))

@@ -1002,6 +1002,9 @@ namespace
     std::vector<double> G(nq), u(nq), gx(nq), gy(nq), v(nq), Fx(nq), Fy(nq), T(nq);
     std::vector<double> Rc((size_t)N * N * nq);
     const Basis1D      &B = d.B;
+    // cells are independent (each writes its own block of Rc); with OpenMP each
+    // thread has its own scratch, and the result is the serial one bit for bit
+#pragma omp parallel for collapse(2) schedule(static) firstprivate(G, u, gx, gy, v, Fx, Fy, T)
     for (int cy = 0; cy < N; ++cy)
       for (int cx = 0; cx < N; ++cx)
         {
@@ -1055,6 +1058,7 @@ namespace
             cell_integrate(d, v.data(), Fx.data(), Fy.data(), hx, hy, Rc.data() + (size_t)(cy * N + cx) * nq);
         }
     d.assemble(Rc.data(), y);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
       if (d.constrained[i])
         y[i] = x[i];
@@ -1113,29 +1117,33 @@ namespace
       {
         ++it;
         rho_prev = rho;
+        // the element-wise loops run in parallel (independent entries); the dot
+        // products stay serial, in index order
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) // DiagonalMatrix::vmult with the inverse entries 1/diag
           v[i] = (1. / diag[i]) * r[i];
         rho = dot(r.data(), v.data(), n);
         if (it > 1)
           {
             const double beta = rho / rho_prev;
+#pragma omp parallel for schedule(static)
             for (int i = 0; i < n; ++i)
               p[i] = beta * p[i] - v[i];
           }
         else
+#pragma omp parallel for schedule(static)
           for (int i = 0; i < n; ++i)
             p[i] = -v[i];
         A(v.data(), p.data());
         const double pAp   = dot(p.data(), v.data(), n);
         const double alpha = rho / pAp;
-        for (int i = 0; i < n; ++i)
-          x[i] += alpha * p[i];
-        double rr = 0.;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i)
           {
+            x[i] += alpha * p[i];
             r[i] += alpha * v[i];
-            rr += r[i] * r[i];
           }
+        const double rr = dot(r.data(), r.data(), n);
         res = std::sqrt(std::abs(rr));
         if (res <= tol)
           {
@@ -1335,6 +1343,7 @@ struct ora_scheme
   double              dt = 0., lambda = 0.;
   std::vector<double> f, feq, rhs, x[2];
   long long           cg_iterations = 0;
+  int                 last_its[2][NM] = {}; // the last step's CG steps per sweep and moving population
   std::vector<std::array<double, Q * 2>> sweeps;
   double              tg3     = 0.;
   bool                extrapolate = false;
@@ -1711,6 +1720,13 @@ extern "C"
     return s->cg_iterations;
   }
   void
+  ora_scheme_last_cg_iterations(const ora_scheme *s, int *its)
+  {
+    for (int sw = 0; sw < 2; ++sw)
+      for (int a = 0; a < NM; ++a)
+        its[sw * NM + a] = s->last_its[sw][a];
+  }
+  void
   ora_scheme_get_increments(const ora_scheme *s, const int which, double *x)
   {
     std::copy(s->x[which].begin(), s->x[which].end(), x);
@@ -1752,6 +1768,7 @@ namespace
           const int st  = mass_solve(d, xa, s.rhs.data() + a * n, s.mass_type, s.richardson, s.cg_tol, s.cg_max,
                                     s.tg3 * s.dt * s.dt, e[2 * (a + 1)], e[2 * (a + 1) + 1], &its);
           s.cg_iterations += its;
+          s.last_its[sw][a] = its; // SolverControl::last_step() of this solve
           if (st != ORA_OK)
             return st;
         }
@@ -1768,6 +1785,9 @@ extern "C"
     const int       n = d.n;
     for (int k = 0; k < n_steps; ++k)
       {
+        for (auto &row : s->last_its)
+          for (int &v : row)
+            v = 0;
         if (s->scheme == ORA_LEELIN) // schemes.h:92-98
           {
             ora_compute_equilibrium(&d, s->f.data(), s->feq.data());

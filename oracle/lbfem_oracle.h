@@ -86,6 +86,9 @@ void        ora_scheme_set_populations(ora_scheme *s, const double *f);
 void        ora_scheme_get_populations(const ora_scheme *s, double *f);
 int         ora_scheme_step(ora_scheme *s, int n_steps); /* ORA_OK or ORA_ENOCONV */
 long long   ora_scheme_cg_iterations(const ora_scheme *s);
+/* the last step's CG steps per solve (SolverControl::last_step()): its[sw * 8 + a - 1]
+ * for sweep sw (0, 1) and population a; 0 for populations not solved */
+void        ora_scheme_last_cg_iterations(const ora_scheme *s, int *its);
 /* the increments of the last step's sweep s (8 blocks), and x[1] */
 void ora_scheme_get_increments(const ora_scheme *s, int which, double *x);
 

@@ -83,6 +83,7 @@ def lib():
             "ora_scheme_get_populations": (None, [vp, _P]),
             "ora_scheme_step": (C.c_int, [vp, C.c_int]),
             "ora_scheme_cg_iterations": (C.c_longlong, [vp]),
+            "ora_scheme_last_cg_iterations": (None, [vp, _IP]),
             "ora_scheme_get_increments": (None, [vp, C.c_int, _P]),
             "ora_diagnostics": (None, [vp, _P, C.c_double, _P, _P]),
             "ora_stream_function": (C.c_int, [vp, _P, _P, _P, C.POINTER(C.c_int)]),
@@ -314,6 +315,12 @@ class Scheme:
     @property
     def cg_iterations(self):
         return lib().ora_scheme_cg_iterations(self.h)
+
+    def last_cg_iterations(self):
+        """The last step's CG steps per solve, shape (2 sweeps, 8 populations); 0 where not solved."""
+        v = np.zeros(16, dtype=np.int32)
+        lib().ora_scheme_last_cg_iterations(self.h, v.ctypes.data_as(_IP))
+        return v.reshape(2, 8)
 
 
 def moments(f):

@@ -166,3 +166,23 @@ def test_mixed_cg_decay_matches_analytic():
     e32 = l2_rel(u32[indep], v32[indep], ue[indep], ve[indep], w)
     e64 = l2_rel(u64[indep], v64[indep], ue[indep], ve[indep], w)
     assert abs(e32 - e64) < 1e-5, (e32, e64)
+
+
+def test_mixed_set_populations_after_queued_steps():
+    """set_populations follows the steps still queued on the scheme's stream: after step(50) with no
+    synchronize, the populations read back are the ones set (to FP32 rounding of the deviations), not
+    a queued step's output."""
+    tc = A.test_cases.make("cavity", mach=0.1, stretch=1.2)
+    tc.set_reynolds(400.0)
+    d = A.Discretization(tc, 128, 1)
+    s = A.LeeLin(d, mass=A.MASS_LUMPED, precision=A.MIXED)
+    s.set_time_step(d.time_step(0.5), tc.relaxation_time())
+    s.set_initial_populations(tc)
+    rng = np.random.default_rng(5)
+    f = A.test_cases.W[:, None] * (1 + 0.01 * rng.standard_normal((9, d.n_nodes)))
+    for _ in range(3):
+        s.step(50)  # queued, not synchronized
+        s.set_populations(f)
+        got = s.f
+        dev = (f - A.test_cases.W[:, None]).astype(np.float32).astype(np.float64) + A.test_cases.W[:, None]
+        assert np.array_equal(got, dev)

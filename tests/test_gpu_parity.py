@@ -47,10 +47,12 @@ def run_both(case, N, p, scheme, mass, steps, cfl=0.25, streaming=A.TG2, cg_tol=
     f0g, f0o = gs.f, osch.populations()
     assert rel(f0g, f0o) < 1e-14, "initial populations"
     errs = []
+    gs.its_diff = []  # per step: |GPU - oracle| CG steps of every solve, shape (2 sweeps, 8 populations)
     for k in range(steps):
         gs.step(1)
         osch.step(1)
         errs.append(rel(gs.f, osch.populations()))
+        gs.its_diff.append(np.abs(gs.last_cg_iterations() - osch.last_cg_iterations()))
     return gs, osch, errs, gd, od
 
 
@@ -87,18 +89,21 @@ def test_consistent_mass_step(case, p, scheme):
     gs, osch, errs, _, _ = run_both(case, N, p, scheme, A.MASS_CG, steps=steps)
     assert errs[0] < 1e-12, errs
     assert max(errs) < 1e-11, errs
+    diff = np.array(gs.its_diff)  # (steps, sweeps, populations)
     if case == "couette" and p > 1:
-        # the x-directed populations of the x-uniform Couette flow have a right-hand side
-        # of pure round-off for p > 1 (a row of the Q_p bands sums to zero only to
-        # round-off); the CG iterations on such noise are arbitrary in any implementation
-        return
-    assert abs(gs.cg_iterations - osch.cg_iterations) <= 8 * steps, (gs.cg_iterations, osch.cg_iterations)
+        # the x-directed populations (1 and 5) of the x-uniform Couette flow have a right-hand
+        # side of pure round-off for p > 1 (a row of the Q_p bands sums to zero only to
+        # round-off); the CG iterations on such noise are arbitrary in any implementation,
+        # so those two solves are left out of the count check and the other six kept
+        diff[:, :, [0, 4]] = 0
+    assert diff.max() <= 1, diff  # every solve of every step within +-1
 
 
 @pytest.mark.parametrize("streaming", [A.TG3, A.TG3_SPLIT])
 def test_tg3_streaming(streaming):
-    _, _, errs, _, _ = run_both("tgv", 12, 1, A.BARDOW, A.MASS_CG, steps=3, cfl=0.9, streaming=streaming)
+    gs, _, errs, _, _ = run_both("tgv", 12, 1, A.BARDOW, A.MASS_CG, steps=3, cfl=0.9, streaming=streaming)
     assert max(errs) < 1e-11, errs
+    assert np.array(gs.its_diff).max() <= 1, gs.its_diff  # per solve, both sweeps of the split form
 
 
 def test_richardson_mass():
@@ -144,12 +149,11 @@ def test_plain_vortex_1000_steps(scheme, cfl):
     osch.set_initial_populations()
     worst = 0
     for _ in range(1000):
-        g0, o0 = gs.cg_iterations, osch.cg_iterations
         gs.step(1)
         osch.step(1)
-        worst = max(worst, abs((gs.cg_iterations - g0) - (osch.cg_iterations - o0)))
+        worst = max(worst, int(np.abs(gs.last_cg_iterations() - osch.last_cg_iterations()).max()))
     assert rel(gs.f, osch.populations()) < 1e-10
-    assert worst <= 8, worst  # 8 solves per step, +-1 each
+    assert worst <= 1, worst  # every solve of every step within +-1
 
 
 # ------------------------------------------------------------------ operators
