@@ -22,16 +22,26 @@ FLAGS = [
 ]
 
 
-def build(verbose=True, jobs=None):
+def build(verbose=True, jobs=None, incremental=False):
+    """Compiles every source (incremental=True keeps the objects in _lib/obj and recompiles a source only
+    when it, a header or this file is newer than its object)."""
     os.makedirs(OUT_DIR, exist_ok=True)
+    obj_dir = os.path.join(OUT_DIR, "obj") if incremental else OUT_DIR
+    os.makedirs(obj_dir, exist_ok=True)
+    deps = [os.path.abspath(__file__)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")] \
+        + [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    newest_dep = max(os.path.getmtime(f) for f in deps)
     objs = []
     procs = []
     t0 = time.time()
     for src in SOURCES:
-        obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
+        obj = os.path.join(obj_dir, src.replace(".hip", ".o"))
+        objs.append(obj)
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(
+                newest_dep, os.path.getmtime(os.path.join(CSRC, src))):
+            continue
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
-        objs.append(obj)
     failed = False
     for src, p in procs:
         out, _ = p.communicate()
@@ -48,8 +58,9 @@ def build(verbose=True, jobs=None):
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
-    for o in objs:
-        os.remove(o)
+    if not incremental:
+        for o in objs:
+            os.remove(o)
     # the C++ driver over the C ABI (host code only: g++, no HIP headers)
     drv = os.path.join(OUT_DIR, "cgdbe_hip")
     subprocess.check_call([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
@@ -62,4 +73,4 @@ def build(verbose=True, jobs=None):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
