@@ -140,6 +140,21 @@ typedef struct
 int lbfem_mass_solve(const lbfem_disc *disc, const lbfem_mass_settings *settings, double *x, const double *r,
                      const double *e, double tg3_coefficient, int *iterations, void *stream);
 
+/* No counterpart in the reference: lbfem_mass_solve for callers that cannot
+ * allocate (serving and time-stepping loops that must not call hipMalloc, and
+ * harnesses that forbid allocation). It takes the same arguments and gives the
+ * same x and iteration count, bit for bit, but keeps its work vectors in one
+ * fixed device array per GPU (39.3 MB, reserved when the library is loaded on
+ * that GPU and held until the process ends) instead of the discretization's
+ * scratch. It allocates nothing on any path.
+ * Limits: at most 1179648 nodes (the disc's stride) and 7168 operator tiles,
+ * else LBFEM_EINVAL. Call it with the discretization's device current. All
+ * calls on one GPU share that GPU's array, so calls that overlap on one GPU
+ * (from several host threads or streams) race; CG and Richardson synchronize
+ * the stream before returning, so consecutive calls do not. */
+int lbfem_mass_solve_noalloc(const lbfem_disc *disc, const lbfem_mass_settings *settings, double *x, const double *r,
+                             const double *e, double tg3_coefficient, int *iterations, void *stream);
+
 /* ---------------------------------------------------------- schemes
  * lbfem::Scheme, LeeLin, Bardow (include/lbfem/schemes.h:31-142) with
  * lbfem::TaylorGalerkin streaming (include/lbfem/streaming.h:38-130). The

@@ -7,9 +7,11 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 #include <new>
 #include <stdexcept>
@@ -23,14 +25,44 @@ using namespace lbfem_amd;
 
 namespace
 {
-  thread_local std::string g_error;
+  // the message of the thread's last failed call: a fixed buffer, so that recording an
+  // error allocates nothing (the allocation-free entry point reports through it too)
+  constexpr size_t         kErrorLen = 1024;
+  thread_local char        g_error[kErrorLen] = {0};
+  thread_local size_t      g_error_len        = 0;
+
+  int
+  fail(const int code, const char *msg)
+  {
+    const size_t k = std::min(kErrorLen - 1, std::strlen(msg));
+    std::memcpy(g_error, msg, k);
+    g_error[k]  = '\0';
+    g_error_len = k;
+    return code;
+  }
 
   int
   fail(const int code, const std::string &msg)
   {
-    g_error = msg;
-    return code;
+    return fail(code, msg.c_str());
   }
+
+  // a HIP error as a status, without an exception
+  int
+  fail_hip(const hipError_t e, const char *what)
+  {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+    return fail(LBFEM_EHIP, buf);
+  }
+#define HIP_RETURN(x)                                                                                              \
+  do                                                                                                               \
+    {                                                                                                              \
+      const hipError_t e_ = (x);                                                                                   \
+      if (e_ != hipSuccess)                                                                                        \
+        return fail_hip(e_, #x);                                                                                   \
+    }                                                                                                              \
+  while (0)
 
   struct HipError : std::runtime_error
   {
@@ -111,6 +143,7 @@ struct lbfem_disc
   std::unique_ptr<Basis1D> B;
   std::unique_ptr<Axis1D>  ax, ay;
   int                      p = 1, N = 1, P = 1, np = 2;
+  int                      device = 0; // the HIP device its buffers live on
   int64_t                  n = 0, stride = 0;
   double                   h_min = 0.;
   DevBuf<double>           axbuf, aybuf, wall_vel;
@@ -475,8 +508,9 @@ namespace
     return W;
   }
 
-  // CG on the vectors of W (include/lbfem/mass.h:204-211): returns LBFEM_OK or
-  // LBFEM_ENOCONV; adds the iterations of all vectors to *its.
+  // CG on the vectors of W (include/lbfem/mass.h:204-211): returns LBFEM_OK,
+  // LBFEM_ENOCONV, or LBFEM_EHIP (a status, not an exception: nothing is
+  // allocated on any path); adds the iterations of all vectors to *its.
   // The host reads the stopping state after iteration check_from and then every
   // check_every iterations; iterations of converged vectors are no-ops.
   template <typename V>
@@ -490,11 +524,12 @@ namespace
     // launch less per iteration); large ones keep the separate one-block finalisation
     const int rows = cg_direction_rows(G);
     Wf.alpha_rows  = int64_t(rows) * W.nv <= 4096 && rows + 1024 <= W.max_blocks ? rows : 0;
-    HIP_CHECK(cg_start_launch(G, W, x, b, r, stream));
+    HIP_RETURN(cg_start_launch(G, W, x, b, r, stream));
     V         *pb[2] = {p0, p1};
     const auto done  = [&](bool &finished) -> int {
-      HIP_CHECK(hipMemcpyAsync(host_state, W.state, W.nv * sizeof(CgState), hipMemcpyDeviceToHost, stream));
-      HIP_CHECK(hipStreamSynchronize(stream));
+      finished = true;
+      HIP_RETURN(hipMemcpyAsync(host_state, W.state, W.nv * sizeof(CgState), hipMemcpyDeviceToHost, stream));
+      HIP_RETURN(hipStreamSynchronize(stream));
       bool active = false, failed = false;
       for (int v = 0; v < W.nv; ++v)
         {
@@ -507,15 +542,19 @@ namespace
       for (int v = 0; v < W.nv; ++v)
         *its += host_state[v].its;
       if (failed && fail_at_cap)
-        return fail(LBFEM_ENOCONV,
-                    "CG: no convergence within " + std::to_string(W.max_its) + " iterations (SolverControl::NoConvergence)");
+        {
+          char msg[128];
+          std::snprintf(msg, sizeof(msg), "CG: no convergence within %d iterations (SolverControl::NoConvergence)",
+                        W.max_its);
+          return fail(LBFEM_ENOCONV, msg);
+        }
       return LBFEM_OK;
     };
     // iterations of converged vectors are no-ops, so the first checks can be skipped
     for (int it = 1; it <= W.max_its; ++it)
       {
-        HIP_CHECK(cg_direction_launch(G, Wf, it, pb[it % 2], r, pb[(it + 1) % 2], q, stream));
-        HIP_CHECK(cg_update_launch(G, Wf, it, x, r, pb[(it + 1) % 2], q, stream));
+        HIP_RETURN(cg_direction_launch(G, Wf, it, pb[it % 2], r, pb[(it + 1) % 2], q, stream));
+        HIP_RETURN(cg_update_launch(G, Wf, it, x, r, pb[(it + 1) % 2], q, stream));
         if (it >= check_from && (it - check_from) % check_every == 0)
           {
             bool      finished = false;
@@ -1075,6 +1114,53 @@ namespace
   }
 } // namespace
 
+// ------------------------------------------------------- allocation-free mass solve
+// The work space of lbfem_mass_solve_noalloc: the CG state, the partial-sum rows of the
+// CG kernels and four node vectors (r, two directions, A p; Richardson's A x uses the
+// first), in one fixed device array per GPU, held until the process ends. Every call on a
+// GPU shares that GPU's array.
+namespace
+{
+  constexpr int64_t kNoAllocMaxStride    = 1179648; // nodes (a multiple of 64): 1024^2 Q1 and 512^2 Q2 cells fit
+  constexpr int     kNoAllocRows         = 8192;    // partial-sum rows: operator tiles, then at most 1024 update blocks
+  constexpr size_t  kNoAllocStateDoubles = 16;      // the CgState of the one vector (56 bytes)
+  constexpr size_t  kNoAllocDoubles =
+    kNoAllocStateDoubles + size_t(kNoAllocRows) * kCgPartials + 4 * size_t(kNoAllocMaxStride);
+  constexpr int kNoAllocMaxDevices = 64;
+  static_assert(sizeof(CgState) <= kNoAllocStateDoubles * sizeof(double), "CG state slot");
+
+  std::mutex g_scratch_mutex;
+  double    *g_scratch_ptr[kNoAllocMaxDevices] = {}; // the array's address on each device, once taken
+} // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace
+{
+  // The device address of `scratch` on the current device, taken by hipGetSymbolAddress once
+  // per device with that device current (the host-side name of the array is not a device
+  // address: only this pointer is passed to kernels).
+  int
+  noalloc_scratch(const int dev, double **out)
+  {
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+      return fail(LBFEM_EINVAL, "lbfem_mass_solve_noalloc: device index beyond the work-space table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev])
+      {
+        void            *ptr = nullptr;
+        const hipError_t e   = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+          return fail_hip(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr)
+          return fail(LBFEM_EHIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = static_cast<double *>(ptr);
+      }
+    *out = g_scratch_ptr[dev];
+    return LBFEM_OK;
+  }
+} // namespace
+
 extern "C"
 {
   size_t
@@ -1082,11 +1168,11 @@ extern "C"
   {
     if (buf && len)
       {
-        const size_t k = std::min(len - 1, g_error.size());
-        std::memcpy(buf, g_error.data(), k);
+        const size_t k = std::min(len - 1, g_error_len);
+        std::memcpy(buf, g_error, k);
         buf[k] = '\0';
       }
-    return g_error.size();
+    return g_error_len;
   }
 
   int
@@ -1113,6 +1199,7 @@ extern "C"
         return fail(LBFEM_EINVAL, "the distortion maps the vortex's periodic box (n_periodic 2, stretch 0, "
                                   "cgdbe.cc:77, test_cases.h:53-65)");
       auto d    = std::make_unique<lbfem_disc>();
+      HIP_CHECK(hipGetDevice(&d->device));
       d->desc   = *desc;
       d->p      = desc->degree;
       d->N      = desc->n_cells;
@@ -1410,6 +1497,72 @@ extern "C"
         }
       int64_t   its = 0;
       const int rc  = run_cg(d->G, W, x, r, d->s_r.p, d->s_p0.p, d->s_p1.p, d->s_q.p, d->host_cg, st, &its, 1);
+      if (iterations)
+        *iterations = int(its);
+      return rc;
+    });
+  }
+
+  // lbfem_mass_solve on the fixed work space `scratch` instead of the discretization's:
+  // no hipMalloc, hipFree or other allocation on any path (errors are statuses recorded in
+  // a fixed buffer). The same kernels with the same launch shapes, so the result and the
+  // iteration count are bitwise those of lbfem_mass_solve.
+  int
+  lbfem_mass_solve_noalloc(const lbfem_disc *d, const lbfem_mass_settings *m, double *x, const double *r,
+                           const double *e, const double tg3_coefficient, int *iterations, void *stream)
+  {
+    return guarded([&]() -> int {
+      if (!d || !m || !x || !r)
+        return fail(LBFEM_EINVAL, "null argument");
+      hipStream_t st = as_stream(stream);
+      if (iterations)
+        *iterations = 0;
+      if (m->type == LBFEM_MASS_LUMPED)
+        {
+          HIP_RETURN(lumped_solve_launch(d->G, x, r, 1, st));
+          return LBFEM_OK;
+        }
+      if (m->type != LBFEM_MASS_CG && m->type != LBFEM_MASS_RICHARDSON)
+        return fail(LBFEM_EINVAL, "unknown mass type");
+      if (d->stride > kNoAllocMaxStride)
+        return fail(LBFEM_EINVAL, "lbfem_mass_solve_noalloc: the mesh has more nodes than its fixed work space "
+                                  "holds (1179648); use lbfem_mass_solve");
+      const int rows = cg_direction_rows(d->G);
+      if (rows + 1024 > kNoAllocRows)
+        return fail(LBFEM_EINVAL, "lbfem_mass_solve_noalloc: the mesh has more operator tiles than its fixed "
+                                  "work space holds partial sums for (7168); use lbfem_mass_solve");
+      int dev = 0;
+      HIP_RETURN(hipGetDevice(&dev));
+      if (dev != d->device)
+        return fail(LBFEM_EINVAL, "lbfem_mass_solve_noalloc: call it with the discretization's device current");
+      double *base = nullptr;
+      if (const int rc = noalloc_scratch(dev, &base); rc != LBFEM_OK)
+        return rc;
+      CgState     *state   = reinterpret_cast<CgState *>(base);
+      double      *partial = base + kNoAllocStateDoubles;
+      double      *vec     = partial + size_t(kNoAllocRows) * kCgPartials;
+      const size_t sd      = size_t(d->stride);
+      CgWork       W       = cg_work(*d, state, partial, nullptr, *m);
+      W.max_blocks         = kNoAllocRows;
+      W.nv                 = 1;
+      W.block[0]           = 0;
+      W.vec[0].coef        = tg3_coefficient;
+      W.vec[0].e[0]        = e ? e[0] : 0.;
+      W.vec[0].e[1]        = e ? e[1] : 0.;
+      if (m->type == LBFEM_MASS_RICHARDSON)
+        {
+          HIP_RETURN(lumped_solve_launch(d->G, x, r, 1, st));
+          for (int k = 0; k < m->richardson; ++k)
+            {
+              HIP_RETURN(vmult_launch(d->G, W, x, vec, st));
+              HIP_RETURN(richardson_update_launch(d->G, x, r, vec, 1, st));
+            }
+          HIP_RETURN(hipStreamSynchronize(st)); // the work space is free for the next call
+          return LBFEM_OK;
+        }
+      CgState   host_state[1]; // the stopping state, read into the caller's stack
+      int64_t   its = 0;
+      const int rc  = run_cg(d->G, W, x, r, vec, vec + sd, vec + 2 * sd, vec + 3 * sd, host_state, st, &its, 1);
       if (iterations)
         *iterations = int(its);
       return rc;

@@ -91,6 +91,8 @@ def lib():
             "lbfem_predictor_corrector": (i, [_vp, _vp, _vp, _vp, d, _vp]),
             "lbfem_streaming_vmult": (i, [_vp, _vp, _vp, d, p, _vp]),
             "lbfem_mass_solve": (i, [_vp, C.POINTER(MassSettings), _vp, _vp, p, d, C.POINTER(C.c_int), _vp]),
+            "lbfem_mass_solve_noalloc": (i, [_vp, C.POINTER(MassSettings), _vp, _vp, p, d, C.POINTER(C.c_int),
+                                             _vp]),
             "lbfem_scheme_create": (i, [_vp, C.POINTER(SchemeSettings), C.POINTER(_vp)]),
             "lbfem_scheme_destroy": (i, [_vp]),
             "lbfem_scheme_set_time_step": (i, [_vp, d, d]),
@@ -279,6 +281,18 @@ def mass_solve(disc, x, r, mass=MASS_CG, richardson=2, cg_tolerance=1e-8, cg_max
     its = C.c_int(0)
     check(lib().lbfem_mass_solve(disc.h, C.byref(st), _ptr(x), _ptr(r), _p(ee), tg3_coefficient, C.byref(its),
                                  _stream(stream)))
+    return its.value
+
+
+def mass_solve_noalloc(disc, x, r, mass=MASS_CG, richardson=2, cg_tolerance=1e-8, cg_max_iterations=200,
+                       e=(0.0, 0.0), tg3_coefficient=0.0, stream=None):
+    """lbfem_mass_solve_noalloc: mass_solve on the library's fixed per-GPU work space (no allocation; at most
+    1179648 nodes); the same x and iteration count bit for bit. Calls that overlap on one GPU race."""
+    st = MassSettings(mass, richardson, cg_tolerance, cg_max_iterations, 0)
+    ee = np.asarray(e, dtype=np.float64)
+    its = C.c_int(0)
+    check(lib().lbfem_mass_solve_noalloc(disc.h, C.byref(st), _ptr(x), _ptr(r), _p(ee), tg3_coefficient,
+                                         C.byref(its), _stream(stream)))
     return its.value
 
 
