@@ -241,6 +241,15 @@ int lbfem_disc_stream_function(const lbfem_disc *disc, const double *u, const do
 int lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const double *v, double *psi,
                           int *iterations);
 
+/* The velocity-change monitor of cgdbe's steady-state test (the diagnostics
+ * of apps/cgdbe/cgdbe.cc:385-393) on the device, for FP64 and mixed schemes:
+ * over the non-slave nodes, the largest |u - u_prev| and the lumped-mass mean
+ * of |u - u_prev|, u_prev the velocity at the scheme's previous call (kept by
+ * the scheme, 16 bytes per node in FP64, allocated on the first call, which only
+ * records it and returns +inf for both), and the number of nodes whose moments
+ * are not finite (a diverged run). Synchronizes the scheme's stream. */
+int lbfem_scheme_velocity_change(lbfem_scheme *scheme, double *max_change, double *mean_change, int64_t *nonfinite);
+
 /* ---------------------------------------------------------- batches
  * No counterpart in the reference (one mesh per process there): many lumped
  * TG2 schemes (any degree, scheme, FP64 or mixed, sizes and flows) stepped
@@ -251,7 +260,22 @@ int lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const do
  * members are stepped only through the batch and read after
  * lbfem_batch_synchronize; destroy the batch before its members. */
 typedef struct lbfem_batch lbfem_batch;
+/* n may be 0 (members join later with lbfem_batch_add) */
 int lbfem_batch_create(lbfem_scheme *const *schemes, int n, lbfem_batch **batch);
+/* A scheme joins a running batch, from its current populations, at any step; a
+ * member leaves it (the batch's stream is synchronized first, so the member can
+ * then be read, stepped alone or destroyed). The group's device table is
+ * rewritten; the captured graphs are kept unless a (degree, scheme, precision)
+ * group appears or disappears, or a group launched one block per tile changes
+ * its tile count. */
+int lbfem_batch_add(lbfem_batch *batch, lbfem_scheme *scheme);
+int lbfem_batch_remove(lbfem_batch *batch, lbfem_scheme *scheme);
+/* lbfem_scheme_velocity_change for the members of the batch in one launch:
+ * arrays of the batch's size in member order (creation order, then joins);
+ * mask (optional): nonzero for the members to measure, the others' entries and
+ * u_prev untouched. */
+int lbfem_batch_velocity_change(lbfem_batch *batch, const int32_t *mask, double *max_change, double *mean_change,
+                                int64_t *nonfinite);
 int lbfem_batch_destroy(lbfem_batch *batch);
 /* steps per captured graph: even, 0 = launch every step eagerly (default 8) */
 int lbfem_batch_set_graph_steps(lbfem_batch *batch, int steps);
@@ -260,8 +284,9 @@ int lbfem_batch_synchronize(lbfem_batch *batch);
 int lbfem_batch_get_stream(const lbfem_batch *batch, void **stream);
 /* groups, nodes (slaves included) and tiles of the batch */
 int lbfem_batch_info(const lbfem_batch *batch, int *n_groups, int64_t *n_nodes, int64_t *n_tiles);
-/* every group's tile variant and grid (0: one block per tile; else about that
- * many persistent blocks, rounded up to a multiple of 8) */
+/* every group's tile variant (-1: each keeps its own) and grid (0: one block
+ * per tile; else that many persistent blocks, rounded up to a multiple of 8);
+ * groups created later take the same shape */
 int lbfem_batch_set_shape(lbfem_batch *batch, int variant, int grid);
 /* times variant x grid for every group (no state change) and keeps the fastest;
  * seconds: the sum over groups of the best time per step */

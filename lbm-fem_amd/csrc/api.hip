@@ -376,6 +376,67 @@ struct lbfem_disc
   }
 };
 
+namespace
+{
+  // the velocity-change monitor's device table, partial sums and results, and the pinned
+  // host copy of the results (allocated on first use, grown as needed)
+  struct Monitor
+  {
+    DevBuf<char>   table;
+    DevBuf<double> partial, results;
+    double        *host     = nullptr;
+    size_t         host_cap = 0;
+
+    Monitor() = default;
+    Monitor(const Monitor &) = delete;
+    Monitor &operator=(const Monitor &) = delete;
+    ~Monitor()
+    {
+      if (host)
+        (void)hipHostFree(host);
+    }
+
+    // runs the monitor over the entries on the stream; out: kMonitorPartials doubles per entry
+    void
+    run(std::vector<MonitorEntry> &E, hipStream_t stream, double *out)
+    {
+      const int n = int(E.size());
+      if (n == 0)
+        return;
+      int blocks = 0;
+      for (auto &e : E)
+        {
+          e.block_begin = blocks;
+          e.nblocks     = monitor_blocks(int64_t(e.G.np) * e.G.np);
+          blocks += e.nblocks;
+        }
+      if (table.n < E.size() * sizeof(MonitorEntry))
+        table.alloc(E.size() * sizeof(MonitorEntry));
+      if (partial.n < size_t(blocks) * kMonitorPartials)
+        partial.alloc(size_t(blocks) * kMonitorPartials);
+      const size_t nr = size_t(n) * kMonitorPartials;
+      if (results.n < nr)
+        results.alloc(nr);
+      if (host_cap < nr)
+        {
+          if (host)
+            (void)hipHostFree(host);
+          host     = nullptr;
+          host_cap = 0;
+          HIP_CHECK(hipHostMalloc(&host, nr * sizeof(double)));
+          host_cap = nr;
+        }
+      HIP_CHECK(hipMemcpyAsync(table.p, E.data(), E.size() * sizeof(MonitorEntry), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(monitor_launch(reinterpret_cast<const MonitorEntry *>(table.p), n, blocks, partial.p, results.p,
+                               stream));
+      HIP_CHECK(hipMemcpyAsync(host, results.p, nr * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      std::copy(host, host + nr, out);
+    }
+  };
+
+} // namespace
+
 struct lbfem_scheme
 {
   const lbfem_disc     *d = nullptr;
@@ -397,6 +458,10 @@ struct lbfem_scheme
   CgState              *host_state = nullptr; // pinned
   int64_t               cg_iterations = 0;
   int                   last_step_its[2][NM] = {}; // the last step's CG steps per sweep and moving population
+  // the velocity-change monitor: the velocity at its last call (2 per node) and its buffers
+  DevBuf<double>        u_prev;
+  bool                  u_prev_set = false;
+  Monitor               monitor;
   std::vector<std::array<double, 2 * Q>> sweeps;
   double                tg3         = 0.;
   bool                  extrapolate = false;
@@ -936,26 +1001,39 @@ namespace
 // ---------------------------------------------------------------------- batches
 // Many lumped-mass TG2 schemes stepped together: the members are grouped by
 // (degree, scheme, precision), and a step is one launch per group of the
-// batched tile kernel over the group's concatenated tile list.
+// batched tile kernel over the group's concatenated tile list. Members join and
+// leave a running batch: a group's device table (its live counts, then the
+// entries) is rewritten, and the captured step graphs, which read the counts at
+// run time, are kept unless a group appears or disappears.
+namespace
+{
+  // the monitor entry of a scheme at its current populations (its u_prev allocated on first use)
+  MonitorEntry
+  monitor_entry(lbfem_scheme &s);
+} // namespace
+
 struct lbfem_batch
 {
   struct Group
   {
-    int              p = 1, mode = 0, variant = kTileWide, grid = 0; // grid 0: one block per tile
-    bool             mixed = false;
-    std::vector<int> members;
-    int              n_tiles = 0;
-    int64_t          n_nodes = 0;
-    DevBuf<char>     table; // BatchEntry<double or float>[members]
+    int                         p = 1, mode = 0, variant = kTileWide, grid = 0; // grid 0: one block per tile
+    bool                        mixed = false;
+    std::vector<lbfem_scheme *> members;
+    int                         n_tiles = 0, launch_grid = 0, capacity = 0;
+    int64_t                     n_nodes = 0;
+    DevBuf<char>                table; // BatchCounts (kCountsBytes), then BatchEntry<double or float>[capacity]
   };
+  static constexpr size_t     kCountsBytes = 256;
   std::vector<lbfem_scheme *> members;
-  std::vector<int>            base_cur; // members' cur when the tables were written (parity 0)
+  std::vector<int>            par0; // each member's buffer read at batch parity 0
   std::vector<Group>          groups;
-  hipStream_t                 stream = nullptr;
-  int                         parity = 0;
+  hipStream_t                 stream      = nullptr;
+  int                         parity      = 0;
   int                         graph_steps = 8; // steps per captured graph, 0: no graphs
-  hipGraph_t                  graph[2]  = {nullptr, nullptr};
-  hipGraphExec_t              exec[2]   = {nullptr, nullptr};
+  int                         shape_variant = -1, shape_grid = 0; // the shape of groups created from here
+  hipGraph_t                  graph[2]    = {nullptr, nullptr};
+  hipGraphExec_t              exec[2]     = {nullptr, nullptr};
+  Monitor                     monitor;
 
   void
   drop_graphs()
@@ -971,6 +1049,15 @@ struct lbfem_batch
       }
   }
 
+  int
+  index_of(const lbfem_scheme *s) const
+  {
+    for (size_t k = 0; k < members.size(); ++k)
+      if (members[k] == s)
+        return int(k);
+    return -1;
+  }
+
   ~lbfem_batch()
   {
     drop_graphs();
@@ -981,25 +1068,29 @@ struct lbfem_batch
 
 namespace
 {
+  // The group's device table for its current members: entries at the members' buffer
+  // parities, tile lists at the group's variant, and the live counts. Rewritten on the
+  // batch's stream after the work queued there (the copy is synchronized). A larger table
+  // or another launch grid (one block per tile) drops the captured graphs.
   template <typename T>
   void
   write_table(lbfem_batch &b, lbfem_batch::Group &g)
   {
     std::vector<BatchEntry<T>> E(g.members.size());
     int                        begin = 0;
-    g.n_nodes                        = 0;
+    int                        tx, ty;
+    tile_dims(g.variant, tx, ty);
+    g.n_nodes = 0;
     for (size_t k = 0; k < g.members.size(); ++k)
       {
-        lbfem_scheme &s  = *b.members[g.members[k]];
+        lbfem_scheme &s  = *g.members[k];
         const Geo    &G  = s.d->G;
         const bool    ll = s.set.scheme == LBFEM_LEELIN;
-        int           tx, ty;
-        tile_dims(g.variant, tx, ty);
-        E[k].G          = G;
-        E[k].S          = tile_params(s, s.sweeps[0], ll);
-        E[k].S.collide  = 1;
-        E[k].S.tiles_x  = (G.P + 1 + tx - 1) / tx;
-        E[k].S.tiles_y  = (G.P + 1 + ty - 1) / ty;
+        E[k].G           = G;
+        E[k].S           = tile_params(s, s.sweeps[0], ll);
+        E[k].S.collide   = 1;
+        E[k].S.tiles_x   = (G.P + 1 + tx - 1) / tx;
+        E[k].S.tiles_y   = (G.P + 1 + ty - 1) / ty;
         if constexpr (sizeof(T) == 4)
           {
             E[k].f[0] = s.g32[0].p;
@@ -1010,45 +1101,67 @@ namespace
             E[k].f[0] = s.f[0].p;
             E[k].f[1] = s.f[1].p;
           }
-        E[k].par        = s.cur;
+        E[k].par        = b.par0[b.index_of(&s)];
         E[k].tile_begin = begin;
         begin += E[k].S.tiles_x * E[k].S.tiles_y;
         g.n_nodes += int64_t(G.np) * G.np;
       }
-    g.n_tiles = begin;
-    g.table.alloc(E.size() * sizeof(BatchEntry<T>));
-    HIP_CHECK(hipMemcpy(g.table.p, E.data(), E.size() * sizeof(BatchEntry<T>), hipMemcpyHostToDevice));
+    g.n_tiles            = begin;
+    const int nb8        = (g.n_tiles + 7) & ~7;
+    const int grid       = g.grid > 0 ? (g.grid + 7) & ~7 : std::max(8, nb8);
+    if (grid != g.launch_grid)
+      b.drop_graphs();
+    g.launch_grid = grid;
+    if (int(E.size()) > g.capacity)
+      {
+        HIP_CHECK(hipStreamSynchronize(b.stream));
+        b.drop_graphs(); // the graphs hold the table's address
+        const int cap = std::max<int>(8, 2 * int(E.size()));
+        g.table.alloc(lbfem_batch::kCountsBytes + size_t(cap) * sizeof(BatchEntry<T>));
+        g.capacity = cap;
+      }
+    std::vector<char> host(lbfem_batch::kCountsBytes + E.size() * sizeof(BatchEntry<T>), 0);
+    const BatchCounts c{int(E.size()), g.n_tiles};
+    std::memcpy(host.data(), &c, sizeof(c));
+    if (!E.empty())
+      std::memcpy(host.data() + lbfem_batch::kCountsBytes, E.data(), E.size() * sizeof(BatchEntry<T>));
+    HIP_CHECK(hipMemcpyAsync(g.table.p, host.data(), host.size(), hipMemcpyHostToDevice, b.stream));
+    HIP_CHECK(hipStreamSynchronize(b.stream)); // the host buffer is released on return
   }
 
-  // the tables at the members' current buffers (parity 0 from here)
+  void
+  write_group(lbfem_batch &b, lbfem_batch::Group &g)
+  {
+    if (g.mixed)
+      write_table<float>(b, g);
+    else
+      write_table<double>(b, g);
+  }
+
   void
   rebuild_tables(lbfem_batch &b)
   {
     HIP_CHECK(hipStreamSynchronize(b.stream));
-    b.drop_graphs();
-    b.parity = 0;
-    for (size_t k = 0; k < b.members.size(); ++k)
-      b.base_cur[k] = b.members[k]->cur;
     for (auto &g : b.groups)
-      {
-        if (g.mixed)
-          write_table<float>(b, g);
-        else
-          write_table<double>(b, g);
-      }
+      write_group(b, g);
   }
 
   int
-  launch_group(lbfem_batch &b, lbfem_batch::Group &g, const int parity, const int variant, const int grid)
+  launch_group(lbfem_batch &b, const lbfem_batch::Group &g, const int parity)
   {
-    const int n = int(g.members.size());
+    if (g.members.empty())
+      return LBFEM_OK;
+    const BatchCounts *counts = reinterpret_cast<const BatchCounts *>(g.table.p);
     if (g.mixed)
-      HIP_CHECK(tile_batch_launch<float>(g.p, g.mode, variant, reinterpret_cast<const BatchEntry<float> *>(g.table.p),
-                                         n, g.n_tiles, parity, grid, b.stream));
+      HIP_CHECK(tile_batch_launch<float>(
+        g.p, g.mode, g.variant,
+        reinterpret_cast<const BatchEntry<float> *>(g.table.p + lbfem_batch::kCountsBytes), counts, parity,
+        g.launch_grid, b.stream));
     else
-      HIP_CHECK(tile_batch_launch<double>(g.p, g.mode, variant,
-                                          reinterpret_cast<const BatchEntry<double> *>(g.table.p), n, g.n_tiles,
-                                          parity, grid, b.stream));
+      HIP_CHECK(tile_batch_launch<double>(
+        g.p, g.mode, g.variant,
+        reinterpret_cast<const BatchEntry<double> *>(g.table.p + lbfem_batch::kCountsBytes), counts, parity,
+        g.launch_grid, b.stream));
     return LBFEM_OK;
   }
 
@@ -1056,7 +1169,7 @@ namespace
   batch_step_eager(lbfem_batch &b, const int parity)
   {
     for (auto &g : b.groups)
-      if (const int st = launch_group(b, g, parity, g.variant, g.grid); st != LBFEM_OK)
+      if (const int st = launch_group(b, g, parity); st != LBFEM_OK)
         return st;
     return LBFEM_OK;
   }
@@ -1110,7 +1223,74 @@ namespace
   sync_members(lbfem_batch &b)
   {
     for (size_t k = 0; k < b.members.size(); ++k)
-      b.members[k]->cur = (b.base_cur[k] + b.parity) & 1;
+      b.members[k]->cur = (b.par0[k] + b.parity) & 1;
+  }
+
+  // what a batch steps: lumped TG2 schemes on tensor meshes, time step set
+  int
+  check_batchable(const lbfem_scheme *s)
+  {
+    if (!s)
+      return fail(LBFEM_EINVAL, "null scheme");
+    if (!(s->set.mass.type == LBFEM_MASS_LUMPED && s->set.streaming == LBFEM_TG2))
+      return fail(LBFEM_EINVAL, "a batch steps schemes of the lumped mass and TG2 streaming (one fused kernel)");
+    if (s->d->general)
+      return fail(LBFEM_EINVAL, "a batch steps tensor-product meshes only (no distort)");
+    if (!(s->dt > 0.))
+      return fail(LBFEM_EINVAL, "set the time step of every scheme before batching");
+    return LBFEM_OK;
+  }
+
+  // a member joins: its buffer parity taken relative to the batch's, its group's table rewritten
+  // (a new group drops the captured graphs)
+  void
+  join(lbfem_batch &b, lbfem_scheme *s)
+  {
+    HIP_CHECK(hipStreamSynchronize(s->stream)); // its own queued work first
+    b.members.push_back(s);
+    b.par0.push_back((s->cur ^ b.parity) & 1);
+    const int mode = s->set.scheme == LBFEM_LEELIN ? kStepLeeLin : kStepBardow;
+    auto      it   = std::find_if(b.groups.begin(), b.groups.end(), [&](const auto &g) {
+      return g.p == s->d->G.p && g.mode == mode && g.mixed == s->mixed;
+    });
+    if (it == b.groups.end())
+      {
+        b.drop_graphs();
+        b.groups.emplace_back();
+        it        = b.groups.end() - 1;
+        it->p     = s->d->G.p;
+        it->mode  = mode;
+        it->mixed = s->mixed;
+        it->grid  = b.shape_grid;
+        it->variant =
+          b.shape_variant >= 0 ? b.shape_variant : size_default_variant(int64_t(s->d->G.np) * s->d->G.np, s->mixed);
+      }
+    it->members.push_back(s);
+  }
+} // namespace
+
+namespace
+{
+  MonitorEntry
+  monitor_entry(lbfem_scheme &s)
+  {
+    MonitorEntry e{};
+    e.G = s.d->G;
+    if (s.mixed)
+      {
+        e.f    = s.g32[s.cur].p;
+        e.fp32 = 1;
+      }
+    else
+      e.f = s.f[s.cur].p;
+    if (!s.u_prev.p)
+      {
+        s.u_prev.alloc(2 * size_t(s.d->n));
+        s.u_prev_set = false;
+      }
+    e.u_prev = s.u_prev.p;
+    e.init   = s.u_prev_set ? 0 : 1;
+    return e;
   }
 } // namespace
 
@@ -1849,6 +2029,26 @@ extern "C"
   }
 
   int
+  lbfem_scheme_velocity_change(lbfem_scheme *s, double *max_change, double *mean_change, int64_t *nonfinite)
+  {
+    return guarded([&]() -> int {
+      if (!s)
+        return fail(LBFEM_EINVAL, "null argument");
+      std::vector<MonitorEntry> E{monitor_entry(*s)};
+      double                    r[kMonitorPartials];
+      s->monitor.run(E, s->stream, r);
+      s->u_prev_set = true;
+      if (max_change)
+        *max_change = r[0];
+      if (mean_change)
+        *mean_change = r[1] / r[2];
+      if (nonfinite)
+        *nonfinite = int64_t(r[3]);
+      return LBFEM_OK;
+    });
+  }
+
+  int
   lbfem_scheme_enable_stage_timers(lbfem_scheme *s, const int enable)
   {
     return guarded([&]() -> int {
@@ -2004,55 +2204,83 @@ extern "C"
   lbfem_batch_create(lbfem_scheme *const *schemes, const int n, lbfem_batch **out)
   {
     return guarded([&]() -> int {
-      if (!schemes || !out || n <= 0)
-        return fail(LBFEM_EINVAL, "lbfem_batch_create: no schemes");
-      auto b = std::make_unique<lbfem_batch>();
+      if (!out || n < 0 || (n > 0 && !schemes))
+        return fail(LBFEM_EINVAL, "lbfem_batch_create: invalid arguments");
       for (int k = 0; k < n; ++k)
         {
-          lbfem_scheme *s = schemes[k];
-          if (!s)
-            return fail(LBFEM_EINVAL, "null scheme");
-          if (!(s->set.mass.type == LBFEM_MASS_LUMPED && s->set.streaming == LBFEM_TG2))
-            return fail(LBFEM_EINVAL, "a batch steps schemes of the lumped mass and TG2 streaming (one fused kernel)");
-          if (s->d->general)
-            return fail(LBFEM_EINVAL, "a batch steps tensor-product meshes only (no distort)");
-          if (!(s->dt > 0.))
-            return fail(LBFEM_EINVAL, "set the time step of every scheme before batching");
+          if (const int st = check_batchable(schemes[k]); st != LBFEM_OK)
+            return st;
           for (int j = 0; j < k; ++j)
-            if (schemes[j] == s)
+            if (schemes[j] == schemes[k])
               return fail(LBFEM_EINVAL, "a scheme appears twice in the batch");
-          HIP_CHECK(hipStreamSynchronize(s->stream));
-          b->members.push_back(s);
         }
-      b->base_cur.resize(n);
+      auto b = std::make_unique<lbfem_batch>();
+      HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       for (int k = 0; k < n; ++k)
-        {
-          const lbfem_scheme &s    = *b->members[k];
-          const int           mode = s.set.scheme == LBFEM_LEELIN ? kStepLeeLin : kStepBardow;
-          auto                it   = std::find_if(b->groups.begin(), b->groups.end(), [&](const auto &g) {
-            return g.p == s.d->G.p && g.mode == mode && g.mixed == s.mixed;
-          });
-          if (it == b->groups.end())
-            {
-              b->groups.emplace_back();
-              it        = b->groups.end() - 1;
-              it->p     = s.d->G.p;
-              it->mode  = mode;
-              it->mixed = s.mixed;
-            }
-          it->members.push_back(k);
-        }
+        join(*b, schemes[k]);
       // default shape: the variant the size-based rule picks for the group's total node count
       for (auto &g : b->groups)
         {
           int64_t nodes = 0;
-          for (const int k : g.members)
-            nodes += int64_t(b->members[k]->d->G.np) * b->members[k]->d->G.np;
+          for (const lbfem_scheme *m : g.members)
+            nodes += int64_t(m->d->G.np) * m->d->G.np;
           g.variant = size_default_variant(nodes, g.mixed);
         }
-      HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       rebuild_tables(*b);
       *out = b.release();
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_add(lbfem_batch *b, lbfem_scheme *s)
+  {
+    return guarded([&]() -> int {
+      if (!b)
+        return fail(LBFEM_EINVAL, "null batch");
+      if (const int st = check_batchable(s); st != LBFEM_OK)
+        return st;
+      if (b->index_of(s) >= 0)
+        return fail(LBFEM_EINVAL, "the scheme is already a member of the batch");
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      join(*b, s);
+      const int mode = s->set.scheme == LBFEM_LEELIN ? kStepLeeLin : kStepBardow;
+      for (auto &g : b->groups)
+        if (g.p == s->d->G.p && g.mode == mode && g.mixed == s->mixed)
+          write_group(*b, g);
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_remove(lbfem_batch *b, lbfem_scheme *s)
+  {
+    return guarded([&]() -> int {
+      if (!b || !s)
+        return fail(LBFEM_EINVAL, "null argument");
+      const int k = b->index_of(s);
+      if (k < 0)
+        return fail(LBFEM_EINVAL, "the scheme is not a member of the batch");
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      sync_members(*b);
+      b->members.erase(b->members.begin() + k);
+      b->par0.erase(b->par0.begin() + k);
+      for (size_t gi = 0; gi < b->groups.size(); ++gi)
+        {
+          auto &g  = b->groups[gi];
+          auto  it = std::find(g.members.begin(), g.members.end(), s);
+          if (it == g.members.end())
+            continue;
+          g.members.erase(it);
+          if (g.members.empty())
+            {
+              b->drop_graphs(); // the graphs launch that group
+              b->groups.erase(b->groups.begin() + gi);
+            }
+          else
+            write_group(*b, g);
+          break;
+        }
       return LBFEM_OK;
     });
   }
@@ -2089,6 +2317,8 @@ extern "C"
     return guarded([&]() -> int {
       if (!b || n_steps < 0)
         return fail(LBFEM_EINVAL, "invalid argument");
+      if (b->members.empty())
+        return LBFEM_OK;
       int left = n_steps;
       if (b->graph_steps > 0 && left >= b->graph_steps)
         {
@@ -2157,14 +2387,18 @@ extern "C"
   lbfem_batch_set_shape(lbfem_batch *b, const int variant, const int grid)
   {
     return guarded([&]() -> int {
-      if (!b || variant < 0 || variant >= kTileVariants || grid < 0)
+      if (!b || variant < -1 || variant >= kTileVariants || grid < 0)
         return fail(LBFEM_EINVAL, "invalid tile variant or grid");
       HIP_CHECK(hipStreamSynchronize(b->stream));
+      b->shape_variant = variant;
+      b->shape_grid    = grid;
       for (auto &g : b->groups)
         {
-          g.variant = variant;
-          g.grid    = grid;
+          if (variant >= 0)
+            g.variant = variant;
+          g.grid = grid;
         }
+      b->drop_graphs();
       rebuild_tables(*b);
       return LBFEM_OK;
     });
@@ -2176,7 +2410,8 @@ extern "C"
     return guarded([&]() -> int {
       if (!b)
         return fail(LBFEM_EINVAL, "null argument");
-      rebuild_tables(*b); // parity 0 at the members' current buffers
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      sync_members(*b);
       hipEvent_t e0, e1;
       HIP_CHECK(hipEventCreate(&e0));
       HIP_CHECK(hipEventCreate(&e1));
@@ -2187,53 +2422,85 @@ extern "C"
           double best = 1e300;
           int    bv = g.variant, bg = g.grid;
           for (int v = 0; v < kTileVariants; ++v)
-            {
-              // tile counts depend on the variant
-              g.variant = v;
-              if (g.mixed)
-                write_table<float>(*b, g);
-              else
-                write_table<double>(*b, g);
-              for (const int grid : grids)
-                {
-                  // launches at a fixed parity read one buffer and write the other: no state changes
-                  for (int k = 0; k < 2; ++k)
-                    if (const int st = launch_group(*b, g, b->parity, v, grid); st != LBFEM_OK)
-                      return st;
-                  float ms = 0.f;
-                  HIP_CHECK(hipEventRecord(e0, b->stream));
-                  for (int k = 0; k < 3; ++k)
-                    if (const int st = launch_group(*b, g, b->parity, v, grid); st != LBFEM_OK)
-                      return st;
-                  HIP_CHECK(hipEventRecord(e1, b->stream));
-                  HIP_CHECK(hipEventSynchronize(e1));
-                  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                  const int n = std::max(5, std::min(100, int(2.0 / std::max(1e-4, double(ms) / 3))));
-                  HIP_CHECK(hipEventRecord(e0, b->stream));
-                  for (int k = 0; k < n; ++k)
-                    if (const int st = launch_group(*b, g, b->parity, v, grid); st != LBFEM_OK)
-                      return st;
-                  HIP_CHECK(hipEventRecord(e1, b->stream));
-                  HIP_CHECK(hipEventSynchronize(e1));
-                  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                  const double t = 1e-3 * ms / n;
-                  if (t < best)
-                    {
-                      best = t;
-                      bv   = v;
-                      bg   = grid;
-                    }
-                }
-            }
+            for (const int grid : grids)
+              {
+                // tile counts depend on the variant; launches at a fixed parity read one buffer and write the
+                // other: no state changes
+                g.variant = v;
+                g.grid    = grid;
+                write_group(*b, g);
+                for (int k = 0; k < 2; ++k)
+                  if (const int st = launch_group(*b, g, b->parity); st != LBFEM_OK)
+                    return st;
+                float ms = 0.f;
+                HIP_CHECK(hipEventRecord(e0, b->stream));
+                for (int k = 0; k < 3; ++k)
+                  if (const int st = launch_group(*b, g, b->parity); st != LBFEM_OK)
+                    return st;
+                HIP_CHECK(hipEventRecord(e1, b->stream));
+                HIP_CHECK(hipEventSynchronize(e1));
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                const int n = std::max(5, std::min(100, int(2.0 / std::max(1e-4, double(ms) / 3))));
+                HIP_CHECK(hipEventRecord(e0, b->stream));
+                for (int k = 0; k < n; ++k)
+                  if (const int st = launch_group(*b, g, b->parity); st != LBFEM_OK)
+                    return st;
+                HIP_CHECK(hipEventRecord(e1, b->stream));
+                HIP_CHECK(hipEventSynchronize(e1));
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                const double t = 1e-3 * ms / n;
+                if (t < best)
+                  {
+                    best = t;
+                    bv   = v;
+                    bg   = grid;
+                  }
+              }
           g.variant = bv;
           g.grid    = bg;
           total += best;
         }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
+      b->drop_graphs();
       rebuild_tables(*b); // the chosen shapes
       if (seconds)
         *seconds = total;
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_velocity_change(lbfem_batch *b, const int32_t *mask, double *max_change, double *mean_change,
+                              int64_t *nonfinite)
+  {
+    return guarded([&]() -> int {
+      if (!b)
+        return fail(LBFEM_EINVAL, "null argument");
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      sync_members(*b);
+      std::vector<MonitorEntry> E;
+      std::vector<int>          which;
+      for (size_t k = 0; k < b->members.size(); ++k)
+        if (!mask || mask[k])
+          {
+            E.push_back(monitor_entry(*b->members[k]));
+            which.push_back(int(k));
+          }
+      std::vector<double> r(E.size() * kMonitorPartials);
+      b->monitor.run(E, b->stream, r.data());
+      for (size_t q = 0; q < which.size(); ++q)
+        {
+          const int     k = which[q];
+          const double *x = r.data() + q * kMonitorPartials;
+          b->members[k]->u_prev_set = true;
+          if (max_change)
+            max_change[k] = x[0];
+          if (mean_change)
+            mean_change[k] = x[1] / x[2];
+          if (nonfinite)
+            nonfinite[k] = int64_t(x[3]);
+        }
       return LBFEM_OK;
     });
   }

@@ -99,12 +99,18 @@ namespace lbfem_amd
     int        tile_begin; // first tile in the batch's tile list
   };
 
+  // the live member and tile counts of a batch group's table (read by the kernel at run time)
+  struct BatchCounts
+  {
+    int n_inst, n_tiles;
+  };
+
   template <typename T, int p>
-  hipError_t tile_batch_launch_degree(int mode, int variant, const BatchEntry<T> *table, int n_inst, int n_tiles,
+  hipError_t tile_batch_launch_degree(int mode, int variant, const BatchEntry<T> *table, const BatchCounts *counts,
                                       int parity, int grid, hipStream_t stream);
 #define LBFEM_BATCH_EXTERN(T, P)                                                                                   \
-  extern template hipError_t tile_batch_launch_degree<T, P>(int, int, const BatchEntry<T> *, int, int, int, int,   \
-                                                            hipStream_t);
+  extern template hipError_t tile_batch_launch_degree<T, P>(int, int, const BatchEntry<T> *, const BatchCounts *,  \
+                                                            int, int, hipStream_t);
   LBFEM_BATCH_EXTERN(double, 1)
   LBFEM_BATCH_EXTERN(double, 2)
   LBFEM_BATCH_EXTERN(double, 3)
@@ -113,19 +119,20 @@ namespace lbfem_amd
   LBFEM_BATCH_EXTERN(float, 3)
 #undef LBFEM_BATCH_EXTERN
 
+  // grid: blocks of the launch, a multiple of 8
   template <typename T>
   inline hipError_t
-  tile_batch_launch(const int p, const int mode, const int variant, const BatchEntry<T> *table, const int n_inst,
-                    const int n_tiles, const int parity, const int grid, hipStream_t stream)
+  tile_batch_launch(const int p, const int mode, const int variant, const BatchEntry<T> *table,
+                    const BatchCounts *counts, const int parity, const int grid, hipStream_t stream)
   {
     switch (p)
       {
         case 1:
-          return tile_batch_launch_degree<T, 1>(mode, variant, table, n_inst, n_tiles, parity, grid, stream);
+          return tile_batch_launch_degree<T, 1>(mode, variant, table, counts, parity, grid, stream);
         case 2:
-          return tile_batch_launch_degree<T, 2>(mode, variant, table, n_inst, n_tiles, parity, grid, stream);
+          return tile_batch_launch_degree<T, 2>(mode, variant, table, counts, parity, grid, stream);
         case 3:
-          return tile_batch_launch_degree<T, 3>(mode, variant, table, n_inst, n_tiles, parity, grid, stream);
+          return tile_batch_launch_degree<T, 3>(mode, variant, table, counts, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }
@@ -249,6 +256,32 @@ namespace lbfem_amd
   // Jacobi diagonal 1 / (Kx My + Mx Ky)_ii, with b = 0 and 1 on the walls (io.h:55-85)
   hipError_t psi_setup_launch(const Geo &G, const double *u, const double *v, double *b, double *dinv,
                               hipStream_t stream);
+
+  // ---- the velocity-change monitor of the steady-state tests (cgdbe.cc:385-393) on the device: per member, over
+  // its non-slave nodes, max |u - u_prev|, sum of w |u - u_prev|, sum of w (w the lumped mass) and the count of
+  // nodes whose moments are not finite; u_prev updated. The members of one launch are a table; member k owns
+  // blocks [block_begin, block_begin + nblocks) of the launch, so every member's sums keep a fixed order.
+  struct MonitorEntry
+  {
+    Geo         G;
+    const void *f;           // the current populations: double, or float deviations from the rest state
+    double     *u_prev;      // 2 per node
+    int         fp32;        // 1: float deviations
+    int         init;        // 1: record u_prev only (the results are then +inf)
+    int         block_begin; // first block of the member
+    int         nblocks;
+  };
+  constexpr int kMonitorPartials = 4;
+  // blocks a member of n nodes gets (the same at every call, so its sums are reproducible)
+  inline int
+  monitor_blocks(const long long n)
+  {
+    const long long b = (n + 2047) / 2048;
+    return int(b < 1 ? 1 : b > 256 ? 256 : b);
+  }
+  // results: kMonitorPartials doubles per member (max, sum w |du|, sum w, nonfinite nodes)
+  hipError_t monitor_launch(const MonitorEntry *table, int n_members, int total_blocks, double *partial,
+                            double *results, hipStream_t stream);
 
   // diagnostics sums over non-slave nodes (cgdbe.cc:353-368)
   hipError_t stream_copy_launch(void *dst, const void *src, size_t bytes, hipStream_t s);

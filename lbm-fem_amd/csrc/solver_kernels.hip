@@ -682,6 +682,129 @@ namespace lbfem_amd
         sums[k] = acc[k];
   }
 
+  // ------------------------------------------------------------------ velocity-change monitor
+  __global__ void __launch_bounds__(kThreads)
+  k_monitor(const MonitorEntry *__restrict__ E, const int n_members, double *__restrict__ partial)
+  {
+    __shared__ double red[3][kThreads / 64];
+    __shared__ double mx[kThreads / 64];
+    const int         b  = blockIdx.x;
+    int               lo = 0, hi = n_members - 1; // the member: last block_begin <= b
+    while (lo < hi)
+      {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E[mid].block_begin <= b)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+    const MonitorEntry &e = E[lo];
+    const Geo          &G = e.G;
+    const int64_t       n = int64_t(G.np) * G.np;
+    double              dmax = 0., s[3] = {0., 0., 0.};
+    for (int64_t k = int64_t(b - e.block_begin) * kThreads + threadIdx.x; k < n; k += int64_t(e.nblocks) * kThreads)
+      {
+        const int i = int(k % G.np), j = int(k / G.np);
+        if (is_slave(G, i, j))
+          continue;
+        double rho, ux, uy;
+        if (e.fp32)
+          {
+            const float *f = static_cast<const float *>(e.f);
+            float        v[Q];
+#pragma unroll
+            for (int a = 0; a < Q; ++a)
+              v[a] = f[a * G.stride + k];
+            double drho;
+            moments_dev(v, drho, rho, ux, uy);
+          }
+        else
+          {
+            const double *f = static_cast<const double *>(e.f);
+            double        v[Q];
+#pragma unroll
+            for (int a = 0; a < Q; ++a)
+              v[a] = f[a * G.stride + k];
+            moments(v, rho, ux, uy);
+          }
+        if (!(isfinite(rho) && isfinite(ux) && isfinite(uy)))
+          {
+            s[2] += 1.;
+            continue;
+          }
+        if (!e.init)
+          {
+            const double w  = G.gen.ml ? G.gen.ml[k] : G.ax.ml[i] * G.ay.ml[j];
+            const double du = hypot(ux - e.u_prev[2 * k], uy - e.u_prev[2 * k + 1]);
+            dmax            = fmax(dmax, du);
+            s[0] += w * du;
+            s[1] += w;
+          }
+        e.u_prev[2 * k]     = ux;
+        e.u_prev[2 * k + 1] = uy;
+      }
+    // the maximum (order-free), then the sums in a fixed order
+    for (int off = 32; off > 0; off >>= 1)
+      dmax = fmax(dmax, __shfl_down(dmax, off, 64));
+    if ((threadIdx.x & 63) == 0)
+      mx[threadIdx.x >> 6] = dmax;
+    block_sum<3>(s, red);
+    if (threadIdx.x == 0)
+      {
+        double m = 0.;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w)
+          m = fmax(m, mx[w]);
+        partial[size_t(b) * kMonitorPartials + 0] = m;
+        partial[size_t(b) * kMonitorPartials + 1] = s[0];
+        partial[size_t(b) * kMonitorPartials + 2] = s[1];
+        partial[size_t(b) * kMonitorPartials + 3] = s[2];
+      }
+  }
+
+  // one block per member: its blocks' partials in block order
+  __global__ void __launch_bounds__(64)
+  k_monitor_finalize(const MonitorEntry *__restrict__ E, const double *__restrict__ partial,
+                     double *__restrict__ results)
+  {
+    const MonitorEntry &e = E[blockIdx.x];
+    double              m = 0., s[3] = {0., 0., 0.};
+    for (int b = threadIdx.x; b < e.nblocks; b += 64)
+      {
+        const double *q = partial + size_t(e.block_begin + b) * kMonitorPartials;
+        m               = fmax(m, q[0]);
+        s[0] += q[1];
+        s[1] += q[2];
+        s[2] += q[3];
+      }
+    for (int off = 32; off > 0; off >>= 1)
+      {
+        m = fmax(m, __shfl_down(m, off, 64));
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          s[k] += __shfl_down(s[k], off, 64);
+      }
+    if (threadIdx.x == 0)
+      {
+        double *r = results + size_t(blockIdx.x) * kMonitorPartials;
+        r[0]      = e.init ? HUGE_VAL : m;
+        r[1]      = e.init ? HUGE_VAL : s[0];
+        r[2]      = s[1];
+        r[3]      = s[2];
+      }
+  }
+
+  hipError_t
+  monitor_launch(const MonitorEntry *table, const int n_members, const int total_blocks, double *partial,
+                 double *results, hipStream_t s)
+  {
+    if (n_members <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL(k_monitor, dim3(total_blocks), dim3(kThreads), 0, s, table, n_members, partial);
+    hipLaunchKernelGGL(k_monitor_finalize, dim3(n_members), dim3(64), 0, s, table, partial, results);
+    return hipGetLastError();
+  }
+
   // ------------------------------------------------------------------ launchers
   static int
   nodal_blocks(const Geo &G)

@@ -7,6 +7,6 @@ namespace lbfem_amd
 {
   template hipError_t tile_launch_degree<float, 1>(int, int, const Geo &, const TileParams &, const float *,
                                                      const float *, float *, float *, hipStream_t);
-  template hipError_t tile_batch_launch_degree<float, 1>(int, int, const BatchEntry<float> *, int, int, int, int,
+  template hipError_t tile_batch_launch_degree<float, 1>(int, int, const BatchEntry<float> *, const BatchCounts *, int, int,
                                                            hipStream_t);
 } // namespace lbfem_amd

@@ -7,6 +7,6 @@ namespace lbfem_amd
 {
   template hipError_t tile_launch_degree<double, 1>(int, int, const Geo &, const TileParams &, const double *,
                                                      const double *, double *, double *, hipStream_t);
-  template hipError_t tile_batch_launch_degree<double, 1>(int, int, const BatchEntry<double> *, int, int, int, int,
+  template hipError_t tile_batch_launch_degree<double, 1>(int, int, const BatchEntry<double> *, const BatchCounts *, int, int,
                                                            hipStream_t);
 } // namespace lbfem_amd

@@ -646,13 +646,16 @@ namespace lbfem_amd
   // 8), so that a grid of a few blocks per CU is persistent and a grid of one
   // block per tile is not. Each instance flips its own population buffers: the
   // step reads f[(par + parity) & 1] and writes the other.
+  // The member and tile counts are read from the table's header at run time, so that a
+  // captured launch keeps working after members join or leave the batch.
   template <class C>
   __global__ void __launch_bounds__(C::NT)
-  k_tile_batch(const BatchEntry<typename C::T> *__restrict__ E, const int n_inst, const int n_tiles,
+  k_tile_batch(const BatchEntry<typename C::T> *__restrict__ E, const BatchCounts *__restrict__ counts,
                const int parity)
   {
     __shared__ tile::Smem<C> sm;
-    const int nb8 = (n_tiles + 7) & ~7;
+    const int n_inst = counts->n_inst, n_tiles = counts->n_tiles;
+    const int nb8    = (n_tiles + 7) & ~7;
     for (int b = blockIdx.x; b < nb8; b += gridDim.x)
       {
         const int t = (b & 7) * (nb8 >> 3) + (b >> 3);
@@ -674,14 +677,13 @@ namespace lbfem_amd
       }
   }
 
+  // grid: the blocks of the launch (a multiple of 8; blocks stride over the tiles)
   template <class C>
   hipError_t
-  launch_batch(const BatchEntry<typename C::T> *table, const int n_inst, const int n_tiles, const int parity,
-               const int grid, hipStream_t stream)
+  launch_batch(const BatchEntry<typename C::T> *table, const BatchCounts *counts, const int parity, const int grid,
+               hipStream_t stream)
   {
-    const int nb8 = (n_tiles + 7) & ~7;
-    const int g   = grid > 0 ? std::min(nb8, (grid + 7) & ~7) : nb8;
-    hipLaunchKernelGGL((k_tile_batch<C>), dim3(g), dim3(C::NT), 0, stream, table, n_inst, n_tiles, parity);
+    hipLaunchKernelGGL((k_tile_batch<C>), dim3(grid), dim3(C::NT), 0, stream, table, counts, parity);
     return hipGetLastError();
   }
 
@@ -733,27 +735,27 @@ namespace lbfem_amd
 
   template <typename T, int p, int MODE>
   hipError_t
-  launch_batch_variant(const int variant, const BatchEntry<T> *table, const int n_inst, const int n_tiles,
-                       const int parity, const int grid, hipStream_t stream)
+  launch_batch_variant(const int variant, const BatchEntry<T> *table, const BatchCounts *counts, const int parity,
+                       const int grid, hipStream_t stream)
   {
     switch (variant)
       {
         case kTileWide:
-          return launch_batch<tile::Cfg<T, p, 32, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 32, 8, 1, MODE>>(table, counts, parity, grid, stream);
         case kTileTall:
-          return launch_batch<tile::Cfg<T, p, 32, 16, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 32, 16, 1, MODE>>(table, counts, parity, grid, stream);
         case kTileSplit2:
-          return launch_batch<tile::Cfg<T, p, 32, 8, 2, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 32, 8, 2, MODE>>(table, counts, parity, grid, stream);
         case kTileSplit4:
-          return launch_batch<tile::Cfg<T, p, 32, 8, 4, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 32, 8, 4, MODE>>(table, counts, parity, grid, stream);
         case kTileSmall4:
-          return launch_batch<tile::Cfg<T, p, 16, 8, 4, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 16, 8, 4, MODE>>(table, counts, parity, grid, stream);
         case kTileWide40:
-          return launch_batch<tile::Cfg<T, p, 40, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 40, 8, 1, MODE>>(table, counts, parity, grid, stream);
         case kTileWide48:
-          return launch_batch<tile::Cfg<T, p, 48, 8, 1, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 48, 8, 1, MODE>>(table, counts, parity, grid, stream);
         case kTileSplit2W40:
-          return launch_batch<tile::Cfg<T, p, 40, 8, 2, MODE>>(table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch<tile::Cfg<T, p, 40, 8, 2, MODE>>(table, counts, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }
@@ -762,15 +764,15 @@ namespace lbfem_amd
   // the fused lumped step of many instances of one degree and form
   template <typename T, int p>
   hipError_t
-  tile_batch_launch_degree(const int mode, const int variant, const BatchEntry<T> *table, const int n_inst,
-                           const int n_tiles, const int parity, const int grid, hipStream_t stream)
+  tile_batch_launch_degree(const int mode, const int variant, const BatchEntry<T> *table, const BatchCounts *counts,
+                           const int parity, const int grid, hipStream_t stream)
   {
     switch (mode)
       {
         case kStepBardow:
-          return launch_batch_variant<T, p, kStepBardow>(variant, table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch_variant<T, p, kStepBardow>(variant, table, counts, parity, grid, stream);
         case kStepLeeLin:
-          return launch_batch_variant<T, p, kStepLeeLin>(variant, table, n_inst, n_tiles, parity, grid, stream);
+          return launch_batch_variant<T, p, kStepLeeLin>(variant, table, counts, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }

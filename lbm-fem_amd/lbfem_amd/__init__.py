@@ -111,6 +111,10 @@ def lib():
             "lbfem_stream_function": (i, [C.POINTER(MeshDesc), p, p, p, C.POINTER(C.c_int)]),
             "lbfem_disc_stream_function": (i, [_vp, _vp, _vp, _vp, C.POINTER(C.c_int), _vp]),
             "lbfem_batch_create": (i, [C.POINTER(_vp), i, C.POINTER(_vp)]),
+            "lbfem_batch_add": (i, [_vp, _vp]),
+            "lbfem_batch_remove": (i, [_vp, _vp]),
+            "lbfem_batch_velocity_change": (i, [_vp, i32, p, p, i64]),
+            "lbfem_scheme_velocity_change": (i, [_vp, p, p, i64]),
             "lbfem_batch_destroy": (i, [_vp]),
             "lbfem_batch_set_graph_steps": (i, [_vp, i]),
             "lbfem_batch_step": (i, [_vp, i]),
@@ -362,6 +366,13 @@ class Scheme:
         check(lib().lbfem_scheme_last_cg_iterations(self.h, v.ctypes.data_as(C.POINTER(C.c_int32))))
         return v.reshape(2, 8)
 
+    def velocity_change(self):
+        """(max |u - u_prev|, lumped-mass mean |u - u_prev|, non-finite nodes) against the velocity at the
+        previous call, computed on the device (the first call records the velocity and returns inf)."""
+        mx, mean, bad = C.c_double(0), C.c_double(0), C.c_int64(0)
+        check(lib().lbfem_scheme_velocity_change(self.h, C.byref(mx), C.byref(mean), C.byref(bad)))
+        return mx.value, mean.value, bad.value
+
     def enable_stage_timers(self, on=True):
         check(lib().lbfem_scheme_enable_stage_timers(self.h, int(on)))
 
@@ -435,9 +446,9 @@ class Batch:
     per step, steps replayed from captured HIP graphs. Members are read after synchronize(); the batch keeps
     references to them, and is destroyed (close()) before they are."""
 
-    def __init__(self, schemes, graph_steps=8):
+    def __init__(self, schemes=(), graph_steps=8):
         self.schemes = list(schemes)
-        arr = (_vp * len(self.schemes))(*[s.h.value for s in self.schemes])
+        arr = (_vp * max(1, len(self.schemes)))(*[s.h.value for s in self.schemes])
         h = _vp()
         check(lib().lbfem_batch_create(arr, len(self.schemes), C.byref(h)))
         self.h = h
@@ -459,7 +470,33 @@ class Batch:
     def synchronize(self):
         check(lib().lbfem_batch_synchronize(self.h))
 
-    def set_shape(self, variant, grid=0):
+    def add(self, scheme):
+        """lbfem_batch_add: the scheme joins the running batch from its current populations."""
+        check(lib().lbfem_batch_add(self.h, scheme.h))
+        self.schemes.append(scheme)
+
+    def remove(self, scheme):
+        """lbfem_batch_remove: the scheme leaves the batch (its populations are then readable)."""
+        check(lib().lbfem_batch_remove(self.h, scheme.h))
+        self.schemes = [s for s in self.schemes if s is not scheme]
+
+    def velocity_change(self, members=None):
+        """Per member (in member order): max |u - u_prev|, mean |u - u_prev|, non-finite nodes, computed on the
+        device in one launch; members (optional): the schemes to measure (the others are left untouched, their
+        entries NaN / -1)."""
+        n = len(self.schemes)
+        mx, mean = np.full(n, np.nan), np.full(n, np.nan)
+        bad = np.full(n, -1, dtype=np.int64)
+        mask = None
+        if members is not None:
+            ids = {id(s) for s in members}
+            mask = np.array([1 if id(s) in ids else 0 for s in self.schemes], dtype=np.int32)
+        check(lib().lbfem_batch_velocity_change(
+            self.h, None if mask is None else mask.ctypes.data_as(C.POINTER(C.c_int32)), _p(mx), _p(mean),
+            bad.ctypes.data_as(C.POINTER(C.c_int64))))
+        return mx, mean, bad
+
+    def set_shape(self, variant=-1, grid=0):
         check(lib().lbfem_batch_set_shape(self.h, int(variant), int(grid)))
 
     def autotune(self):

@@ -52,10 +52,12 @@ def nodes(inst):
     return (inst["p"] * inst["N"] + 1) ** 2
 
 
-def request_stream(base_seed, n, rate, sizes=(32, 64, 128, 256), steps=(2000, 20000), re=(100.0, 5000.0), q2=0.25):
+def request_stream(base_seed, n, rate, sizes=(32, 64, 128, 256), steps=(2000, 20000), re=(100.0, 5000.0), q2=0.25,
+                   steady=0.5, tol=1e-6):
     """n served requests: Poisson arrivals at `rate` per second; cavity or Couette; N from `sizes`, Q2 with
-    probability q2, else Q1; Re log-uniform in re; a fixed number of steps, uniform in `steps` (multiples
-    of 100)."""
+    probability q2, else Q1; Re log-uniform in re; a number of steps uniform in `steps` (multiples of 100),
+    which is the request's length, or, with probability `steady`, the cap of a steady-state solve to
+    tolerance `tol` (mode "steady")."""
     r = Stream(splitmix64(base_seed))
     t = 0.0
     out = []
@@ -67,8 +69,12 @@ def request_stream(base_seed, n, rate, sizes=(32, 64, 128, 256), steps=(2000, 20
         Re = re[0] * (re[1] / re[0]) ** r.uniform()
         U = 0.02 + 0.08 * r.uniform()
         k = int(steps[0] + (steps[1] - steps[0]) * r.uniform()) // 100 * 100
-        out.append({"id": i, "arrival": t, "case": case, "N": N, "p": p, "Re": Re, "U": U, "mach": U / CS,
-                    "steps": max(100, k)})
+        req = {"id": i, "arrival": t, "case": case, "N": N, "p": p, "Re": Re, "U": U, "mach": U / CS,
+               "steps": max(100, k), "mode": "steps"}
+        if r.uniform() < steady:
+            req["mode"] = "steady"
+            req["tol"] = tol
+        out.append(req)
     return out
 
 

@@ -242,17 +242,24 @@ def test_seeded_96_instance_batch(precision):
 # ------------------------------------------------------------------ serving
 @pytest.mark.timeout(600)
 def test_served_stream_results_equal_alone():
+    """One arrival rate of the served stream: half the requests to the steady tolerance 1e-6 (capped at their
+    step draw), half for their steps; every result, steps and status equal to the request run alone under
+    the same retirement rules (diverged requests: the same status)."""
     rate = 50.0
     reqs = workloads.request_stream(20261203 + 50, 24, rate=rate, sizes=(32, 64, 128, 256, 512, 1024))
     t0 = time.perf_counter()
     srv = serving.Server(A, keep_results=True)
     done, summary = srv.run(reqs)
     log(f"served {summary['requests']} requests at {rate:g}/s: p50 {summary['tts_p50_s']:.3f} s, "
-        f"p99 {summary['tts_p99_s']:.3f} s, {summary['MLUPS']:.0f} MLUPS ({time.perf_counter() - t0:.1f} s)")
+        f"p99 {summary['tts_p99_s']:.3f} s, {summary['MLUPS']:.0f} MLUPS, {summary['statuses']} "
+        f"({time.perf_counter() - t0:.1f} s)")
     assert summary["requests"] == len(reqs)
     by_id = {d["id"]: d for d in done}
     assert any(a["admit"] < b["admit"] < a["done"] for a in done for b in done)  # overlap on the device
     for r in reqs:
-        s = workloads.make_scheme(A, r, A.MIXED)
-        s.step(r["steps"])
-        assert bitwise(by_id[r["id"]]["f"], s.f), r
+        s, steps, status = serving.run_alone(A, r, A.MIXED)
+        d = by_id[r["id"]]
+        assert d["status"] == status, (r, d["status"], status)
+        if status != "diverged":
+            assert d["steps"] == steps, (r, d["steps"], steps)
+            assert bitwise(d["f"], s.f), r
