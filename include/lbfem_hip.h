@@ -251,14 +251,24 @@ int lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const do
 int lbfem_scheme_velocity_change(lbfem_scheme *scheme, double *max_change, double *mean_change, int64_t *nonfinite);
 
 /* ---------------------------------------------------------- batches
- * No counterpart in the reference (one mesh per process there): many lumped
- * TG2 schemes (any degree, scheme, FP64 or mixed, sizes and flows) stepped
- * together. Members are grouped by (degree, scheme, precision); a step is one
- * launch per group of the tile kernel over the group's concatenated tile list,
- * and steps are replayed from captured HIP graphs (graph_steps per graph). Each
- * member's populations are bitwise those it gets stepped alone. While batched,
- * members are stepped only through the batch and read after
- * lbfem_batch_synchronize; destroy the batch before its members. */
+ * No counterpart in the reference (one mesh per process there): many schemes
+ * (any degree, scheme, sizes and flows) stepped together.
+ *   - Lumped TG2 members (FP64 or mixed) are grouped by (degree, scheme,
+ *     precision); a step is one launch per group of the tile kernel over the
+ *     group's concatenated tile list, replayed from captured HIP graphs
+ *     (graph_steps per graph).
+ *   - Consistent-mass members (FP64, TG2, TG3 or split TG3) are grouped by
+ *     (degree, scheme, streaming); a step is the single-mesh step's stages
+ *     (advection, extrapolated start, the eight CG solves of each member,
+ *     nodal update), each stage one launch over all members of the group, with
+ *     one host check of all members' stopping states per CG iteration checked,
+ *     on a second stream beside the lumped groups.
+ * Each member's populations and CG iteration counts are bitwise those it gets
+ * stepped alone. While batched, members are stepped only through the batch and
+ * read after lbfem_batch_synchronize; destroy the batch before its members.
+ * lbfem_batch_step returns LBFEM_ENOCONV when a member's solve reached its
+ * iteration cap (the step of every member is complete; lbfem_batch_member_status
+ * names the member). */
 typedef struct lbfem_batch lbfem_batch;
 /* n may be 0 (members join later with lbfem_batch_add) */
 int lbfem_batch_create(lbfem_scheme *const *schemes, int n, lbfem_batch **batch);
@@ -281,6 +291,9 @@ int lbfem_batch_destroy(lbfem_batch *batch);
 int lbfem_batch_set_graph_steps(lbfem_batch *batch, int steps);
 int lbfem_batch_step(lbfem_batch *batch, int n_steps);
 int lbfem_batch_synchronize(lbfem_batch *batch);
+/* per member (member order): LBFEM_OK, or LBFEM_ENOCONV once its CG reached its
+ * iteration cap in some step (SolverControl::NoConvergence, mass.h:38-43) */
+int lbfem_batch_member_status(const lbfem_batch *batch, int32_t *status);
 int lbfem_batch_get_stream(const lbfem_batch *batch, void **stream);
 /* groups, nodes (slaves included) and tiles of the batch */
 int lbfem_batch_info(const lbfem_batch *batch, int *n_groups, int64_t *n_nodes, int64_t *n_tiles);

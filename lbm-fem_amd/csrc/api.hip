@@ -1023,10 +1023,45 @@ struct lbfem_batch
     int64_t                     n_nodes = 0;
     DevBuf<char>                table; // BatchCounts (kCountsBytes), then BatchEntry<double or float>[capacity]
   };
-  static constexpr size_t     kCountsBytes = 256;
+  static constexpr size_t kCountsBytes = 256;
+
+  // Members with the consistent mass (FP64, tensor meshes) of one (degree, scheme,
+  // streaming): each step is the single-mesh step's sequence of stages (advection,
+  // extrapolated start, CG start, direction and update per iteration, nodal update), each
+  // stage one launch over all members from per-sweep device tables, with one host check
+  // of all members' stopping states per CG iteration checked. A member's buffers flip
+  // with the group's own parity (once per Bardow step; Lee-Lin updates in place).
+  struct CgGroup
+  {
+    int                         p = 1, scheme = 0, streaming = 0, n_sweeps = 1;
+    std::vector<lbfem_scheme *> members;
+    std::vector<int>            par0; // each member's buffer read at group parity 0
+    std::vector<char>           failed; // NoConvergence in some step
+    int                         gparity = 0, last_its = 1, variant = kTileWide;
+    DevBuf<char>                adv[2];      // BatchCounts (kCountsBytes), then BatchEntry<double>[] per sweep
+    DevBuf<CgMember>            cgt[2];      // per sweep
+    DevBuf<NodalMember>         nod[2];      // per sweep
+    DevBuf<CgState>             states;      // members x NM
+    CgState                    *host_states = nullptr;
+    int                         adv_tiles[2] = {0, 0}, op_blocks[2] = {0, 0}, upd_blocks[2] = {0, 0};
+    int                         nod_blocks = 0;
+    int64_t                     n_nodes = 0;
+
+    CgGroup() = default;
+    CgGroup(const CgGroup &) = delete;
+    CgGroup &operator=(const CgGroup &) = delete;
+    ~CgGroup()
+    {
+      if (host_states)
+        (void)hipHostFree(host_states);
+    }
+  };
+
   std::vector<lbfem_scheme *> members;
-  std::vector<int>            par0; // each member's buffer read at batch parity 0
+  std::vector<int>            par0; // each member's buffer read at batch parity 0 (lumped members)
   std::vector<Group>          groups;
+  std::vector<std::unique_ptr<CgGroup>> cg_groups;
+  hipStream_t                 cg_stream = nullptr; // the CG groups' work, beside the lumped groups'
   hipStream_t                 stream      = nullptr;
   int                         parity      = 0;
   int                         graph_steps = 8; // steps per captured graph, 0: no graphs
@@ -1058,11 +1093,22 @@ struct lbfem_batch
     return -1;
   }
 
+  CgGroup *
+  cg_group_of(const lbfem_scheme *s) const
+  {
+    for (const auto &g : cg_groups)
+      if (std::find(g->members.begin(), g->members.end(), s) != g->members.end())
+        return g.get();
+    return nullptr;
+  }
+
   ~lbfem_batch()
   {
     drop_graphs();
     if (stream)
       (void)hipStreamDestroy(stream);
+    if (cg_stream)
+      (void)hipStreamDestroy(cg_stream);
   }
 };
 
@@ -1174,6 +1220,227 @@ namespace
     return LBFEM_OK;
   }
 
+  // The device tables of a CG group for its current members (on its stream, synchronized):
+  // per sweep, the advection entries (the single-mesh tile kernel's, any variant gives the
+  // same bits), the CG members (each with its single-mesh operator tiles, partial rows,
+  // update grid and folded alpha reduction, its states in the group's array) and the nodal
+  // members.
+  void
+  write_cg_group(lbfem_batch &b, lbfem_batch::CgGroup &g)
+  {
+    HIP_CHECK(hipStreamSynchronize(b.cg_stream));
+    const int n = int(g.members.size());
+    if (g.states.n < size_t(n) * NM || !g.host_states)
+      {
+        const size_t cap = size_t(std::max(8, 2 * n)) * NM;
+        g.states.alloc(cap);
+        if (g.host_states)
+          (void)hipHostFree(g.host_states);
+        g.host_states = nullptr;
+        HIP_CHECK(hipHostMalloc(&g.host_states, cap * sizeof(CgState)));
+      }
+    g.n_nodes = 0;
+    for (const lbfem_scheme *s : g.members)
+      g.n_nodes += int64_t(s->d->G.np) * s->d->G.np;
+    g.variant = size_default_variant(g.n_nodes, false);
+    int tx, ty;
+    tile_dims(g.variant, tx, ty);
+    const bool ll = g.scheme == LBFEM_LEELIN;
+    for (int sw = 0; sw < g.n_sweeps; ++sw)
+      {
+        std::vector<BatchEntry<double>> E(n);
+        std::vector<CgMember>           M(n);
+        std::vector<NodalMember>        N(n);
+        int                             tiles = 0, op_blocks = 0, upd_blocks = 0, nod_blocks = 0;
+        for (int k = 0; k < n; ++k)
+          {
+            lbfem_scheme &s = *g.members[k];
+            const Geo    &G = s.d->G;
+            const auto   &e = s.sweeps[sw];
+            unsigned      mask = 0;
+            for (int a = 0; a < NM; ++a)
+              if (moves(e, a))
+                mask |= 1u << a;
+            // advection (streaming.h:66-72), the collision fused into Bardow's first sweep
+            E[k].G          = G;
+            E[k].S          = tile_params(s, e, ll);
+            E[k].S.collide  = (!ll && sw == 0) ? 1 : 0;
+            E[k].S.tiles_x  = (G.P + 1 + tx - 1) / tx;
+            E[k].S.tiles_y  = (G.P + 1 + ty - 1) / ty;
+            E[k].f[0]       = s.f[0].p;
+            E[k].f[1]       = s.f[1].p;
+            E[k].rhs        = s.rhs.p;
+            E[k].par        = g.par0[k];
+            E[k].tile_begin = tiles;
+            tiles += E[k].S.tiles_x * E[k].S.tiles_y;
+            // the solves (mass.h:204-211), as increment() sets them up for the scheme alone
+            CgWork W = cg_work(*s.d, g.states.p + size_t(k) * NM, s.partial.p, nullptr, s.set.mass);
+            W.nv     = 0;
+            for (int a = 0; a < NM; ++a)
+              if (mask & (1u << a))
+                {
+                  W.block[W.nv]    = a;
+                  W.vec[W.nv].e[0] = e[2 * (a + 1)];
+                  W.vec[W.nv].e[1] = e[2 * (a + 1) + 1];
+                  W.vec[W.nv].coef = s.tg3 * s.dt * s.dt;
+                  ++W.nv;
+                }
+            const int rows = cg_direction_rows(G);
+            W.alpha_rows   = int64_t(rows) * W.nv <= 4096 && rows + 1024 <= W.max_blocks ? rows : 0;
+            M[k].G         = G;
+            M[k].W         = W;
+            M[k].b         = s.rhs.p;
+            M[k].x         = s.x[sw].p;
+            M[k].r         = s.r.p;
+            M[k].p[0]      = s.pb[0].p;
+            M[k].p[1]      = s.pb[1].p;
+            M[k].q         = s.q.p;
+            M[k].tiles_x   = (G.P + 1 + 31) / 32;
+            M[k].tiles_y   = (G.P + 1 + 15) / 16;
+            M[k].nt        = M[k].tiles_x * M[k].tiles_y;
+            M[k].op_begin  = op_blocks;
+            op_blocks += (M[k].nt + 7) & ~7;
+            M[k].upd_blocks = cg_update_blocks(G, W.max_blocks, W.alpha_rows);
+            M[k].upd_begin  = upd_blocks;
+            upd_blocks += M[k].upd_blocks;
+            // the nodal stages
+            N[k].G           = G;
+            N[k].f[0]        = s.f[0].p;
+            N[k].f[1]        = s.f[1].p;
+            N[k].x           = s.x[sw].p;
+            N[k].x_prev      = s.x[1].p;
+            N[k].par         = g.par0[k];
+            N[k].mask        = mask;
+            N[k].theta       = s.dt / s.lambda;
+            N[k].block_begin = nod_blocks;
+            N[k].nblocks     = nodal_batch_blocks(G);
+            nod_blocks += N[k].nblocks;
+          }
+        g.adv_tiles[sw]  = tiles;
+        g.op_blocks[sw]  = op_blocks;
+        g.upd_blocks[sw] = upd_blocks;
+        g.nod_blocks     = nod_blocks;
+        std::vector<char> host(lbfem_batch::kCountsBytes + E.size() * sizeof(BatchEntry<double>), 0);
+        const BatchCounts c{n, tiles};
+        std::memcpy(host.data(), &c, sizeof(c));
+        if (n)
+          std::memcpy(host.data() + lbfem_batch::kCountsBytes, E.data(), E.size() * sizeof(BatchEntry<double>));
+        if (g.adv[sw].n < host.size())
+          g.adv[sw].alloc(host.size() + 16 * sizeof(BatchEntry<double>));
+        if (g.cgt[sw].n < M.size())
+          g.cgt[sw].alloc(M.size() + 16);
+        if (g.nod[sw].n < N.size())
+          g.nod[sw].alloc(N.size() + 16);
+        HIP_CHECK(hipMemcpyAsync(g.adv[sw].p, host.data(), host.size(), hipMemcpyHostToDevice, b.cg_stream));
+        if (n)
+          {
+            HIP_CHECK(hipMemcpyAsync(g.cgt[sw].p, M.data(), M.size() * sizeof(CgMember), hipMemcpyHostToDevice,
+                                     b.cg_stream));
+            HIP_CHECK(hipMemcpyAsync(g.nod[sw].p, N.data(), N.size() * sizeof(NodalMember), hipMemcpyHostToDevice,
+                                     b.cg_stream));
+          }
+        HIP_CHECK(hipStreamSynchronize(b.cg_stream)); // the host tables are released on return
+      }
+  }
+
+  // One step of every member of a CG group (schemes.h:92-98, 118-124; streaming.h:66-100),
+  // the single-mesh step's stages each one launch over all members. Returns LBFEM_OK, or
+  // LBFEM_ENOCONV when a member's solve hit its cap (that member is marked failed; the step
+  // of the others is complete).
+  int
+  cg_group_step(lbfem_batch &b, lbfem_batch::CgGroup &g)
+  {
+    const int   n  = int(g.members.size());
+    const bool  ll = g.scheme == LBFEM_LEELIN;
+    hipStream_t st = b.cg_stream;
+    if (n == 0)
+      return LBFEM_OK;
+    for (lbfem_scheme *s : g.members)
+      for (auto &row : s->last_step_its)
+        for (int &v : row)
+          v = 0;
+    const int  op         = g.streaming != LBFEM_TG2 ? 1 : 0;
+    const bool extrapolate = g.streaming != LBFEM_TG3_SPLIT;
+    int        max_its    = 0;
+    for (const lbfem_scheme *s : g.members)
+      max_its = std::max(max_its, s->set.mass.cg_max_iterations);
+    bool failed = false;
+    for (int sw = 0; sw < g.n_sweeps; ++sw)
+      {
+        const BatchCounts *counts = reinterpret_cast<const BatchCounts *>(g.adv[sw].p);
+        const auto *E = reinterpret_cast<const BatchEntry<double> *>(g.adv[sw].p + lbfem_batch::kCountsBytes);
+        HIP_CHECK(tile_batch_launch<double>(g.p, ll ? kAdvectLeeLin : kAdvectBardow, g.variant, E, counts,
+                                            g.gparity, std::max(8, (g.adv_tiles[sw] + 7) & ~7), st));
+        if (!ll && sw == 0)
+          g.gparity ^= 1; // g* went to the other buffer
+        if (extrapolate)
+          HIP_CHECK(nodal_batch_launch(0, g.nod[sw].p, n, g.nod_blocks, g.gparity, st));
+        HIP_CHECK(cg_batch_start_launch(g.p, op, g.cgt[sw].p, n, g.op_blocks[sw], st));
+        // the vectors each member solves in this sweep (its W.nv): the states past them are unused
+        std::vector<int> nvs(n);
+        for (int k = 0; k < n; ++k)
+          {
+            const auto &e = g.members[k]->sweeps[sw];
+            int         c = 0;
+            for (int a = 0; a < NM; ++a)
+              c += moves(e, a) ? 1 : 0;
+            nvs[k] = c;
+          }
+        const auto active = [&]() -> bool {
+          HIP_CHECK(hipMemcpyAsync(g.host_states, g.states.p, size_t(n) * NM * sizeof(CgState),
+                                   hipMemcpyDeviceToHost, st));
+          HIP_CHECK(hipStreamSynchronize(st));
+          for (int k = 0; k < n; ++k)
+            for (int v = 0; v < nvs[k]; ++v)
+              if (g.host_states[size_t(k) * NM + v].status == 0)
+                return true;
+          return false;
+        };
+        // iterations of converged vectors are no-ops, so the first check waits for the last
+        // step's largest count, as for a scheme alone
+        bool running = true;
+        for (int it = 1; it <= max_its && running; ++it)
+          {
+            HIP_CHECK(cg_batch_direction_launch(g.p, op, g.cgt[sw].p, n, g.op_blocks[sw], it, st));
+            HIP_CHECK(cg_batch_update_launch(g.cgt[sw].p, n, g.upd_blocks[sw], it, st));
+            if (it >= g.last_its)
+              running = active();
+          }
+        if (running)
+          (void)active(); // the update at the cap leaves no vector active; read the final states
+        int largest = 0;
+        for (int k = 0; k < n; ++k)
+          {
+            lbfem_scheme &s  = *g.members[k];
+            const auto   &e  = s.sweeps[sw];
+            int           v  = 0;
+            s.last_its       = 0;
+            for (int a = 0; a < NM; ++a)
+              if (moves(e, a))
+                {
+                  const CgState &c = g.host_states[size_t(k) * NM + v];
+                  s.cg_iterations += c.its;
+                  s.last_step_its[sw][a] = c.its;
+                  s.last_its             = std::max(s.last_its, c.its);
+                  largest                = std::max(largest, c.its);
+                  if (c.status == 2)
+                    {
+                      g.failed[k] = 1;
+                      failed      = true;
+                    }
+                  ++v;
+                }
+          }
+        g.last_its = std::max(1, largest);
+        if (ll) // the predictor-corrector after the solves
+          HIP_CHECK(nodal_batch_launch(2, g.nod[sw].p, n, g.nod_blocks, g.gparity, st));
+        else // f_{a+1} += x_a (streaming.h:94-98)
+          HIP_CHECK(nodal_batch_launch(1, g.nod[sw].p, n, g.nod_blocks, g.gparity, st));
+      }
+    return failed ? fail(LBFEM_ENOCONV, "batch: a member's CG reached its iteration cap (SolverControl::NoConvergence)")
+                  : LBFEM_OK;
+  }
+
   // graph[k]: graph_steps steps starting at parity k (graph_steps even, so parity returns to k).
   // Both graphs or neither: any failure drops what was captured.
   int
@@ -1224,21 +1491,61 @@ namespace
   {
     for (size_t k = 0; k < b.members.size(); ++k)
       b.members[k]->cur = (b.par0[k] + b.parity) & 1;
+    for (const auto &g : b.cg_groups)
+      for (size_t k = 0; k < g->members.size(); ++k)
+        g->members[k]->cur = (g->par0[k] + g->gparity) & 1;
   }
 
-  // what a batch steps: lumped TG2 schemes on tensor meshes, time step set
+  bool
+  is_lumped_fused(const lbfem_scheme *s)
+  {
+    return s->set.mass.type == LBFEM_MASS_LUMPED && s->set.streaming == LBFEM_TG2;
+  }
+
+  // what a batch steps: tensor meshes, time step set; the lumped mass with TG2 streaming (one
+  // fused kernel per step), or, in FP64, the consistent mass with any streaming
   int
   check_batchable(const lbfem_scheme *s)
   {
     if (!s)
       return fail(LBFEM_EINVAL, "null scheme");
-    if (!(s->set.mass.type == LBFEM_MASS_LUMPED && s->set.streaming == LBFEM_TG2))
-      return fail(LBFEM_EINVAL, "a batch steps schemes of the lumped mass and TG2 streaming (one fused kernel)");
     if (s->d->general)
       return fail(LBFEM_EINVAL, "a batch steps tensor-product meshes only (no distort)");
     if (!(s->dt > 0.))
       return fail(LBFEM_EINVAL, "set the time step of every scheme before batching");
+    if (is_lumped_fused(s))
+      return LBFEM_OK;
+    if (s->set.mass.type != LBFEM_MASS_CG)
+      return fail(LBFEM_EINVAL, "a batch steps the lumped or the consistent (CG) mass, not Richardson");
+    if (s->mixed)
+      return fail(LBFEM_EINVAL, "a batch steps consistent-mass schemes in FP64 (the mixed form's refined solves "
+                                "are stepped alone)");
     return LBFEM_OK;
+  }
+
+  // a consistent-mass member joins its (degree, scheme, streaming) CG group, its buffer
+  // parity taken relative to the group's (the tables are written by the caller)
+  void
+  join_cg(lbfem_batch &b, lbfem_scheme *s)
+  {
+    if (!b.cg_stream)
+      HIP_CHECK(hipStreamCreateWithFlags(&b.cg_stream, hipStreamNonBlocking));
+    lbfem_batch::CgGroup *g = nullptr;
+    for (const auto &c : b.cg_groups)
+      if (c->p == s->d->G.p && c->scheme == s->set.scheme && c->streaming == s->set.streaming)
+        g = c.get();
+    if (!g)
+      {
+        b.cg_groups.push_back(std::make_unique<lbfem_batch::CgGroup>());
+        g            = b.cg_groups.back().get();
+        g->p         = s->d->G.p;
+        g->scheme    = s->set.scheme;
+        g->streaming = s->set.streaming;
+        g->n_sweeps  = int(s->sweeps.size());
+      }
+    g->members.push_back(s);
+    g->par0.push_back((s->cur ^ g->gparity) & 1);
+    g->failed.push_back(0);
   }
 
   // a member joins: its buffer parity taken relative to the batch's, its group's table rewritten
@@ -1249,6 +1556,11 @@ namespace
     HIP_CHECK(hipStreamSynchronize(s->stream)); // its own queued work first
     b.members.push_back(s);
     b.par0.push_back((s->cur ^ b.parity) & 1);
+    if (!is_lumped_fused(s))
+      {
+        join_cg(b, s);
+        return;
+      }
     const int mode = s->set.scheme == LBFEM_LEELIN ? kStepLeeLin : kStepBardow;
     auto      it   = std::find_if(b.groups.begin(), b.groups.end(), [&](const auto &g) {
       return g.p == s->d->G.p && g.mode == mode && g.mixed == s->mixed;
@@ -2227,6 +2539,8 @@ extern "C"
           g.variant = size_default_variant(nodes, g.mixed);
         }
       rebuild_tables(*b);
+      for (auto &g : b->cg_groups)
+        write_cg_group(*b, *g);
       *out = b.release();
       return LBFEM_OK;
     });
@@ -2243,7 +2557,14 @@ extern "C"
       if (b->index_of(s) >= 0)
         return fail(LBFEM_EINVAL, "the scheme is already a member of the batch");
       HIP_CHECK(hipStreamSynchronize(b->stream));
+      if (b->cg_stream)
+        HIP_CHECK(hipStreamSynchronize(b->cg_stream));
       join(*b, s);
+      if (!is_lumped_fused(s))
+        {
+          write_cg_group(*b, *b->cg_group_of(s));
+          return LBFEM_OK;
+        }
       const int mode = s->set.scheme == LBFEM_LEELIN ? kStepLeeLin : kStepBardow;
       for (auto &g : b->groups)
         if (g.p == s->d->G.p && g.mode == mode && g.mixed == s->mixed)
@@ -2262,9 +2583,24 @@ extern "C"
       if (k < 0)
         return fail(LBFEM_EINVAL, "the scheme is not a member of the batch");
       HIP_CHECK(hipStreamSynchronize(b->stream));
+      if (b->cg_stream)
+        HIP_CHECK(hipStreamSynchronize(b->cg_stream));
       sync_members(*b);
       b->members.erase(b->members.begin() + k);
       b->par0.erase(b->par0.begin() + k);
+      if (lbfem_batch::CgGroup *cg = b->cg_group_of(s))
+        {
+          const size_t j = size_t(std::find(cg->members.begin(), cg->members.end(), s) - cg->members.begin());
+          cg->members.erase(cg->members.begin() + j);
+          cg->par0.erase(cg->par0.begin() + j);
+          cg->failed.erase(cg->failed.begin() + j);
+          if (cg->members.empty())
+            b->cg_groups.erase(std::find_if(b->cg_groups.begin(), b->cg_groups.end(),
+                                            [&](const auto &u) { return u.get() == cg; }));
+          else
+            write_cg_group(*b, *cg);
+          return LBFEM_OK;
+        }
       for (size_t gi = 0; gi < b->groups.size(); ++gi)
         {
           auto &g  = b->groups[gi];
@@ -2292,6 +2628,8 @@ extern "C"
       if (b)
         {
           (void)hipStreamSynchronize(b->stream);
+          if (b->cg_stream)
+            (void)hipStreamSynchronize(b->cg_stream);
           delete b;
         }
       return LBFEM_OK;
@@ -2319,7 +2657,7 @@ extern "C"
         return fail(LBFEM_EINVAL, "invalid argument");
       if (b->members.empty())
         return LBFEM_OK;
-      int left = n_steps;
+      int left = b->groups.empty() ? 0 : n_steps;
       if (b->graph_steps > 0 && left >= b->graph_steps)
         {
           if (!b->exec[0] || !b->exec[1])
@@ -2334,8 +2672,21 @@ extern "C"
             return st;
           b->parity ^= 1;
         }
+      // the consistent-mass groups on their own stream, beside the lumped groups' queued work
+      int status = LBFEM_OK;
+      for (int k = 0; k < n_steps; ++k)
+        for (auto &g : b->cg_groups)
+          if (const int st = cg_group_step(*b, *g); st != LBFEM_OK)
+            {
+              if (st != LBFEM_ENOCONV)
+                {
+                  sync_members(*b);
+                  return st;
+                }
+              status = st;
+            }
       sync_members(*b);
-      return LBFEM_OK;
+      return status;
     });
   }
 
@@ -2346,6 +2697,26 @@ extern "C"
       if (!b)
         return fail(LBFEM_EINVAL, "null argument");
       HIP_CHECK(hipStreamSynchronize(b->stream));
+      if (b->cg_stream)
+        HIP_CHECK(hipStreamSynchronize(b->cg_stream));
+      return LBFEM_OK;
+    });
+  }
+
+  int
+  lbfem_batch_member_status(const lbfem_batch *b, int32_t *status)
+  {
+    return guarded([&]() -> int {
+      if (!b || !status)
+        return fail(LBFEM_EINVAL, "null argument");
+      for (size_t k = 0; k < b->members.size(); ++k)
+        {
+          status[k] = LBFEM_OK;
+          for (const auto &g : b->cg_groups)
+            for (size_t j = 0; j < g->members.size(); ++j)
+              if (g->members[j] == b->members[k] && g->failed[j])
+                status[k] = LBFEM_ENOCONV;
+        }
       return LBFEM_OK;
     });
   }
@@ -2373,8 +2744,13 @@ extern "C"
           nodes += g.n_nodes;
           tiles += g.n_tiles;
         }
+      for (const auto &g : b->cg_groups)
+        {
+          nodes += g->n_nodes;
+          tiles += g->adv_tiles[0];
+        }
       if (n_groups)
-        *n_groups = int(b->groups.size());
+        *n_groups = int(b->groups.size() + b->cg_groups.size());
       if (n_nodes)
         *n_nodes = nodes;
       if (n_tiles)
@@ -2477,6 +2853,8 @@ extern "C"
     return guarded([&]() -> int {
       if (!b)
         return fail(LBFEM_EINVAL, "null argument");
+      if (b->cg_stream)
+        HIP_CHECK(hipStreamSynchronize(b->cg_stream));
       HIP_CHECK(hipStreamSynchronize(b->stream));
       sync_members(*b);
       std::vector<MonitorEntry> E;

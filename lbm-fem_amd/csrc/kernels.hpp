@@ -95,6 +95,7 @@ namespace lbfem_amd
     Geo        G;
     TileParams S;          // tiles_x, tiles_y of the batch's variant
     T         *f[2];       // the instance's population buffers
+    T         *rhs;        // the advection modes' right-hand side (8 blocks), else null
     int        par;        // buffer read at parity 0
     int        tile_begin; // first tile in the batch's tile list
   };
@@ -202,6 +203,42 @@ namespace lbfem_amd
 
   // partial rows the CG direction stage writes (its blocks per vector)
   int cg_direction_rows(const Geo &G);
+  // the update stage's grid of a CG with the partial rows alpha_rows ahead of its own
+  int cg_update_blocks(const Geo &G, int max_blocks, int alpha_rows);
+
+  // ---- the consistent-mass solves of many meshes in one launch per stage (a batch of
+  // schemes of one degree, scheme and streaming, FP64): each member's blocks, partial rows,
+  // states and stopping tests are its single-mesh ones, so its solves are bitwise the same
+  struct CgMember
+  {
+    Geo           G;
+    CgWork        W;          // the member's vectors, states (in the batch's state array), partial rows
+    const double *b;          // right-hand sides (8 blocks)
+    double       *x, *r, *p[2], *q;
+    int           tiles_x, tiles_y, nt, op_begin; // operator tiles (32 x 16) and the first block of the member
+    int           upd_blocks, upd_begin;          // the update grid and its first block
+  };
+  // start (r = A x - b, finalisation 1); direction (p, q = A p, finalisation 2 where the
+  // update does not reduce it); update (x, r, finalisation 3). op: 0 mass, 1 mass + c K_e.
+  hipError_t cg_batch_start_launch(int p, int op, const CgMember *T, int n, int op_blocks, hipStream_t stream);
+  hipError_t cg_batch_direction_launch(int p, int op, const CgMember *T, int n, int op_blocks, int it,
+                                       hipStream_t stream);
+  hipError_t cg_batch_update_launch(const CgMember *T, int n, int upd_blocks, int it, hipStream_t stream);
+
+  struct NodalMember
+  {
+    Geo      G;
+    double  *f[2];  // population buffers, f[(par + parity) & 1] the current one
+    double  *x, *x_prev;
+    int      par;
+    unsigned mask;  // the populations of the sweep (kinds 0, 1)
+    double   theta; // Lee-Lin dt / lambda (kind 2)
+    int      block_begin, nblocks;
+  };
+  // kind 0: extrapolated start; 1: f_{a+1} += x_a; 2: Lee-Lin predictor-corrector
+  hipError_t nodal_batch_launch(int kind, const NodalMember *T, int n, int blocks, int parity, hipStream_t stream);
+  // the blocks a member of the nodal batch kernels gets
+  int nodal_batch_blocks(const Geo &G);
 
   // r = A x - b, tolerances, rho = r.D^-1 r (iteration 0 of SolverCG)
   hipError_t cg_start_launch(const Geo &G, const CgWork &W, const double *x, const double *b, double *r,

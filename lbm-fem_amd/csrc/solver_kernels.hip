@@ -254,10 +254,11 @@ namespace lbfem_amd
     return which == 0 ? a.M : which == 1 ? a.D : which == 2 ? a.K : a.DT;
   }
 
+  // The body of one operator block: block bid of the mesh's (XCD-ordered) tile blocks, vector v.
   template <typename V, int p, int TX, int TY, int KIND, int OP>
-  __global__ void __launch_bounds__(kThreads)
-  k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const V *x,
-             const V *b, const V *r_in, V *y, V *y2)
+  __device__ __forceinline__ void
+  operator_body(const Geo &G, const CgWork &W, const int it, const int tiles_x, const int tiles_y, const int bid,
+                const int v, const V *x, const V *b, const V *r_in, V *y, V *y2)
   {
     using B                  = OpBands<OP>;
     constexpr int  W_        = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
@@ -270,7 +271,6 @@ namespace lbfem_amd
 
     const int nt   = tiles_x * tiles_y;
     const int nb8  = (nt + 7) & ~7;
-    const int bid  = blockIdx.x;
     const int tile = (bid & 7) * (nb8 >> 3) + (bid >> 3);
     if (tile >= nt)
       return;
@@ -278,8 +278,7 @@ namespace lbfem_amd
     const int P = G.P, tid = threadIdx.x;
     const auto wall = [P](const int i, const int j) { return i == 0 || j == 0 || i == P || j == P; };
 
-    // this block's vector (blockIdx.y, uniform); a converged one is skipped
-    const int v = blockIdx.y;
+    // this block's vector (uniform over the block); a converged one is skipped
     if (KIND == 2 && W.state[v].status != 0)
       {
         if (tid == 0)
@@ -447,6 +446,49 @@ namespace lbfem_amd
       }
   }
 
+  template <typename V, int p, int TX, int TY, int KIND, int OP>
+  __global__ void __launch_bounds__(kThreads)
+  k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const V *x,
+             const V *b, const V *r_in, V *y, V *y2)
+  {
+    operator_body<V, p, TX, TY, KIND, OP>(G, W, it, tiles_x, tiles_y, blockIdx.x, blockIdx.y, x, b, r_in, y, y2);
+  }
+
+  // the member of a batched launch that owns block b: the last whose first block is <= b
+  template <typename M, typename F>
+  __device__ __forceinline__ int
+  member_of(const M *__restrict__ T, const int n, const int b, F first)
+  {
+    int lo = 0, hi = n - 1;
+    while (lo < hi)
+      {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first(T[mid]) <= b)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+    return lo;
+  }
+
+  // The operator stage of many meshes' CGs in one launch: member m owns the blocks
+  // [op_begin, op_begin + nb8) of the launch and runs exactly its own single-mesh blocks
+  // (the CG start, KIND 1, on x, b, r; the direction, KIND 2, on the directions of
+  // iteration it and q).
+  template <int p, int TX, int TY, int KIND, int OP>
+  __global__ void __launch_bounds__(kThreads)
+  k_operator_batch(const CgMember *__restrict__ T, const int n, const int it)
+  {
+    const int       k = member_of(T, n, int(blockIdx.x), [](const CgMember &m) { return m.op_begin; });
+    const CgMember &m = T[k];
+    if (int(blockIdx.y) >= m.W.nv) // the launch has a row per population; a sweep may solve fewer
+      return;
+    const double   *x = KIND == 1 ? m.x : m.p[it % 2];
+    double         *y = KIND == 1 ? m.r : m.p[(it + 1) % 2];
+    operator_body<double, p, TX, TY, KIND, OP>(m.G, m.W, it, m.tiles_x, m.tiles_y, int(blockIdx.x) - m.op_begin,
+                                               int(blockIdx.y), x, m.b, m.r, y, m.q);
+  }
+
   // The scalars of one CG stage from the per-block partials, one block per vector and a
   // fixed summation order (a kernel boundary, not a device-wide fence, orders it after the
   // partials): thread t takes rows t, t + 256, ..., then the wavefront (shuffles), then the
@@ -457,14 +499,13 @@ namespace lbfem_amd
   constexpr int kFinThreads = kThreads;
 
   template <int STAGE>
-  __global__ void __launch_bounds__(kFinThreads)
-  k_cg_finalize(const CgWork W, const int nparts, const int it, const int row0)
+  __device__ __forceinline__ void
+  finalize_body(const CgWork &W, const int v, const int nparts, const int it, const int row0)
   {
     constexpr int stride = STAGE == 3 ? 2 : 3, nk = STAGE == 1 ? 3 : STAGE == 2 ? 1 : 2;
     constexpr int NW = kFinThreads / 64;
     __shared__ double red[nk][NW];
     __shared__ double tot[nk];
-    const int         v = blockIdx.x; // the vector (uniform)
     if (v >= W.nv)
       return;
     CgState st = W.state[v];
@@ -528,19 +569,37 @@ namespace lbfem_amd
     W.state[v] = st;
   }
 
+  template <int STAGE>
+  __global__ void __launch_bounds__(kFinThreads)
+  k_cg_finalize(const CgWork W, const int nparts, const int it, const int row0)
+  {
+    finalize_body<STAGE>(W, blockIdx.x, nparts, it, row0);
+  }
+
+  // every member's finalisation of one stage in one launch: block (member, vector); the
+  // direction stage skips the members whose update kernel reduces its partials itself
+  template <int STAGE>
+  __global__ void __launch_bounds__(kFinThreads)
+  k_cg_finalize_batch(const CgMember *__restrict__ T, const int it)
+  {
+    const CgMember &m = T[blockIdx.x];
+    if (STAGE == 2 && m.W.alpha_rows > 0)
+      return;
+    finalize_body<STAGE>(m.W, blockIdx.y, STAGE == 3 ? m.upd_blocks : m.nt, it, STAGE == 3 ? m.W.alpha_rows : 0);
+  }
+
   // x += alpha p; r += alpha q, with the partials of |r|^2 and r.D^-1 r (the stopping
   // test and beta follow in k_cg_finalize<3>). blockIdx.y is the vector: each block updates
   // one vector over the nodes its blockIdx.x strides through, so a batch of nv vectors has
   // nv times the wavefronts in flight, and each vector's sums keep the order they have with
   // one vector per launch.
   template <typename V>
-  __global__ void __launch_bounds__(kThreads)
-  k_cg_update(const Geo G, const CgWork W, V *__restrict__ x, V *__restrict__ r, const V *__restrict__ p,
-              const V *__restrict__ q)
+  __device__ __forceinline__ void
+  update_body(const Geo &G, const CgWork &W, V *__restrict__ x, V *__restrict__ r, const V *__restrict__ p,
+              const V *__restrict__ q, const int v, const int blk, const int nblk)
   {
     __shared__ double red[2][kThreads / 64];
     __shared__ double alpha_s;
-    const int         v = blockIdx.y; // uniform over the block
     if (v >= W.nv || W.state[v].status != 0)
       return;
     const int64_t n   = int64_t(G.np) * G.np;
@@ -562,7 +621,7 @@ namespace lbfem_amd
       }
     const V alpha   = V(a);
     double  part[2] = {0., 0.};
-    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
+    for (int64_t k = blk * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(nblk) * kThreads)
       {
         const int    i = int(k % G.np), j = int(k / G.np);
         const double d  = mass_diag_inv(G, i, j);
@@ -578,8 +637,82 @@ namespace lbfem_amd
     // after the direction stage's rows (other blocks may still be reading them for alpha)
     if (threadIdx.x == 0)
       {
-        W.partial[size_t(W.alpha_rows + blockIdx.x) * kPart + 2 * v]     = part[0];
-        W.partial[size_t(W.alpha_rows + blockIdx.x) * kPart + 2 * v + 1] = part[1];
+        W.partial[size_t(W.alpha_rows + blk) * kPart + 2 * v]     = part[0];
+        W.partial[size_t(W.alpha_rows + blk) * kPart + 2 * v + 1] = part[1];
+      }
+  }
+
+  template <typename V>
+  __global__ void __launch_bounds__(kThreads)
+  k_cg_update(const Geo G, const CgWork W, V *__restrict__ x, V *__restrict__ r, const V *__restrict__ p,
+              const V *__restrict__ q)
+  {
+    update_body<V>(G, W, x, r, p, q, blockIdx.y, blockIdx.x, gridDim.x);
+  }
+
+  // every member's update in one launch: member m owns the blocks [upd_begin, upd_begin +
+  // upd_blocks) of the launch, its single-mesh update grid
+  __global__ void __launch_bounds__(kThreads)
+  k_cg_update_batch(const CgMember *__restrict__ T, const int n, const int it)
+  {
+    const int       k = member_of(T, n, int(blockIdx.x), [](const CgMember &m) { return m.upd_begin; });
+    const CgMember &m = T[k];
+    update_body<double>(m.G, m.W, m.x, m.r, m.p[(it + 1) % 2], m.q, blockIdx.y, int(blockIdx.x) - m.upd_begin,
+                        m.upd_blocks);
+  }
+
+  // The nodal stages of the batched consistent-mass step, elementwise over each member's
+  // nodes (member m owns blocks [block_begin, block_begin + nblocks)): KIND 0 the
+  // extrapolated start x = 2 x - x_prev, x_prev = x (streaming.h:76-80); 1 the increments
+  // f_{a+1} += x_a (streaming.h:94-98); 2 Lee-Lin's predictor-corrector after the solves.
+  template <int KIND>
+  __global__ void __launch_bounds__(kThreads)
+  k_nodal_batch(const NodalMember *__restrict__ T, const int n, const int parity)
+  {
+    const int          k = member_of(T, n, int(blockIdx.x), [](const NodalMember &m) { return m.block_begin; });
+    const NodalMember &m = T[k];
+    const Geo         &G = m.G;
+    const int64_t      nn = int64_t(G.np) * G.np;
+    double            *f  = m.f[(m.par + parity) & 1];
+    for (int64_t q = int64_t(int(blockIdx.x) - m.block_begin) * kThreads + threadIdx.x; q < nn;
+         q += int64_t(m.nblocks) * kThreads)
+      {
+        if constexpr (KIND == 0)
+          {
+#pragma unroll
+            for (int a = 0; a < NM; ++a)
+              if (m.mask & (1u << a))
+                {
+                  const double xa = m.x[a * G.stride + q], xp = m.x_prev[a * G.stride + q];
+                  m.x[a * G.stride + q]      = -xp + 2. * xa;
+                  m.x_prev[a * G.stride + q] = xa;
+                }
+          }
+        else if constexpr (KIND == 1)
+          {
+#pragma unroll
+            for (int a = 0; a < NM; ++a)
+              if (m.mask & (1u << a))
+                f[(a + 1) * G.stride + q] += m.x[a * G.stride + q];
+          }
+        else
+          {
+            const int i = int(q % G.np), j = int(q / G.np);
+            double    v[Q], e[Q], x[NM];
+#pragma unroll
+            for (int a = 0; a < Q; ++a)
+              v[a] = f[a * G.stride + q];
+#pragma unroll
+            for (int a = 0; a < NM; ++a)
+              x[a] = m.x[a * G.stride + q];
+            double     wx = 0., wy = 0.;
+            const bool wall = wall_of(G, i, j, wx, wy);
+            equilibrium_node(v, e, wall, wx, wy);
+            predictor_corrector_node(v, e, x, m.theta, wall, wx, wy);
+#pragma unroll
+            for (int a = 0; a < Q; ++a)
+              f[a * G.stride + q] = v[a];
+          }
       }
   }
 
@@ -961,14 +1094,116 @@ namespace lbfem_amd
     return ((G.P + 1 + TX - 1) / TX) * ((G.P + 1 + TY - 1) / TY);
   }
 
+  int
+  cg_update_blocks(const Geo &G, const int max_blocks, const int alpha_rows)
+  {
+    return std::min(nodal_blocks(G), std::min(max_blocks - alpha_rows, 1024));
+  }
+
+  int
+  nodal_batch_blocks(const Geo &G)
+  {
+    return nodal_blocks(G);
+  }
+
   template <typename V>
   static hipError_t
   update_launch(const Geo &G, const CgWork &W, const int it, V *x, V *r, const V *p, const V *q, hipStream_t s)
   {
-    int nb = nodal_blocks(G);
-    nb     = std::min(nb, std::min(W.max_blocks - W.alpha_rows, 1024));
+    const int nb = cg_update_blocks(G, W.max_blocks, W.alpha_rows);
     hipLaunchKernelGGL(k_cg_update<V>, dim3(nb, std::max(W.nv, 1)), dim3(kThreads), 0, s, G, W, x, r, p, q);
     hipLaunchKernelGGL((k_cg_finalize<3>), dim3(std::max(W.nv, 1)), dim3(kFinThreads), 0, s, W, nb, it, W.alpha_rows);
+    return hipGetLastError();
+  }
+
+  // ------------------------------------------------------------------ batched CG launchers
+  template <int KIND>
+  static hipError_t
+  operator_batch_launch(const int p, const int op, const CgMember *T, const int n, const int op_blocks, const int it,
+                        hipStream_t s)
+  {
+    constexpr int TX = 32, TY = 16;
+    const dim3    grid(op_blocks, NM), block(kThreads);
+    const auto    go = [&](auto pc, auto oc) {
+      constexpr int P_ = decltype(pc)::value, OP_ = decltype(oc)::value;
+      hipLaunchKernelGGL((k_operator_batch<P_, TX, TY, KIND, OP_>), grid, block, 0, s, T, n, it);
+    };
+    const auto by_op = [&](auto pc) {
+      if (op == kOpMass)
+        go(pc, std::integral_constant<int, kOpMass>{});
+      else
+        go(pc, std::integral_constant<int, kOpStreaming>{});
+    };
+    switch (p)
+      {
+        case 1:
+          by_op(std::integral_constant<int, 1>{});
+          break;
+        case 2:
+          by_op(std::integral_constant<int, 2>{});
+          break;
+        case 3:
+          by_op(std::integral_constant<int, 3>{});
+          break;
+        default:
+          return hipErrorInvalidValue;
+      }
+    return hipGetLastError();
+  }
+
+  hipError_t
+  cg_batch_start_launch(const int p, const int op, const CgMember *T, const int n, const int op_blocks, hipStream_t s)
+  {
+    if (n <= 0)
+      return hipSuccess;
+    if (const hipError_t e = operator_batch_launch<1>(p, op, T, n, op_blocks, 0, s); e != hipSuccess)
+      return e;
+    hipLaunchKernelGGL((k_cg_finalize_batch<1>), dim3(n, NM), dim3(kFinThreads), 0, s, T, 0);
+    return hipGetLastError();
+  }
+
+  hipError_t
+  cg_batch_direction_launch(const int p, const int op, const CgMember *T, const int n, const int op_blocks,
+                            const int it, hipStream_t s)
+  {
+    if (n <= 0)
+      return hipSuccess;
+    if (const hipError_t e = operator_batch_launch<2>(p, op, T, n, op_blocks, it, s); e != hipSuccess)
+      return e;
+    hipLaunchKernelGGL((k_cg_finalize_batch<2>), dim3(n, NM), dim3(kFinThreads), 0, s, T, it);
+    return hipGetLastError();
+  }
+
+  hipError_t
+  cg_batch_update_launch(const CgMember *T, const int n, const int upd_blocks, const int it, hipStream_t s)
+  {
+    if (n <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL(k_cg_update_batch, dim3(upd_blocks, NM), dim3(kThreads), 0, s, T, n, it);
+    hipLaunchKernelGGL((k_cg_finalize_batch<3>), dim3(n, NM), dim3(kFinThreads), 0, s, T, it);
+    return hipGetLastError();
+  }
+
+  hipError_t
+  nodal_batch_launch(const int kind, const NodalMember *T, const int n, const int blocks, const int parity,
+                     hipStream_t s)
+  {
+    if (n <= 0)
+      return hipSuccess;
+    switch (kind)
+      {
+        case 0:
+          hipLaunchKernelGGL(k_nodal_batch<0>, dim3(blocks), dim3(kThreads), 0, s, T, n, parity);
+          break;
+        case 1:
+          hipLaunchKernelGGL(k_nodal_batch<1>, dim3(blocks), dim3(kThreads), 0, s, T, n, parity);
+          break;
+        case 2:
+          hipLaunchKernelGGL(k_nodal_batch<2>, dim3(blocks), dim3(kThreads), 0, s, T, n, parity);
+          break;
+        default:
+          return hipErrorInvalidValue;
+      }
     return hipGetLastError();
   }
 

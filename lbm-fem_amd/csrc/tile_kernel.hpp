@@ -672,7 +672,10 @@ namespace lbfem_amd
           }
         const BatchEntry<typename C::T> &e   = E[lo];
         const int                        cur = (e.par + parity) & 1;
-        tile::body<C>(e.G, e.S, t - e.tile_begin, e.f[cur], nullptr, e.f[cur ^ 1], nullptr, sm);
+        // the fused step writes the other buffer; the advection writes the right-hand side and, with
+        // the collision fused, g* into the other buffer
+        typename C::T *out = (C::STEP || e.S.collide) ? e.f[cur ^ 1] : nullptr;
+        tile::body<C>(e.G, e.S, t - e.tile_begin, e.f[cur], nullptr, out, e.rhs, sm);
         __syncthreads(); // the next tile reuses the LDS
       }
   }
@@ -773,6 +776,10 @@ namespace lbfem_amd
           return launch_batch_variant<T, p, kStepBardow>(variant, table, counts, parity, grid, stream);
         case kStepLeeLin:
           return launch_batch_variant<T, p, kStepLeeLin>(variant, table, counts, parity, grid, stream);
+        case kAdvectBardow:
+          return launch_batch_variant<T, p, kAdvectBardow>(variant, table, counts, parity, grid, stream);
+        case kAdvectLeeLin:
+          return launch_batch_variant<T, p, kAdvectLeeLin>(variant, table, counts, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }

@@ -113,6 +113,7 @@ def lib():
             "lbfem_batch_create": (i, [C.POINTER(_vp), i, C.POINTER(_vp)]),
             "lbfem_batch_add": (i, [_vp, _vp]),
             "lbfem_batch_remove": (i, [_vp, _vp]),
+            "lbfem_batch_member_status": (i, [_vp, i32]),
             "lbfem_batch_velocity_change": (i, [_vp, i32, p, p, i64]),
             "lbfem_scheme_velocity_change": (i, [_vp, p, p, i64]),
             "lbfem_batch_destroy": (i, [_vp]),
@@ -442,9 +443,10 @@ def hbm_copy(dst, src, stream=None):
 
 
 class Batch:
-    """lbfem_batch: lumped TG2 schemes stepped together, one launch per (degree, scheme, precision) group
-    per step, steps replayed from captured HIP graphs. Members are read after synchronize(); the batch keeps
-    references to them, and is destroyed (close()) before they are."""
+    """lbfem_batch: schemes stepped together. Lumped TG2 members: one launch per (degree, scheme, precision)
+    group per step, steps replayed from captured HIP graphs; consistent-mass FP64 members: each stage of the
+    step one launch over the (degree, scheme, streaming) group. Members are read after synchronize(); the
+    batch keeps references to them, and is destroyed (close()) before they are."""
 
     def __init__(self, schemes=(), graph_steps=8):
         self.schemes = list(schemes)
@@ -469,6 +471,12 @@ class Batch:
 
     def synchronize(self):
         check(lib().lbfem_batch_synchronize(self.h))
+
+    def member_status(self):
+        """per member: OK, or ENOCONV once its CG reached its iteration cap"""
+        st = np.zeros(max(1, len(self.schemes)), dtype=np.int32)
+        check(lib().lbfem_batch_member_status(self.h, st.ctypes.data_as(C.POINTER(C.c_int32))))
+        return st[: len(self.schemes)]
 
     def add(self, scheme):
         """lbfem_batch_add: the scheme joins the running batch from its current populations."""

@@ -78,15 +78,15 @@ def request_stream(base_seed, n, rate, sizes=(32, 64, 128, 256), steps=(2000, 20
     return out
 
 
-def make_scheme(A, inst, precision, scheme=None, cfl=0.5):
-    """A lumped TG2 scheme for a workload instance, initial populations set. The default stepper is
-    Bardow's: Lee & Lin's with the lumped mass is unstable for dt / lambda below about 1 (the low-Re,
-    finely stretched cavities of these workloads), as in the reference, which runs it at 1.15 and above."""
+def make_scheme(A, inst, precision, scheme=None, cfl=0.5, mass=None, cg_tolerance=1e-8):
+    """A TG2 scheme for a workload instance (the lumped mass unless `mass` says otherwise), initial
+    populations set. The default stepper is Bardow's: Lee & Lin's is unstable for dt / lambda below about 1
+    (the low-Re, finely stretched cavities of these workloads) with either mass, in the oracle as here."""
     tc = A.test_cases.make(inst["case"], mach=inst["mach"], modes=(1, 1), stretch=1.2)
     tc.set_reynolds(inst["Re"])
     d = A.Discretization(tc, inst["N"], inst["p"])
     cls = A.LeeLin if scheme == A.LEELIN else A.Bardow
-    s = cls(d, mass=A.MASS_LUMPED, precision=precision)
+    s = cls(d, mass=A.MASS_LUMPED if mass is None else mass, precision=precision, cg_tolerance=cg_tolerance)
     s.set_time_step(d.time_step(cfl), tc.relaxation_time())
     s.set_initial_populations(tc)
     return s
