@@ -77,6 +77,17 @@ namespace
   }
 #define HIP_CHECK(x) check((x), #x)
 
+  // HIP runtime calls that allocate or free memory, create or destroy streams, events and
+  // graphs, or copy on the null stream are serialized with the capture of the batches' step
+  // graphs: a capture in one thread fails when another thread makes such calls meanwhile.
+  std::recursive_mutex &
+  runtime_mutex()
+  {
+    static std::recursive_mutex m;
+    return m;
+  }
+  using RuntimeLock = std::lock_guard<std::recursive_mutex>;
+
   template <typename T>
   struct DevBuf
   {
@@ -97,6 +108,7 @@ namespace
     {
       if (this != &o)
         {
+          RuntimeLock lock(runtime_mutex());
           if (p)
             (void)hipFree(p);
           p   = o.p;
@@ -108,12 +120,14 @@ namespace
     }
     ~DevBuf()
     {
+      RuntimeLock lock(runtime_mutex());
       if (p)
         (void)hipFree(p);
     }
     void
     alloc(const size_t count)
     {
+      RuntimeLock lock(runtime_mutex());
       if (p)
         (void)hipFree(p);
       p = nullptr;
@@ -125,6 +139,7 @@ namespace
     void
     upload(const std::vector<T> &v)
     {
+      RuntimeLock lock(runtime_mutex());
       alloc(v.size());
       if (!v.empty())
         HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -168,6 +183,7 @@ struct lbfem_disc
   lbfem_disc &operator=(const lbfem_disc &) = delete;
   ~lbfem_disc()
   {
+    RuntimeLock lock(runtime_mutex());
     if (host_cg)
       (void)hipHostFree(host_cg);
   }
@@ -353,6 +369,7 @@ struct lbfem_disc
   void
   ensure_scratch()
   {
+    RuntimeLock lock(runtime_mutex());
     if (!host_cg)
       HIP_CHECK(hipHostMalloc(&host_cg, sizeof(CgState) * NM));
     if (counter.p) // allocated last: the rest is there
@@ -392,6 +409,7 @@ namespace
     Monitor &operator=(const Monitor &) = delete;
     ~Monitor()
     {
+      RuntimeLock lock(runtime_mutex());
       if (host)
         (void)hipHostFree(host);
     }
@@ -419,6 +437,7 @@ namespace
         results.alloc(nr);
       if (host_cap < nr)
         {
+          RuntimeLock lock(runtime_mutex());
           if (host)
             (void)hipHostFree(host);
           host     = nullptr;
@@ -475,6 +494,7 @@ struct lbfem_scheme
 
   ~lbfem_scheme()
   {
+    RuntimeLock lock(runtime_mutex());
     for (auto &m : marks)
       (void)hipEventDestroy(m.first);
     if (host_state)
@@ -497,7 +517,10 @@ struct lbfem_scheme
     if (!timing)
       return;
     hipEvent_t e;
-    HIP_CHECK(hipEventCreate(&e));
+    {
+      RuntimeLock lock(runtime_mutex());
+      HIP_CHECK(hipEventCreate(&e));
+    }
     HIP_CHECK(hipEventRecord(e, stream));
     marks.push_back({e, stage_id});
   }
@@ -1052,6 +1075,7 @@ struct lbfem_batch
     CgGroup &operator=(const CgGroup &) = delete;
     ~CgGroup()
     {
+      RuntimeLock lock(runtime_mutex());
       if (host_states)
         (void)hipHostFree(host_states);
     }
@@ -1073,6 +1097,7 @@ struct lbfem_batch
   void
   drop_graphs()
   {
+    RuntimeLock lock(runtime_mutex());
     for (int k = 0; k < 2; ++k)
       {
         if (exec[k])
@@ -1104,6 +1129,7 @@ struct lbfem_batch
 
   ~lbfem_batch()
   {
+    RuntimeLock lock(runtime_mutex());
     drop_graphs();
     if (stream)
       (void)hipStreamDestroy(stream);
@@ -1232,6 +1258,7 @@ namespace
     const int n = int(g.members.size());
     if (g.states.n < size_t(n) * NM || !g.host_states)
       {
+        RuntimeLock  lock(runtime_mutex());
         const size_t cap = size_t(std::max(8, 2 * n)) * NM;
         g.states.alloc(cap);
         if (g.host_states)
@@ -1446,6 +1473,7 @@ namespace
   int
   capture_graphs(lbfem_batch &b)
   {
+    RuntimeLock lock(runtime_mutex()); // no allocation in another thread while the stream captures
     b.drop_graphs();
     try
       {
@@ -1529,7 +1557,10 @@ namespace
   join_cg(lbfem_batch &b, lbfem_scheme *s)
   {
     if (!b.cg_stream)
-      HIP_CHECK(hipStreamCreateWithFlags(&b.cg_stream, hipStreamNonBlocking));
+      {
+        RuntimeLock lock(runtime_mutex());
+        HIP_CHECK(hipStreamCreateWithFlags(&b.cg_stream, hipStreamNonBlocking));
+      }
     lbfem_batch::CgGroup *g = nullptr;
     for (const auto &c : b.cg_groups)
       if (c->p == s->d->G.p && c->scheme == s->set.scheme && c->streaming == s->set.streaming)
@@ -1681,6 +1712,7 @@ extern "C"
   int
   lbfem_disc_create(const lbfem_mesh_desc *desc, lbfem_disc **out)
   {
+    RuntimeLock lock(runtime_mutex());
     return guarded([&]() -> int {
       if (!desc || !out)
         return fail(LBFEM_EINVAL, "null argument");
@@ -1725,6 +1757,7 @@ extern "C"
   int
   lbfem_disc_destroy(lbfem_disc *d)
   {
+    RuntimeLock lock(runtime_mutex());
     delete d;
     return LBFEM_OK;
   }
@@ -1815,6 +1848,7 @@ extern "C"
   int
   lbfem_disc_set_walls(lbfem_disc *d, const int32_t n, const int32_t *nodes, const double *velocity)
   {
+    RuntimeLock lock(runtime_mutex());
     return guarded([&]() -> int {
       if (!d || n < 0 || (n > 0 && (!nodes || !velocity)))
         return fail(LBFEM_EINVAL, "null argument");
@@ -2065,6 +2099,7 @@ extern "C"
   int
   lbfem_scheme_create(const lbfem_disc *d, const lbfem_scheme_settings *set, lbfem_scheme **out)
   {
+    RuntimeLock lock(runtime_mutex());
     return guarded([&]() -> int {
       if (!d || !set || !out)
         return fail(LBFEM_EINVAL, "null argument");
@@ -2168,6 +2203,7 @@ extern "C"
   int
   lbfem_scheme_destroy(lbfem_scheme *s)
   {
+    RuntimeLock lock(runtime_mutex());
     delete s;
     return LBFEM_OK;
   }
@@ -2259,15 +2295,14 @@ extern "C"
       const int64_t n = s->d->n;
       if (s->mixed)
         {
-          HIP_CHECK(hipStreamSynchronize(s->stream));
-          std::vector<float> dv(static_cast<size_t>(n));
+          std::vector<float> dv(static_cast<size_t>(Q * n));
           for (int a = 0; a < Q; ++a)
-            {
-              HIP_CHECK(hipMemcpy(dv.data(), s->g32[s->cur].p + a * s->d->stride, n * sizeof(float),
-                                  hipMemcpyDeviceToHost));
-              for (int64_t i = 0; i < n; ++i)
-                f[a * n + i] = w_of(a) + double(dv[i]);
-            }
+            HIP_CHECK(hipMemcpyAsync(dv.data() + a * n, s->g32[s->cur].p + a * s->d->stride, n * sizeof(float),
+                                     hipMemcpyDeviceToHost, s->stream));
+          HIP_CHECK(hipStreamSynchronize(s->stream));
+          for (int a = 0; a < Q; ++a)
+            for (int64_t i = 0; i < n; ++i)
+              f[a * n + i] = w_of(a) + double(dv[a * n + i]);
           return LBFEM_OK;
         }
       for (int a = 0; a < Q; ++a)
@@ -2465,7 +2500,8 @@ extern "C"
         s->tile_variant = hit->second;
       else
         {
-          hipEvent_t e0, e1;
+          RuntimeLock lock(runtime_mutex());
+          hipEvent_t  e0, e1;
           HIP_CHECK(hipEventCreate(&e0));
           HIP_CHECK(hipEventCreate(&e1));
           double best = 1e300;
@@ -2515,6 +2551,7 @@ extern "C"
   int
   lbfem_batch_create(lbfem_scheme *const *schemes, const int n, lbfem_batch **out)
   {
+    RuntimeLock lock(runtime_mutex());
     return guarded([&]() -> int {
       if (!out || n < 0 || (n > 0 && !schemes))
         return fail(LBFEM_EINVAL, "lbfem_batch_create: invalid arguments");
@@ -2788,7 +2825,8 @@ extern "C"
         return fail(LBFEM_EINVAL, "null argument");
       HIP_CHECK(hipStreamSynchronize(b->stream));
       sync_members(*b);
-      hipEvent_t e0, e1;
+      RuntimeLock lock(runtime_mutex());
+      hipEvent_t  e0, e1;
       HIP_CHECK(hipEventCreate(&e0));
       HIP_CHECK(hipEventCreate(&e1));
       const int grids[3] = {0, 512, 2048};
@@ -2935,6 +2973,7 @@ extern "C"
   lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const double *v, double *psi,
                         int *iterations)
   {
+    RuntimeLock lock(runtime_mutex());
     return guarded([&]() -> int {
       if (!desc || !u || !v || !psi)
         return fail(LBFEM_EINVAL, "null argument");

@@ -16,11 +16,16 @@ by adding them to the running batch, and retires requests by removing them:
     reference's run ends with an error, apps/cgdbe/cgdbe.cc:583-590).
 The velocity change and the finiteness check are one device reduction over the
 members concerned (lbfem_batch_velocity_change); the batch is never rebuilt
-and its graphs are kept while members come and go. A request's result is
-bitwise the one it gets stepped alone for the same number of steps.
+and its graphs are kept while members come and go. A request's scheme (mesh,
+initial populations: host work of up to a second for the largest meshes) is
+prepared on worker threads as soon as it arrives and the budget allows, while
+the device steps the others; admission takes prepared requests in arrival
+order, at the end of each tick (at most tick_steps steps). A request's result
+is bitwise the one it gets stepped alone under the same retirement rules.
 Reported per request: arrival, admission and completion times, steps, status;
 overall: time-to-solution percentiles and node.population updates per second.
 """
+import concurrent.futures
 import time
 
 import numpy as np
@@ -35,8 +40,8 @@ def request_bytes(inst, precision_bytes=4):
 
 
 class Server:
-    def __init__(self, A, precision=None, budget_bytes=240e9, max_slots=96, tick_steps=1000, window=1000,
-                 graph_steps=8, grid=2048, scheme=None, cfl=0.5, keep_results=False):
+    def __init__(self, A, precision=None, budget_bytes=240e9, max_slots=96, tick_steps=200, window=1000,
+                 graph_steps=8, grid=2048, scheme=None, cfl=0.5, keep_results=False, workers=2):
         self.A = A
         self.precision = A.MIXED if precision is None else precision
         self.budget = budget_bytes
@@ -48,6 +53,7 @@ class Server:
         self.scheme = scheme
         self.cfl = cfl
         self.keep_results = keep_results
+        self.workers = workers
 
     def _next_event(self, a):
         """steps until this member's next retirement check"""
@@ -62,7 +68,9 @@ class Server:
         A = self.A
         pending = sorted(requests, key=lambda r: r["arrival"])
         waiting, active, done = [], [], []
+        preparing = []  # (request, future of its scheme), in arrival order; their memory is reserved
         used = 0
+        pool = concurrent.futures.ThreadPoolExecutor(max_workers=self.workers)
         pb = 4 if self.precision == A.MIXED else 8
         batch = A.Batch([], graph_steps=self.graph_steps)
         batch.set_shape(-1, self.grid)
@@ -80,20 +88,29 @@ class Server:
             done.append(rec)
             used -= request_bytes(r, pb)
 
-        while pending or waiting or active:
+        while pending or waiting or preparing or active:
             now = clock() - t0
             while pending and pending[0]["arrival"] <= now:
                 waiting.append(pending.pop(0))
-            # admission: FIFO while slots and memory allow; the request joins the running batch
-            fresh = []
-            while waiting and len(active) < self.max_slots and used + request_bytes(waiting[0], pb) <= self.budget:
+            # preparation: FIFO while slots and memory allow (the memory is reserved from here)
+            while (waiting and len(active) + len(preparing) < self.max_slots
+                   and used + request_bytes(waiting[0], pb) <= self.budget):
                 r = waiting.pop(0)
-                s = workloads.make_scheme(A, r, self.precision, self.scheme, self.cfl)
+                used += request_bytes(r, pb)
+                preparing.append((r, pool.submit(workloads.make_scheme, A, r, self.precision, self.scheme,
+                                                 self.cfl)))
+            # admission: prepared requests in arrival order join the running batch; with nothing to step,
+            # wait for the next one
+            if preparing and not active:
+                preparing[0][1].result()
+            fresh = []
+            while preparing and preparing[0][1].done():
+                r, fut = preparing.pop(0)
+                s = fut.result()
                 a = {"req": r, "scheme": s, "done": 0, "since": 0, "cap": r["steps"], "admit": clock() - t0}
                 batch.add(s)
                 active.append(a)
                 fresh.append(a)
-                used += request_bytes(r, pb)
             steady_new = [a["scheme"] for a in fresh if a["req"].get("mode") == "steady"]
             if steady_new:  # the start of their first window
                 batch.velocity_change(steady_new)
@@ -135,6 +152,7 @@ class Server:
                     retire(a, status, t_now)
             active = still
         batch.close()
+        pool.shutdown()
         t_end = clock() - t0
         tts = np.array([d["tts"] for d in done])
         work = sum(d["nodes"] * d["steps"] for d in done)
