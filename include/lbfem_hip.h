@@ -241,6 +241,16 @@ int lbfem_disc_stream_function(const lbfem_disc *disc, const double *u, const do
 int lbfem_stream_function(const lbfem_mesh_desc *desc, const double *u, const double *v, double *psi,
                           int *iterations);
 
+/* The six diagnostics sums of cgdbe (apps/cgdbe/cgdbe.cc:353-368) from the
+ * scheme's own populations (FP64 or mixed), on the device: the sums of
+ * lbfem_diagnostics with u_prev the velocity at the scheme's previous call
+ * (zero before the first) kept by the scheme. u_exact (optional, host, 2
+ * doubles per node) replaces the scheme's exact velocity field; the field used
+ * is exact_scale times the last one given (zero if none was): a separable field
+ * such as the Taylor-Green vortex's (test_cases.h:92-99) is given once and
+ * scaled by its decay factor. Synchronizes the scheme's stream. */
+int lbfem_scheme_diagnostics(lbfem_scheme *scheme, const double *u_exact, double exact_scale, double *sums);
+
 /* The velocity-change monitor of cgdbe's steady-state test (the diagnostics
  * of apps/cgdbe/cgdbe.cc:385-393) on the device, for FP64 and mixed schemes:
  * over the non-slave nodes, the largest |u - u_prev| and the lumped-mass mean

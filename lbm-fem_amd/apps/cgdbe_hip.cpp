@@ -55,7 +55,7 @@ namespace
     std::array<int, 2>  modes = {{1, 4}};
     std::vector<double> reynolds;
     unsigned            max_steps = 0, n_diagnostic = 20;
-    bool                output = true, fused = false;
+    bool                output = true, fused = false, stage_timers = false;
     std::string         checkpoint, restart;
 
     void
@@ -80,7 +80,8 @@ namespace
     "  --mach Ma [0.1]   --modes n1,n2 [1,4]   --reynolds Re[,Re...] [100 | 10 | 400]\n"
     "  --cfl c [0.25; cavity 0.4, with leelin 0.5]   --tend t [1; cavity 100]   --steps n\n"
     "  --stretch gamma [1.2]   --distort eps [0]   --steady-tol eps [1e-4]   --cg-tol eps [1e-8]   --fused\n"
-    "  --checkpoint name   --restart name   --precision fp64|mixed [fp64]   --no-output\n";
+    "  --checkpoint name   --restart name   --precision fp64|mixed [fp64]   --no-output\n"
+    "  --stage-timers (the summary's stage breakdown; GPU events at every stage, inside the timed run)\n";
 
   std::vector<double>
   split_doubles(const std::string &s)
@@ -156,6 +157,8 @@ namespace
           prm.precision = val();
         else if (a == "--no-output")
           prm.output = false;
+        else if (a == "--stage-timers")
+          prm.stage_timers = true;
         else
           throw std::runtime_error("unknown option " + a);
       }
@@ -365,7 +368,8 @@ namespace
                        prm.richardson, prm.cg_tol, 200, prm.fused ? 1 : 0};
       set.precision = prm.precision == "mixed" ? LBFEM_MIXED : LBFEM_FP64;
       check(lbfem_scheme_create(disc, &set, &scheme));
-      check(lbfem_scheme_enable_stage_timers(scheme, 1)); // StageTimers, always on as in the reference
+      // StageTimers (timers.h): on request only, their GPU events would time themselves inside the run
+      check(lbfem_scheme_enable_stage_timers(scheme, prm.stage_timers ? 1 : 0));
 
       // time step (cgdbe.cc:236-242): steps of the first Reynolds number; a transient hits t_end exactly
       flow.set_reynolds(prm.reynolds.front());
@@ -385,7 +389,6 @@ namespace
             f0[a * n + i] = fi[a];
         }
       check(lbfem_scheme_set_initial_populations(scheme, f0.data()));
-      u_previous.assign(n, {{0., 0.}});
       if (!prm.restart.empty())
         read_checkpoint(prm.restart);
 
@@ -507,27 +510,46 @@ namespace
       return {{rho, mx / rho, my / rho}};
     }
 
-    // cgdbe.cc:348-397: nodal quadrature over the non-slave nodes with lumped-mass weights
+    // the exact velocity of the flow at time t, 2 per node (host)
+    std::vector<double>
+    exact_velocity(const double t) const
+    {
+      std::vector<double> u(2 * n);
+      for (size_t i = 0; i < n; ++i)
+        {
+          const auto ex = flow.exact(xy[2 * i], xy[2 * i + 1], t);
+          u[2 * i]      = ex[1];
+          u[2 * i + 1]  = ex[2];
+        }
+      return u;
+    }
+
+    // cgdbe.cc:348-397: nodal quadrature over the non-slave nodes with lumped-mass weights, on the
+    // device (lbfem_scheme_diagnostics): the vortex's exact field is the t = 0 field times its decay
+    // factor, given once; the Couette series is given at each line; the cavity has none
     bool
     diagnostics()
     {
-      const auto f = populations();
-      double     s[6] = {0., 0., 0., 0., 0., 0.};
-      for (size_t i = 0; i < n; ++i)
+      double s[6] = {0., 0., 0., 0., 0., 0.};
+      if (flow.kind == "tgv")
         {
-          if (slave[i])
-            continue;
-          const auto   m  = moments(f, n, i);
-          const auto   ex = flow.exact(xy[2 * i], xy[2 * i + 1], time);
-          const double w  = ml[i];
-          s[0] += w * ((m[1] - ex[1]) * (m[1] - ex[1]) + (m[2] - ex[2]) * (m[2] - ex[2]));
-          s[1] += w * (ex[1] * ex[1] + ex[2] * ex[2]);
-          s[2] += w * (m[1] * m[1] + m[2] * m[2]);
-          s[3] += w * m[0];
-          s[4] += w;
-          s[5] += w * std::hypot(m[1] - u_previous[i][0], m[2] - u_previous[i][1]);
-          u_previous[i] = {{m[1], m[2]}};
+          const double F = std::exp(-flow.nu * (flow.k1() * flow.k1() + flow.k2() * flow.k2()) * time);
+          if (!exact_given)
+            {
+              const auto u0 = exact_velocity(0.);
+              check(lbfem_scheme_diagnostics(scheme, u0.data(), F, s));
+              exact_given = true;
+            }
+          else
+            check(lbfem_scheme_diagnostics(scheme, nullptr, F, s));
         }
+      else if (flow.kind == "couette")
+        {
+          const auto u = exact_velocity(time);
+          check(lbfem_scheme_diagnostics(scheme, u.data(), 1., s));
+        }
+      else
+        check(lbfem_scheme_diagnostics(scheme, nullptr, 0., s));
       const double mass_drift = s[3] / s[4] - flow.rho0;
       const double E0         = flow.energy_scale();
       auto         line       = [&](const std::array<double, 4> &v) {
@@ -669,6 +691,11 @@ namespace
       // collision, advection and the lumped solve in one kernel: its time is under advection.
       double t[4] = {0., 0., 0., 0.};
       check(lbfem_scheme_stage_times(scheme, t));
+      if (!prm.stage_timers)
+        {
+          std::cout << "  (the stage breakdown: run with --stage-timers)\n";
+          return;
+        }
       const bool   leelin = prm.scheme == "leelin";
       const double sweeps = prm.streaming == "tg3-split" ? 2. : 1.;
       const double gb     = 1e-9 * double(n) * 8. * total_steps;
@@ -721,7 +748,8 @@ namespace
     size_t                       n = 0;
     std::vector<double>          xy, ml;
     std::vector<int32_t>         slave;
-    std::vector<std::array<double, 2>> walls, u_previous;
+    std::vector<std::array<double, 2>> walls;
+    bool                               exact_given = false; // the vortex's t = 0 field is on the device
     std::ofstream                history;
     double                       dt = 0., t_ref = 1., time = 0., time_previous = 0., wall = 0.;
     unsigned                     n_steps = 0, step_no = 0, total_steps = 0;

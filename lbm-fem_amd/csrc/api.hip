@@ -481,6 +481,11 @@ struct lbfem_scheme
   DevBuf<double>        u_prev;
   bool                  u_prev_set = false;
   Monitor               monitor;
+  // cgdbe's diagnostics on the device: the exact velocity (2 per node, scaled per call), the
+  // velocity at the previous call (zero before the first), partial sums and the six sums
+  DevBuf<double>        diag_exact, diag_prev, diag_partial, diag_sums;
+  bool                  diag_exact_set = false;
+  double               *diag_host      = nullptr; // pinned, 6
   std::vector<std::array<double, 2 * Q>> sweeps;
   double                tg3         = 0.;
   bool                  extrapolate = false;
@@ -501,6 +506,8 @@ struct lbfem_scheme
       (void)hipHostFree(host_state);
     if (host_state32)
       (void)hipHostFree(host_state32);
+    if (diag_host)
+      (void)hipHostFree(diag_host);
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -2373,6 +2380,42 @@ extern "C"
       for (int a = 0; a < NM; ++a)
         its[sw * NM + a] = s->last_step_its[sw][a];
     return LBFEM_OK;
+  }
+
+  int
+  lbfem_scheme_diagnostics(lbfem_scheme *s, const double *u_exact, const double exact_scale, double *sums)
+  {
+    return guarded([&]() -> int {
+      if (!s || !sums)
+        return fail(LBFEM_EINVAL, "null argument");
+      const size_t n2 = 2 * size_t(s->d->n);
+      if (!s->diag_host)
+        {
+          RuntimeLock lock(runtime_mutex());
+          s->diag_exact.alloc(n2);
+          s->diag_prev.alloc(n2);
+          s->diag_partial.alloc(size_t(1024) * kCgPartials);
+          s->diag_sums.alloc(8);
+          HIP_CHECK(hipHostMalloc(&s->diag_host, 8 * sizeof(double)));
+          HIP_CHECK(hipMemsetAsync(s->diag_prev.p, 0, n2 * sizeof(double), s->stream));
+        }
+      if (u_exact)
+        {
+          HIP_CHECK(hipMemcpyAsync(s->diag_exact.p, u_exact, n2 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+          s->diag_exact_set = true;
+        }
+      const double *ex = s->diag_exact_set ? s->diag_exact.p : nullptr;
+      if (s->mixed)
+        HIP_CHECK(diagnostics_scaled_launch<float>(s->d->G, s->g32[s->cur].p, ex, exact_scale, s->diag_prev.p,
+                                                   s->diag_sums.p, s->diag_partial.p, 1024, s->stream));
+      else
+        HIP_CHECK(diagnostics_scaled_launch<double>(s->d->G, s->f[s->cur].p, ex, exact_scale, s->diag_prev.p,
+                                                    s->diag_sums.p, s->diag_partial.p, 1024, s->stream));
+      HIP_CHECK(hipMemcpyAsync(s->diag_host, s->diag_sums.p, 6 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      HIP_CHECK(hipStreamSynchronize(s->stream)); // also releases the caller's u_exact
+      std::copy(s->diag_host, s->diag_host + 6, sums);
+      return LBFEM_OK;
+    });
   }
 
   int

@@ -324,4 +324,9 @@ namespace lbfem_amd
   hipError_t stream_copy_launch(void *dst, const void *src, size_t bytes, hipStream_t s);
   hipError_t diagnostics_launch(const Geo &G, const double *f, const double *u_exact, double *u_prev, double *sums,
                                 double *partial, unsigned *counter, int max_blocks, hipStream_t stream);
+  // the same from FP64 populations or the mixed form's FP32 deviations, the exact velocity scaled by ex_scale
+  template <typename T>
+  hipError_t diagnostics_scaled_launch(const Geo &G, const T *f, const double *u_exact, double ex_scale,
+                                       double *u_prev, double *sums, double *partial, int max_blocks,
+                                       hipStream_t stream);
 } // namespace lbfem_amd

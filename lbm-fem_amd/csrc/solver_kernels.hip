@@ -761,9 +761,12 @@ namespace lbfem_amd
   }
 
   // ------------------------------------------------------------------ diagnostics
+  // T: double populations, or float deviations from the rest state (the mixed form);
+  // u_exact (optional) scaled by ex_scale
+  template <typename T>
   __global__ void __launch_bounds__(kThreads)
-  k_diagnostics(const Geo G, const double *f, const double *u_exact, double *u_prev, double *partial,
-                const int max_blocks)
+  k_diagnostics(const Geo G, const T *f, const double *u_exact, const double ex_scale, double *u_prev,
+                double *partial, const int max_blocks)
   {
     __shared__ double red[6][kThreads / 64];
     const int64_t     n = int64_t(G.np) * G.np;
@@ -773,14 +776,21 @@ namespace lbfem_amd
         const int i = int(k % G.np), j = int(k / G.np);
         if (is_slave(G, i, j))
           continue;
-        double v[Q];
+        T v[Q];
 #pragma unroll
         for (int a = 0; a < Q; ++a)
           v[a] = f[a * G.stride + k];
         double rho, ux, uy;
-        moments(v, rho, ux, uy);
+        if constexpr (sizeof(T) == 4)
+          {
+            double drho;
+            moments_dev(v, drho, rho, ux, uy);
+          }
+        else
+          moments(v, rho, ux, uy);
         const double w   = G.gen.ml ? G.gen.ml[k] : G.ax.ml[i] * G.ay.ml[j];
-        const double exu = u_exact ? u_exact[2 * k] : 0., exv = u_exact ? u_exact[2 * k + 1] : 0.;
+        const double exu = u_exact ? ex_scale * u_exact[2 * k] : 0.;
+        const double exv = u_exact ? ex_scale * u_exact[2 * k + 1] : 0.;
         s[0] += w * ((ux - exu) * (ux - exu) + (uy - exv) * (uy - exv));
         s[1] += w * (exu * exu + exv * exv);
         s[2] += w * (ux * ux + uy * uy);
@@ -1346,10 +1356,27 @@ namespace lbfem_amd
   {
     (void)counter;
     int nb = std::min(nodal_blocks(G), std::min(max_blocks, 1024));
-    hipLaunchKernelGGL(k_diagnostics, dim3(nb), dim3(kThreads), 0, s, G, f, u_exact, u_prev, partial, max_blocks);
+    hipLaunchKernelGGL(k_diagnostics<double>, dim3(nb), dim3(kThreads), 0, s, G, f, u_exact, 1., u_prev, partial,
+                       max_blocks);
     hipLaunchKernelGGL(k_diagnostics_finalize, dim3(1), dim3(kThreads), 0, s, partial, nb, max_blocks, sums);
     return hipGetLastError();
   }
+
+  template <typename T>
+  hipError_t
+  diagnostics_scaled_launch(const Geo &G, const T *f, const double *u_exact, const double ex_scale, double *u_prev,
+                            double *sums, double *partial, const int max_blocks, hipStream_t s)
+  {
+    int nb = std::min(nodal_blocks(G), std::min(max_blocks, 1024));
+    hipLaunchKernelGGL(k_diagnostics<T>, dim3(nb), dim3(kThreads), 0, s, G, f, u_exact, ex_scale, u_prev, partial,
+                       max_blocks);
+    hipLaunchKernelGGL(k_diagnostics_finalize, dim3(1), dim3(kThreads), 0, s, partial, nb, max_blocks, sums);
+    return hipGetLastError();
+  }
+  template hipError_t diagnostics_scaled_launch<double>(const Geo &, const double *, const double *, double,
+                                                        double *, double *, double *, int, hipStream_t);
+  template hipError_t diagnostics_scaled_launch<float>(const Geo &, const float *, const double *, double, double *,
+                                                       double *, double *, int, hipStream_t);
   // ---------------------------------------------------------------- HBM probe
   // A plain copy, four 16 B loads in flight per lane per pass over a
   // grid-strided range: the achievable-bandwidth reference and the known-size

@@ -116,6 +116,7 @@ def lib():
             "lbfem_batch_member_status": (i, [_vp, i32]),
             "lbfem_batch_velocity_change": (i, [_vp, i32, p, p, i64]),
             "lbfem_scheme_velocity_change": (i, [_vp, p, p, i64]),
+            "lbfem_scheme_diagnostics": (i, [_vp, p, d, p]),
             "lbfem_batch_destroy": (i, [_vp]),
             "lbfem_batch_set_graph_steps": (i, [_vp, i]),
             "lbfem_batch_step": (i, [_vp, i]),
@@ -366,6 +367,14 @@ class Scheme:
         v = np.zeros(16, dtype=np.int32)
         check(lib().lbfem_scheme_last_cg_iterations(self.h, v.ctypes.data_as(C.POINTER(C.c_int32))))
         return v.reshape(2, 8)
+
+    def diagnostics(self, u_exact=None, exact_scale=1.0):
+        """The six sums of cgdbe's diagnostics (apps/cgdbe/cgdbe.cc:353-368) from this scheme's populations on
+        the device, against exact_scale times the exact velocity last given (2 per node, host)."""
+        s = np.zeros(6)
+        ue = None if u_exact is None else np.ascontiguousarray(u_exact, dtype=np.float64).reshape(-1)
+        check(lib().lbfem_scheme_diagnostics(self.h, None if ue is None else _p(ue), float(exact_scale), _p(s)))
+        return s
 
     def velocity_change(self):
         """(max |u - u_prev|, lumped-mass mean |u - u_prev|, non-finite nodes) against the velocity at the

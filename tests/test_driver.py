@@ -199,7 +199,7 @@ def test_distorted_vortex_published_error(tmp_path):
     """--distort through the driver: docs/results.md:226-229, 48^2 Q1, modes (1, 4), Re 100, Ma 0.05,
     t = 0.25 t_ref, Bardow, CG 1e-8, TG2 at CFL 0.25 on the +-25 % mesh: rel. L2 error 2.07e-2."""
     r = run(["--case", "tgv", "--cells", "48", "--mach", "0.05", "--modes", "1,4", "--reynolds", "100",
-             "--tend", "0.25", "--distort", "0.04", "--no-output"], tmp_path)
+             "--tend", "0.25", "--distort", "0.04", "--no-output", "--stage-timers"], tmp_path)
     assert r.returncode == 0, r.stderr + r.stdout[-2000:]
     rows = [ln.split() for ln in r.stdout.splitlines()]
     rows = [x for x in rows if len(x) == 6 and x[0].isdigit()]

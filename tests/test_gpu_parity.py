@@ -264,3 +264,30 @@ def test_diagnostics_sums():
     so2 = od.diagnostics(gs.f, 20 * dt)
     assert np.allclose(s2[:5], so2[:5], rtol=1e-12)
     assert not np.allclose(s2[:5], s[:5], rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", [A.FP64, A.MIXED])
+def test_scheme_diagnostics_matches_the_oracle(precision):
+    """lbfem_scheme_diagnostics (the sums from the scheme's own populations, FP64 or mixed, on the device)
+    against the oracle's diagnostics of the same populations; the exact field scaled by a decay factor; the
+    previous velocity kept by the scheme (zero before the first call)."""
+    tc, gd, od = pair("tgv", 16, 2)
+    dt, lam = od.time_step(0.25), od.relaxation_time
+    gs = A.Bardow(gd, mass=A.MASS_LUMPED, precision=precision)
+    gs.set_time_step(dt, lam)
+    gs.set_initial_populations()
+    xy = gd.node
+    _, u0, v0 = tc.exact(xy[:, 0], xy[:, 1], 0.0)
+    prev = np.zeros((gd.n_nodes, 2))
+    for k, t in ((0, 0.0), (7, 7 * dt), (12, 12 * dt)):
+        gs.step(k if k == 0 else k - (7 if k == 12 else 0))
+        F = np.exp(-tc.nu * (tc.k1 ** 2 + tc.k2 ** 2) * t)
+        s = gs.diagnostics(np.stack([u0, v0], 1) if t == 0.0 else None, F)
+        f = gs.f
+        so = od.diagnostics(f, t, u_prev=prev)
+        assert np.allclose(s[1:], so[1:], rtol=1e-11, atol=1e-300), (t, s, so)
+        # the error against the exact field: at t = 0 it is round-off (FP64) or the FP32 storage (mixed), so
+        # its norm, not its square, is compared, with an absolute tolerance
+        assert abs(np.sqrt(s[0] / s[1]) - np.sqrt(so[0] / so[1])) < 1e-13, (t, s, so)
+        _, u, v = A.moments(f)
+        prev = np.stack([u, v], 1)
