@@ -171,11 +171,11 @@ struct lbfem_disc
   bool                     general = false;
   std::vector<double>      gen_xy, gen_ml, gen_diag;
   DevBuf<double>           gen_buf;
+  DevBuf<double>           dinv_buf; // the tensor mesh's Jacobi diagonal per node (Geo::dinv)
   // scratch of the operator-level entry points (the mass solve and the stream function),
   // allocated on first use and kept: one call at a time per discretization
-  DevBuf<CgState>  cg_state;
+  DevBuf<CgState>  cg_state; // two sets of NM: the stopping states of odd and even iterations
   DevBuf<double>   partial, s_r, s_p0, s_p1, s_q, s_ax, s_b, s_dinv, s_sums;
-  DevBuf<unsigned> counter;
   CgState         *host_cg = nullptr; // pinned: the CG stopping states read by the host
 
   lbfem_disc() = default;
@@ -226,6 +226,28 @@ struct lbfem_disc
   // and the lumped mass (row sums of M) and diag(M) from QGauss(p+1) with the
   // Jacobian per point, summed per node over its cells (lower row first, left
   // cell first); constrained entries 1 (discretization.h:123-125).
+  // Geo::dinv: the general cells' per-node inverse diagonal, or on the tensor mesh the
+  // product of the axes' inverse diagonals (the same double product the kernels formed per
+  // node), 1 on slave rows (MatrixFreeOperators' constrained diagonal entries)
+  void
+  build_jacobi()
+  {
+    if (general)
+      {
+        G.dinv = G.gen.diag_inv;
+        return;
+      }
+    std::vector<double> h(static_cast<size_t>(n));
+    for (int j = 0; j < np; ++j)
+      for (int i = 0; i < np; ++i)
+        {
+          const bool slave       = (ax->periodic && i == P) || (ay->periodic && j == P);
+          h[size_t(j) * np + i] = slave ? 1. : (1. / ax->diagonal[i]) * (1. / ay->diagonal[j]);
+        }
+    dinv_buf.upload(h);
+    G.dinv = dinv_buf.p;
+  }
+
   void
   build_general(const double eps)
   {
@@ -372,13 +394,11 @@ struct lbfem_disc
     RuntimeLock lock(runtime_mutex());
     if (!host_cg)
       HIP_CHECK(hipHostMalloc(&host_cg, sizeof(CgState) * NM));
-    if (counter.p) // allocated last: the rest is there
+    if (s_sums.p) // allocated last: the rest is there
       return;
-    cg_state.alloc(NM);
+    cg_state.alloc(2 * NM);
     partial.alloc(size_t(3 * NM) * kMaxCgBlocks);
     s_sums.alloc(6); // the diagnostics' sums
-    counter.alloc(1);
-    HIP_CHECK(hipMemset(counter.p, 0, sizeof(unsigned)));
   }
 
   // the node vectors of the operator-level CG (r, two directions, A p, A x) and of the
@@ -469,11 +489,10 @@ struct lbfem_scheme
   DevBuf<double>        rhs, x[2], r, pb[2], q, ax_tmp;
   // the mixed form's consistent mass: FP32 right-hand side and inner CG vectors
   DevBuf<float>         rhs32, g32r, r32, p32[2], q32, d32;
-  DevBuf<CgState>       state32;
+  DevBuf<CgState>       state32; // two sets of NM, as state
   CgState              *host_state32 = nullptr; // pinned
-  DevBuf<CgState>       state;
+  DevBuf<CgState>       state; // two sets of NM: the stopping states of odd and even iterations
   DevBuf<double>        partial;
-  DevBuf<unsigned>      counter;
   CgState              *host_state = nullptr; // pinned
   int64_t               cg_iterations = 0;
   int                   last_step_its[2][NM] = {}; // the last step's CG steps per sweep and moving population
@@ -590,14 +609,14 @@ namespace
   }
 
   CgWork
-  cg_work(const lbfem_disc &d, CgState *state, double *partial, unsigned *counter, const lbfem_mass_settings &m)
+  cg_work(const lbfem_disc &d, CgState *state, CgState *state2, double *partial, const lbfem_mass_settings &m)
   {
     CgWork W{};
     W.max_its    = m.cg_max_iterations;
     W.tol_rel    = m.cg_tolerance;
     W.state      = state;
+    W.state2     = state2;
     W.partial    = partial;
-    W.counter    = counter;
     W.max_blocks = kMaxCgBlocks;
     (void)d;
     return W;
@@ -615,15 +634,22 @@ namespace
          const int check_every = 1)
   {
     CgWork Wf = W;
-    // small grids: the update kernel reduces the direction stage's partials itself (one
-    // launch less per iteration); large ones keep the separate one-block finalisation
+    // small grids: the update kernel reduces the direction stage's partials itself, and the
+    // direction kernel the update stage's (two launches less per iteration; with two sets of
+    // states, tensor meshes); large ones keep the separate one-block finalisations
     const int rows = cg_direction_rows(G);
     Wf.alpha_rows  = int64_t(rows) * W.nv <= 4096 && rows + 1024 <= W.max_blocks ? rows : 0;
+    const int upd  = cg_update_blocks(G, W.max_blocks, Wf.alpha_rows);
+    Wf.fold_rows   = Wf.alpha_rows > 0 && upd <= kMaxCgFoldRows && W.state2 && G.gen.vert == nullptr ? upd : 0;
     HIP_RETURN(cg_start_launch(G, W, x, b, r, stream));
     V         *pb[2] = {p0, p1};
-    const auto done  = [&](bool &finished) -> int {
+    // the host's read of the states after iteration it (with the fold, they are formed first)
+    const auto done = [&](const int it, bool &finished) -> int {
       finished = true;
-      HIP_RETURN(hipMemcpyAsync(host_state, W.state, W.nv * sizeof(CgState), hipMemcpyDeviceToHost, stream));
+      if (Wf.fold_rows > 0 && it > 0)
+        HIP_RETURN(cg_update_finalize_launch(G, Wf, it, stream));
+      const CgState *src = Wf.fold_rows > 0 && (it & 1) ? W.state2 : W.state;
+      HIP_RETURN(hipMemcpyAsync(host_state, src, W.nv * sizeof(CgState), hipMemcpyDeviceToHost, stream));
       HIP_RETURN(hipStreamSynchronize(stream));
       bool active = false, failed = false;
       for (int v = 0; v < W.nv; ++v)
@@ -653,13 +679,13 @@ namespace
         if (it >= check_from && (it - check_from) % check_every == 0)
           {
             bool      finished = false;
-            const int st       = done(finished);
+            const int st       = done(it, finished);
             if (finished)
               return st;
           }
       }
     bool finished = false;
-    return done(finished); // the update at max_its leaves no vector active
+    return done(std::max(W.max_its, 0), finished); // the update at max_its leaves no vector active
   }
 
   // the increments of one sweep: advection, then the mass solves (streaming.h:66-84)
@@ -696,7 +722,7 @@ namespace
     else if (type == LBFEM_MASS_RICHARDSON) // mass.h:193-202, x_0 = M_L^-1 r, x += M_L^-1 (r - A x)
       {
         HIP_CHECK(lumped_solve_launch(G, xs, s.rhs.p, NM, s.stream));
-        CgWork W = cg_work(d, s.state.p, s.partial.p, s.counter.p, s.set.mass);
+        CgWork W = cg_work(d, s.state.p, nullptr, s.partial.p, s.set.mass);
         W.nv     = NM;
         for (int a = 0; a < NM; ++a)
           {
@@ -715,7 +741,7 @@ namespace
       {
         if (s.extrapolate)
           HIP_CHECK(extrapolate_launch(G, xs, s.x[1].p, mask, s.stream));
-        CgWork W = cg_work(d, s.state.p, s.partial.p, s.counter.p, s.set.mass);
+        CgWork W = cg_work(d, s.state.p, s.state.p + NM, s.partial.p, s.set.mass);
         W.nv     = 0;
         for (int a = 0; a < NM; ++a)
           if (mask & (1u << a))
@@ -767,6 +793,7 @@ namespace
         HIP_CHECK(hipStreamSynchronize(s.stream));
         CgWork W32  = W64;
         W32.state   = s.state32.p;
+        W32.state2  = s.state32.p + NM;
         W32.nv      = 0;
         W32.tol_rel = 1.;
         unsigned mask = 0;
@@ -832,7 +859,7 @@ namespace
     double *xs = s.x[sw].p;
     if (s.extrapolate)
       HIP_CHECK(extrapolate_launch(G, xs, s.x[1].p, mask, s.stream));
-    CgWork W = cg_work(d, s.state.p, s.partial.p, s.counter.p, s.set.mass);
+    CgWork W = cg_work(d, s.state.p, s.state.p + NM, s.partial.p, s.set.mass);
     W.nv     = 0;
     for (int a = 0; a < NM; ++a)
       if (mask & (1u << a))
@@ -1308,7 +1335,7 @@ namespace
             E[k].tile_begin = tiles;
             tiles += E[k].S.tiles_x * E[k].S.tiles_y;
             // the solves (mass.h:204-211), as increment() sets them up for the scheme alone
-            CgWork W = cg_work(*s.d, g.states.p + size_t(k) * NM, s.partial.p, nullptr, s.set.mass);
+            CgWork W = cg_work(*s.d, g.states.p + size_t(k) * NM, nullptr, s.partial.p, s.set.mass);
             W.nv     = 0;
             for (int a = 0; a < NM; ++a)
               if (mask & (1u << a))
@@ -1653,11 +1680,11 @@ namespace
 {
   constexpr int64_t kNoAllocMaxStride    = 1179648; // nodes (a multiple of 64): 1024^2 Q1 and 512^2 Q2 cells fit
   constexpr int     kNoAllocRows         = 8192;    // partial-sum rows: operator tiles, then at most 1024 update blocks
-  constexpr size_t  kNoAllocStateDoubles = 16;      // the CgState of the one vector (56 bytes)
+  constexpr size_t  kNoAllocStateDoubles = 16;      // the CgStates of the one vector (56 bytes each, 64 apart)
   constexpr size_t  kNoAllocDoubles =
     kNoAllocStateDoubles + size_t(kNoAllocRows) * kCgPartials + 4 * size_t(kNoAllocMaxStride);
   constexpr int kNoAllocMaxDevices = 64;
-  static_assert(sizeof(CgState) <= kNoAllocStateDoubles * sizeof(double), "CG state slot");
+  static_assert(sizeof(CgState) <= 8 * sizeof(double), "CG state slot");
 
   std::mutex g_scratch_mutex;
   double    *g_scratch_ptr[kNoAllocMaxDevices] = {}; // the array's address on each device, once taken
@@ -1756,6 +1783,7 @@ extern "C"
       d->G.gen            = GeneralDev{};
       if (desc->distort != 0.)
         d->build_general(desc->distort);
+      d->build_jacobi();
       *out = d.release();
       return LBFEM_OK;
     });
@@ -2010,7 +2038,7 @@ extern "C"
         return fail(LBFEM_EINVAL, "unknown mass type");
       d->ensure_scratch();
       d->ensure_vectors();
-      CgWork W      = cg_work(*d, d->cg_state.p, d->partial.p, d->counter.p, *m);
+      CgWork W      = cg_work(*d, d->cg_state.p, d->cg_state.p + NM, d->partial.p, *m);
       W.nv          = 1;
       W.block[0]    = 0;
       W.vec[0].coef = tg3_coefficient;
@@ -2075,7 +2103,7 @@ extern "C"
       double      *partial = base + kNoAllocStateDoubles;
       double      *vec     = partial + size_t(kNoAllocRows) * kCgPartials;
       const size_t sd      = size_t(d->stride);
-      CgWork       W       = cg_work(*d, state, partial, nullptr, *m);
+      CgWork W = cg_work(*d, state, reinterpret_cast<CgState *>(base + 8), partial, *m);
       W.max_blocks         = kNoAllocRows;
       W.nv                 = 1;
       W.block[0]           = 0;
@@ -2178,10 +2206,8 @@ extern "C"
         s->ax_tmp.alloc(NM * st);
       if (m.type != LBFEM_MASS_LUMPED)
         {
-          s->state.alloc(NM);
+          s->state.alloc(2 * NM);
           s->partial.alloc(size_t(3 * NM) * kMaxCgBlocks);
-          s->counter.alloc(1);
-          HIP_CHECK(hipMemset(s->counter.p, 0, sizeof(unsigned)));
         }
       if (m.type == LBFEM_MASS_CG)
         {
@@ -2192,7 +2218,7 @@ extern "C"
               for (DevBuf<float> *b : {&s->rhs32, &s->g32r, &s->r32, &s->p32[0], &s->p32[1], &s->q32, &s->d32})
                 b->alloc(NM * st);
               HIP_CHECK(hipMemset(s->d32.p, 0, NM * st * sizeof(float)));
-              s->state32.alloc(NM);
+              s->state32.alloc(2 * NM);
               HIP_CHECK(hipHostMalloc(&s->host_state32, sizeof(CgState) * NM));
             }
           else
@@ -2475,7 +2501,7 @@ extern "C"
       d->ensure_scratch();
       hipStream_t st = as_stream(stream);
       HIP_CHECK(
-        diagnostics_launch(d->G, f, u_exact, u_prev, d->s_sums.p, d->partial.p, d->counter.p, kMaxCgBlocks, st));
+        diagnostics_launch(d->G, f, u_exact, u_prev, d->s_sums.p, d->partial.p, kMaxCgBlocks, st));
       HIP_CHECK(hipMemcpyAsync(sums, d->s_sums.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipStreamSynchronize(st));
       return LBFEM_OK;
@@ -2988,7 +3014,7 @@ extern "C"
       d->ensure_scratch();
       d->ensure_vectors();
       Geo G          = d->G;
-      G.gen.diag_inv = d->s_dinv.p; // the Jacobi preconditioner of the Laplacian
+      G.dinv         = d->s_dinv.p; // the Jacobi preconditioner of the Laplacian
       HIP_CHECK(psi_setup_launch(G, u, v, d->s_b.p, d->s_dinv.p, st));
       HIP_CHECK(hipMemsetAsync(psi, 0, sizeof(double) * size_t(d->n), st));
       CgWork W{};
@@ -2997,8 +3023,8 @@ extern "C"
       W.max_its    = 10000;
       W.tol_rel    = 1e-12;
       W.state      = d->cg_state.p;
+      W.state2     = d->cg_state.p + NM;
       W.partial    = d->partial.p;
-      W.counter    = d->counter.p;
       W.max_blocks = kMaxCgBlocks;
       W.laplace    = 1;
       int64_t   its = 0;

@@ -74,6 +74,10 @@ namespace lbfem_amd
     int        p, P, np; // degree, p N, p N + 1
     int64_t    stride;
     GeneralDev gen;
+    // the Jacobi preconditioner of the CG solves per node: the inverse of diag(M) (the axes'
+    // product, or the general cells' per-node diagonal; 1 on constrained rows), or the
+    // stream function's Laplacian diagonal
+    const double *dinv;
   };
 
   // sum over a block of NT threads of up to K values per thread; the result is
@@ -104,6 +108,36 @@ namespace lbfem_amd
             s += lds[k][w];
           v[k] = s;
         }
+    __syncthreads();
+  }
+
+  // block_sum_n with the result in every thread (each sums the wavefronts' partials in the
+  // same order: thread 0's value, bit for bit, with one barrier less)
+  template <int K, int NT>
+  __device__ __forceinline__ void
+  block_sum_all(double (&v)[K], double (*lds)[NT / 64])
+  {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      {
+        double s = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+          s += __shfl_down(s, off, 64);
+        if (lane == 0)
+          lds[k][wave] = s;
+      }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      {
+        double s = 0.;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w)
+          s += lds[k][w];
+        v[k] = s;
+      }
     __syncthreads();
   }
 

@@ -188,18 +188,29 @@ namespace lbfem_amd
     int       max_its;
     double    tol_rel;
     CgVector  vec[NM];
-    CgState  *state;      // device [nv]
+    // device [nv]: the stopping states; with fold_rows > 0 those of even iterations, and
+    // state2 [nv] those of odd ones (the direction stage writes iteration it - 1's while its
+    // blocks read iteration it - 2's)
+    CgState  *state;
+    CgState  *state2;
     double   *partial;    // device [nv * 4 * max_blocks]
-    unsigned *counter;    // device, zero between uses
     int       max_blocks;
     // > 0: the direction stage's partial rows, which the update kernel reduces itself
     // (every block, in the order the finalisation uses) instead of a separate launch
     int       alpha_rows;
+    // > 0 (with alpha_rows > 0 and state2): the update stage's partial rows (after the
+    // direction stage's), which the next direction stage reduces itself (every block, in the
+    // order the finalisation uses) for beta and the stopping test, instead of a separate
+    // launch; the finalisation then runs only where the host reads the states
+    int       fold_rows;
     // 1: the stream function's Laplacian Kx (x) My + Mx (x) Ky on a walled box instead of
     // M + c K_e: wall entries read as zero, wall rows the identity; the Jacobi diagonal
-    // comes from Geo::gen.diag_inv
+    // comes from Geo::dinv
     int       laplace;
   };
+
+  // the most update-stage rows the direction stage reduces itself (CgWork::fold_rows)
+  constexpr int kMaxCgFoldRows = 256;
 
   // partial rows the CG direction stage writes (its blocks per vector)
   int cg_direction_rows(const Geo &G);
@@ -246,9 +257,12 @@ namespace lbfem_amd
   // p_new = beta p_old - D^-1 r (p_new = -D^-1 r at it == 1); q = A p_new; alpha = rho / p.q
   hipError_t cg_direction_launch(const Geo &G, const CgWork &W, int it, const double *p_old, const double *r,
                                  double *p_new, double *q, hipStream_t stream);
-  // x += alpha p; r += alpha q; res = |r|; stopping test; rho = r.D^-1 r, beta
+  // x += alpha p; r += alpha q; res = |r|; stopping test; rho = r.D^-1 r, beta (with
+  // W.fold_rows > 0 the last three are formed by the next direction stage, or by
+  // cg_update_finalize_launch where the host reads the states)
   hipError_t cg_update_launch(const Geo &G, const CgWork &W, int it, double *x, double *r, const double *p,
                               const double *q, hipStream_t stream);
+  hipError_t cg_update_finalize_launch(const Geo &G, const CgWork &W, int it, hipStream_t stream);
   // the same three stages on FP32 vectors (the mixed form's inner solves; scalars in FP64)
   hipError_t cg_start_launch(const Geo &G, const CgWork &W, const float *x, const float *b, float *r,
                              hipStream_t stream);
@@ -323,7 +337,7 @@ namespace lbfem_amd
   // diagnostics sums over non-slave nodes (cgdbe.cc:353-368)
   hipError_t stream_copy_launch(void *dst, const void *src, size_t bytes, hipStream_t s);
   hipError_t diagnostics_launch(const Geo &G, const double *f, const double *u_exact, double *u_prev, double *sums,
-                                double *partial, unsigned *counter, int max_blocks, hipStream_t stream);
+                                double *partial, int max_blocks, hipStream_t stream);
   // the same from FP64 populations or the mixed form's FP32 deviations, the exact velocity scaled by ex_scale
   template <typename T>
   hipError_t diagnostics_scaled_launch(const Geo &G, const T *f, const double *u_exact, double ex_scale,

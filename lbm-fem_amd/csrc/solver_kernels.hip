@@ -10,8 +10,11 @@
 // takes exactly the iterations its own solve would.
 //
 // Reductions: wavefront shuffles and LDS inside a block, one partial per block,
-// and the last block to finish sums the partials in block order (a fixed
-// order, so results do not depend on scheduling) and updates the scalars.
+// then the partials summed in block order (a fixed order, so results do not
+// depend on scheduling) and the scalars updated: by a one-block-per-vector
+// finalisation launch after the start and direction stages, by the last
+// update block of each vector to finish (counted with an atomic) after the
+// update stage.
 #include <algorithm>
 #include <type_traits>
 
@@ -46,14 +49,12 @@ namespace lbfem_amd
       return (G.ax.periodic && i == G.P) || (G.ay.periodic && j == G.P);
     }
 
-    // the Jacobi preconditioner: the inverse of diag(M) = diag(Mx) diag(My), 1 on
+    // the Jacobi preconditioner: the inverse of diag(M) = diag(Mx) diag(My) per node, 1 on
     // constrained rows, applied as a product (DiagonalMatrix::vmult of the inverse entries)
     __device__ __forceinline__ double
     mass_diag_inv(const Geo &G, const int i, const int j)
     {
-      if (G.gen.diag_inv) // general cells: per node
-        return G.gen.diag_inv[node_index(G, i, j)];
-      return is_slave(G, i, j) ? 1. : G.ax.diag_inv[i] * G.ay.diag_inv[j];
+      return G.dinv[node_index(G, i, j)];
     }
 
     // the inverse lumped mass, 1 on constrained rows
@@ -219,6 +220,57 @@ namespace lbfem_amd
       }
   }
 
+  // the stopping states after iteration k (0: the start)
+  __device__ __forceinline__ CgState *
+  states_of(const CgWork &W, const int k)
+  {
+    return (W.fold_rows > 0 && (k & 1)) ? W.state2 : W.state;
+  }
+
+  // A stopping state read across a wavefront's lanes, lane l its l-th 8-byte word: a vector
+  // load, issued with the vector loads around it (a scalar load of it is issued late, at its
+  // first use), the fields taken from the lanes once it has arrived
+  constexpr int kStateWords = int(sizeof(CgState) / sizeof(double));
+  static_assert(sizeof(CgState) == kStateWords * sizeof(double), "CgState in 8-byte words");
+
+  __device__ __forceinline__ double
+  state_word(const CgState *s)
+  {
+    return reinterpret_cast<const double *>(s)[std::min(int(threadIdx.x & 63), kStateWords - 1)];
+  }
+
+  __device__ __forceinline__ CgState
+  state_from_lanes(const double w)
+  {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, w);
+    const int      lo = int(uint32_t(bits)), hi = int(uint32_t(bits >> 32));
+    uint64_t       words[kStateWords];
+#pragma unroll
+    for (int f = 0; f < kStateWords; ++f)
+      words[f] = uint64_t(uint32_t(__builtin_amdgcn_readlane(lo, f))) |
+                 (uint64_t(uint32_t(__builtin_amdgcn_readlane(hi, f))) << 32);
+    return __builtin_bit_cast(CgState, words);
+  }
+
+  // SolverCG's stopping test (SolverControl::check: success first, then failure at the
+  // cap) and beta after iteration it, from |r|^2 and r.D^-1 r; a stopped vector keeps its state
+  __device__ __forceinline__ CgState
+  advance_state(CgState st, const double rr, const double rz, const int it, const int max_its)
+  {
+    if (st.status != 0)
+      return st;
+    st.res = sqrt(fabs(rr));
+    st.its = it;
+    if (st.res <= st.tol)
+      st.status = 1;
+    else if (it >= max_its || isnan(st.res))
+      st.status = 2;
+    st.rho_prev = st.rho;
+    st.rho      = rz;
+    st.beta     = st.rho / st.rho_prev;
+    return st;
+  }
+
   // ------------------------------------------------------------------ operator tile
   // KIND 0: y = A x.  KIND 1 (CG start): r = A x - b, partials |r|^2, |b|^2, r.D^-1 r.
   // KIND 2 (CG direction): p_new = beta p_old - D^-1 r, q = A p_new, partial p_new.q.
@@ -226,7 +278,7 @@ namespace lbfem_amd
   // OP kOpStreaming: A = M + c K_e = Mx (My + c ey^2 Ky) + Kx (c ex^2 My) + Dx^T (c ex ey Dy)
   //   + Dx (c ex ey Dy^T) (mass.h:105-144; a vector with c = 0 or e = 0 takes the mass branch).
   // OP kOpLaplace: the stream function's A = Kx (x) My + Mx (x) Ky on a walled box (io.h:55-85):
-  //   wall entries read as zero, wall rows the identity, the Jacobi diagonal from gen.diag_inv.
+  //   wall entries read as zero, wall rows the identity, the Jacobi diagonal from Geo::dinv.
   // A thread keeps only the 1-D bands its OP uses (registers set the occupancy).
   // One vector per block (blockIdx.y): its operand tile is staged in LDS (one barrier), then
   // each thread applies the bands at its RO owned nodes of one column.
@@ -269,66 +321,174 @@ namespace lbfem_amd
     __shared__ V      X[HY][HX];
     __shared__ double red[3][kThreads / 64];
 
-    const int nt   = tiles_x * tiles_y;
-    const int nb8  = (nt + 7) & ~7;
-    const int tile = (bid & 7) * (nb8 >> 3) + (bid >> 3);
-    if (tile >= nt)
-      return;
-    const int i0 = (tile % tiles_x) * TX, j0 = (tile / tiles_x) * TY;
-    const int P = G.P, tid = threadIdx.x;
+    // No branch before the loads below (a branch on a kernel argument waits for it, and the
+    // argument loads after it wait in turn: a chain of round trips): a padding block loads
+    // the last tile's operand and leaves after the loads. The kernel arguments the loads
+    // need are made available together first: their scalar loads are issued before the
+    // statement (a zero into `early`, which the thread index takes on, so it stays ahead of
+    // the loads; no memory effect, so uniform loads after it may still be scalar) and waited
+    // for once, not each behind the previous one's wait at its first use.
+    int early;
+    asm("s_mov_b32 %0, 0"
+        : "=s"(early)
+        : "s"(tiles_x), "s"(tiles_y), "s"(it), "s"(G.P), "s"(G.np), "s"(G.stride), "s"(G.ax.periodic),
+          "s"(G.ay.periodic), "s"(W.block[v]), "s"(x), "s"(r_in), "s"(G.dinv), "s"(W.state), "s"(W.state2),
+          "s"(W.fold_rows), "s"(W.alpha_rows), "s"(W.partial), "s"(band_array(G.ax, which[0])),
+          "s"(band_array(G.ay, which[0])), "s"(band_array(G.ax, which[B::NB - 1])),
+          "s"(band_array(G.ay, which[B::NB - 1])), "s"(y), "s"(y2), "s"(b));
+    const int  nt   = tiles_x * tiles_y;
+    const int  nb8  = (nt + 7) & ~7;
+    const int  tile = (bid & 7) * (nb8 >> 3) + (bid >> 3);
+    const bool live = tile < nt;
+    const int  tl   = live ? tile : nt - 1;
+    const int  i0 = (tl % tiles_x) * TX, j0 = (tl / tiles_x) * TY;
+    const int  P = G.P, tid = int(threadIdx.x) + early;
     const auto wall = [P](const int i, const int j) { return i == 0 || j == 0 || i == P || j == P; };
+    // the staged node of halo coordinate i, -1 outside: walled, outside [0, P]; periodic, the
+    // image in [0, P) of i in [-p, P + p] (a halo further out, past the last tile's nodes,
+    // is never read and stages zero); selects, not branches
+    const auto halo = [P](const int i, const int periodic) {
+      int w = i < 0 ? i + P : i;
+      w     = w >= P ? w - P : w;
+      w     = w >= P ? w - P : w; // one cell of degree p: P = p
+      const int per = i > P + p ? -1 : w;
+      const int wal = (i < 0 || i > P) ? -1 : i;
+      return periodic ? per : wal;
+    };
 
-    // this block's vector (uniform over the block); a converged one is skipped
-    if (KIND == 2 && W.state[v].status != 0)
+    const int64_t vb = int64_t(W.block[v]) * G.stride;
+    // Stage the operand (KIND 2: the new direction) of the halo'd block. A thread's NS slots
+    // are loaded together, with the vector's state, before anything waits on them: one
+    // memory round trip instead of one per slot and one for the state (this kernel's time
+    // at small meshes is its chain of round trips, not its bytes). A converged vector's
+    // loads are in bounds and unused.
+    constexpr int NS = (NH + kThreads - 1) / kThreads;
+    // KIND 2: the stopping state after iteration it - 1, read, or (fold) formed here from
+    // iteration it - 2's and the update stage's partial rows of iteration it - 1, summed as
+    // the finalisation sums them (thread t takes rows t, t + 256, ..., then the block)
+    constexpr int kFoldPer = kMaxCgFoldRows / kThreads; // update rows per thread
+    const bool    fold     = KIND == 2 && W.fold_rows > 0 && it > 1;
+    const double  st_word  = KIND == 2 ? state_word(states_of(W, fold ? it - 2 : it - 1) + v) : 0.;
+    V             sx[NS], sr[NS];
+    double        sd[NS];
+    unsigned      in_mesh = 0; // per slot: holds a node's value (the rest stage zero)
+    // Every slot loads (an in-mesh address, its value discarded where the slot is outside the
+    // mesh): loads under divergent branches, each joined before the next, were what kept
+    // them in a chain.
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      {
+        const int k  = std::min(tid + s * kThreads, NH - 1);
+        const int kx = k % HX, ky = k / HX;
+        int       gi = halo(i0 - p + kx, G.ax.periodic), gj = halo(j0 - p + ky, G.ay.periodic);
+        // the Laplacian's wall entries are constrained to zero (read as zero by the cells)
+        const bool ok = tid + s * kThreads < NH && gi >= 0 && gj >= 0 && !(LAP && wall(gi, gj));
+        in_mesh |= unsigned(ok) << s;
+        gi = std::max(gi, 0);
+        gj = std::max(gj, 0);
+        const int64_t idx = node_index(G, gi, gj);
+        // KIND 2 at it == 1 has no old direction: the buffer is read (it is there) and its
+        // values unused
+        sx[s] = x[vb + idx];
+        sr[s] = V(0);
+        sd[s] = 1.;
+        if constexpr (KIND == 2)
+          {
+            sr[s] = r_in[vb + idx];
+            sd[s] = G.dinv[idx];
+          }
+      }
+    const int ii = tid % TX, jr = (tid / TX) * RO;
+    const int gi = i0 + ii, gic = std::min(gi, P);
+    V         xc[B::NB][W_], yc[RO][B::NB][W_];
+#pragma unroll
+    for (int o = 0; o < W_; ++o)
+      {
+#pragma unroll
+        for (int t = 0; t < B::NB; ++t)
+          xc[t][o] = V(band<p>(band_array(G.ax, which[t]), gic, o - p));
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+#pragma unroll
+          for (int t = 0; t < B::NB; ++t)
+            yc[r][t][o] = V(band<p>(band_array(G.ay, which[t]), std::min(j0 + jr + r, P), o - p));
+      }
+    // the update stage's rows last: waiting for them is waiting for every load above
+    double ua[kFoldPer], ub[kFoldPer];
+#pragma unroll
+    for (int j = 0; j < kFoldPer; ++j)
+      ua[j] = ub[j] = 0.;
+    if constexpr (KIND == 2)
+#pragma unroll
+      for (int j = 0; j < kFoldPer; ++j)
+        {
+          const int     bb  = tid + j * kThreads;
+          const int     rc  = std::max(std::min(bb, W.fold_rows - 1), 0); // a row that is there
+          const double *row = W.partial + size_t(W.alpha_rows + rc) * kPart + 2 * v;
+          const double  a = row[0], b = row[1];
+          ua[j]           = fold && bb < W.fold_rows ? a : 0.;
+          ub[j]           = fold && bb < W.fold_rows ? b : 0.;
+        }
+    // rows and columns past the mesh hold zero bands
+#pragma unroll
+    for (int o = 0; o < W_; ++o)
+      {
+#pragma unroll
+        for (int t = 0; t < B::NB; ++t)
+          xc[t][o] = gi <= P ? xc[t][o] : V(0);
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+#pragma unroll
+          for (int t = 0; t < B::NB; ++t)
+            yc[r][t][o] = j0 + jr + r <= P ? yc[r][t][o] : V(0);
+      }
+    if (!live)
+      return;
+    CgState st{};
+    if constexpr (KIND == 2)
+      {
+        // reduced whether or not it is used, and the state formed in every thread (the loads
+        // above stay unconditional and together that way)
+        double u[2] = {0., 0.};
+#pragma unroll
+        for (int j = 0; j < kFoldPer; ++j)
+          {
+            u[0] += ua[j];
+            u[1] += ub[j];
+          }
+        block_sum_all<2, kThreads>(u, red);
+        st = state_from_lanes(st_word);
+        if (fold)
+          {
+            st = advance_state(st, u[0], u[1], it - 1, W.max_its);
+            if (tile == 0 && tid == 0) // one block per vector keeps it (the others read iteration it - 2's)
+              states_of(W, it - 1)[v] = st;
+          }
+      }
+    if (KIND == 2 && st.status != 0)
       {
         if (tid == 0)
           W.partial[size_t(tile) * kPart + 3 * v] = 0.;
         return;
       }
-    const V       beta = (KIND == 2 && it > 1) ? V(W.state[v].beta) : V(0);
-    const int64_t vb   = int64_t(W.block[v]) * G.stride;
-
-    // stage the operand (KIND 2: the new direction) of the halo'd block
-    for (int k = tid; k < NH; k += kThreads)
+    const V beta = V((KIND == 2 && it > 1) ? st.beta : 0.);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
       {
-        const int kx = k % HX, ky = k / HX;
-        const int gi = wrap(i0 - p + kx, P, G.ax.periodic), gj = wrap(j0 - p + ky, P, G.ay.periodic);
-        // the Laplacian's wall entries are constrained to zero (read as zero by the cells)
-        const bool ok  = gi >= 0 && gj >= 0 && !(LAP && wall(gi, gj));
-        V          val = V(0);
-        if (ok)
+        const int k = tid + s * kThreads;
+        if (k >= NH)
+          continue;
+        V val = V(0);
+        if (in_mesh >> s & 1u)
           {
-            const int64_t idx = node_index(G, gi, gj);
-            const V       xv  = (KIND != 2 || it > 1) ? x[vb + idx] : V(0);
+            val = sx[s];
             if constexpr (KIND == 2)
               {
-                const V z = V(mass_diag_inv(G, gi, gj)) * r_in[vb + idx];
-                val       = it > 1 ? beta * xv - z : -z;
+                const V z = V(sd[s]) * sr[s];
+                val            = it > 1 ? beta * sx[s] - z : -z;
               }
-            else
-              val = xv;
           }
-        X[ky][kx] = val;
-      }
-    const int ii = tid % TX, jr = (tid / TX) * RO;
-    const int gi = i0 + ii;
-    V         xc[B::NB][W_], yc[RO][B::NB][W_];
-#pragma unroll
-    for (int o = 0; o < W_; ++o)
-      {
-        const bool in = gi <= P;
-#pragma unroll
-        for (int t = 0; t < B::NB; ++t)
-          xc[t][o] = in ? V(band<p>(band_array(G.ax, which[t]), gi, o - p)) : V(0);
-#pragma unroll
-        for (int r = 0; r < RO; ++r)
-          {
-            const int  gj = j0 + jr + r;
-            const bool iy = gj <= P;
-#pragma unroll
-            for (int t = 0; t < B::NB; ++t)
-              yc[r][t][o] = iy ? V(band<p>(band_array(G.ay, which[t]), gj, o - p)) : V(0);
-          }
+        X[k / HX][k % HX] = val;
       }
     __syncthreads();
 
@@ -497,6 +657,8 @@ namespace lbfem_amd
   // side by side on separate CUs (one block for all of them spent most of its time in the
   // partials' loads at a million nodes; 1024 threads in it measured no faster).
   constexpr int kFinThreads = kThreads;
+  // nodes per thread and pass of the CG update (their loads issued together)
+  constexpr int kUpdNodes = 2;
 
   template <int STAGE>
   __device__ __forceinline__ void
@@ -508,10 +670,12 @@ namespace lbfem_amd
     __shared__ double tot[nk];
     if (v >= W.nv)
       return;
-    CgState st = W.state[v];
-    if (STAGE != 1 && st.status != 0)
-      return;
-    double acc[nk];
+    // the state and the partials loaded before the state's test (a converged vector's are in
+    // bounds and unused); stage 3 turns iteration it - 1's state into iteration it's
+    const CgState *in  = states_of(W, STAGE == 1 ? 0 : it - 1);
+    CgState       *out = states_of(W, STAGE == 3 ? it : STAGE == 1 ? 0 : it - 1);
+    CgState        st  = in[v];
+    double         acc[nk];
 #pragma unroll
     for (int k = 0; k < nk; ++k)
       acc[k] = 0.;
@@ -519,6 +683,12 @@ namespace lbfem_amd
 #pragma unroll
       for (int k = 0; k < nk; ++k)
         acc[k] += W.partial[size_t(row0 + bb) * kPart + stride * v + k];
+    if (STAGE != 1 && st.status != 0)
+      {
+        if (STAGE == 3 && threadIdx.x == 0) // carried to the next iteration's set
+          out[v] = st;
+        return;
+      }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < nk; ++k)
@@ -555,18 +725,8 @@ namespace lbfem_amd
     else if constexpr (STAGE == 2)
       st.alpha = st.rho / tot[0];
     else
-      {
-        st.res = sqrt(fabs(tot[0]));
-        st.its = it;
-        if (st.res <= st.tol)
-          st.status = 1;
-        else if (it >= W.max_its || isnan(st.res))
-          st.status = 2;
-        st.rho_prev = st.rho;
-        st.rho      = tot[1];
-        st.beta     = st.rho / st.rho_prev;
-      }
-    W.state[v] = st;
+      st = advance_state(st, tot[0], tot[1], it, W.max_its);
+    out[v] = st;
   }
 
   template <int STAGE>
@@ -596,42 +756,97 @@ namespace lbfem_amd
   template <typename V>
   __device__ __forceinline__ void
   update_body(const Geo &G, const CgWork &W, V *__restrict__ x, V *__restrict__ r, const V *__restrict__ p,
-              const V *__restrict__ q, const int v, const int blk, const int nblk)
+              const V *__restrict__ q, const int v, const int blk, const int nblk, const int it)
   {
     __shared__ double red[2][kThreads / 64];
-    __shared__ double alpha_s;
-    if (v >= W.nv || W.state[v].status != 0)
+    // the kernel arguments the loads below need, loaded together (see operator_body)
+    int early;
+    asm("s_mov_b32 %0, 0"
+        : "=s"(early)
+        : "s"(W.nv), "s"(W.block[std::min(v, NM - 1)]), "s"(G.np), "s"(G.stride), "s"(G.dinv), "s"(W.state),
+          "s"(W.state2), "s"(W.fold_rows), "s"(W.alpha_rows), "s"(W.partial), "s"(x), "s"(r), "s"(p), "s"(q),
+          "s"(it), "s"(nblk));
+    if (v >= W.nv)
       return;
-    const int64_t n   = int64_t(G.np) * G.np;
-    const int64_t off = int64_t(W.block[v]) * G.stride;
-    double        a   = W.state[v].alpha;
+    const int64_t n    = int64_t(G.np) * G.np;
+    const int64_t off  = int64_t(W.block[v]) * G.stride;
+    const int64_t step = int64_t(nblk) * kThreads * kUpdNodes;
+    // the vector's state (across the lanes), the direction stage's partial rows for alpha
+    // and this thread's first kUpdNodes nodes (base + u * kThreads) are loaded together: one
+    // memory round trip; a converged vector's loads, and those of slots past the mesh (at its
+    // last node), are in bounds and unused
+    const double st_word = state_word(states_of(W, it - 1) + v); // after iteration it - 1
+    int64_t      base    = blk * int64_t(kThreads) * kUpdNodes + int(threadIdx.x) + early;
+    V            xv[kUpdNodes], pv[kUpdNodes], qv[kUpdNodes], rv[kUpdNodes];
+    double       dv[kUpdNodes];
+    const auto   load = [&](const int64_t b0) {
+#pragma unroll
+      for (int u = 0; u < kUpdNodes; ++u)
+        {
+          const int64_t kk = std::min(b0 + u * kThreads, n - 1);
+          dv[u]            = G.dinv[kk];
+          xv[u] = x[off + kk];
+          pv[u] = p[off + kk];
+          qv[u] = q[off + kk];
+          rv[u] = r[off + kk];
+        }
+    };
+    load(std::min(base, n - 1));
+    // alpha = rho / p.q from the direction stage's partials, summed in a fixed order (thread t
+    // takes rows t, t + 256, ..., then the block), so every block and thread holds the same
+    // value, bit for bit, without a launch of its own; the first kAlphaPer rows of a thread
+    // loaded with the rest
+    constexpr int kAlphaPer = 2;
+    double        pq[1]     = {0.};
     if (W.alpha_rows > 0)
       {
-        // alpha = rho / p.q from the direction stage's partials, summed in a fixed order
-        // (thread t takes rows t, t + 256, ..., then the block), so every block holds the
-        // same value, bit for bit, without a launch of its own
-        double pq[1] = {0.};
-        for (int bb = threadIdx.x; bb < W.alpha_rows; bb += kThreads)
+        double pr[kAlphaPer];
+#pragma unroll
+        for (int j = 0; j < kAlphaPer; ++j)
+          {
+            const int bb = int(threadIdx.x) + j * kThreads;
+            const double a = W.partial[size_t(std::min(bb, W.alpha_rows - 1)) * kPart + 3 * v];
+            pr[j]          = bb < W.alpha_rows ? a : 0.;
+          }
+#pragma unroll
+        for (int j = 0; j < kAlphaPer; ++j)
+          pq[0] += pr[j];
+        for (int bb = int(threadIdx.x) + kAlphaPer * kThreads; bb < W.alpha_rows; bb += kThreads)
           pq[0] += W.partial[size_t(bb) * kPart + 3 * v];
-        block_sum<1>(pq, red);
-        if (threadIdx.x == 0)
-          alpha_s = W.state[v].rho / pq[0];
-        __syncthreads();
-        a = alpha_s;
+        block_sum_all<1, kThreads>(pq, red);
       }
+    const CgState st     = state_from_lanes(st_word);
+    const int     status = st.status;
+    const double  a      = W.alpha_rows > 0 ? st.rho / pq[0] : st.alpha;
+    if (status != 0)
+      return;
     const V alpha   = V(a);
     double  part[2] = {0., 0.};
-    for (int64_t k = blk * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(nblk) * kThreads)
+    for (; base < n; base += step)
       {
-        const int    i = int(k % G.np), j = int(k / G.np);
-        const double d  = mass_diag_inv(G, i, j);
-        const V      xv = x[off + k], pv = p[off + k], qv = q[off + k], rv = r[off + k];
-        x[off + k]      = xv + alpha * pv;
-        const V rr      = rv + alpha * qv;
-        r[off + k]      = rr;
-        const double r8 = double(rr);
-        part[0] += r8 * r8;
-        part[1] += r8 * (d * r8);
+        double r8[kUpdNodes], dk[kUpdNodes];
+#pragma unroll
+        for (int u = 0; u < kUpdNodes; ++u)
+          {
+            const int64_t kk = base + u * kThreads;
+            const V       rr = rv[u] + alpha * qv[u];
+            r8[u]            = double(rr);
+            dk[u]            = dv[u];
+            if (kk < n)
+              {
+                x[off + kk] = xv[u] + alpha * pv[u];
+                r[off + kk] = rr;
+              }
+          }
+        if (base + step < n) // the next pass's loads before this one's sums
+          load(base + step);
+#pragma unroll
+        for (int u = 0; u < kUpdNodes; ++u)
+          if (base + u * kThreads < n)
+            {
+              part[0] += r8[u] * r8[u];
+              part[1] += r8[u] * (dk[u] * r8[u]);
+            }
       }
     block_sum<2>(part, red);
     // after the direction stage's rows (other blocks may still be reading them for alpha)
@@ -645,9 +860,11 @@ namespace lbfem_amd
   template <typename V>
   __global__ void __launch_bounds__(kThreads)
   k_cg_update(const Geo G, const CgWork W, V *__restrict__ x, V *__restrict__ r, const V *__restrict__ p,
-              const V *__restrict__ q)
+              const V *__restrict__ q, const int it, const int nblk)
   {
-    update_body<V>(G, W, x, r, p, q, blockIdx.y, blockIdx.x, gridDim.x);
+    // nblk: gridDim.x as an argument (gridDim is read from the dispatch packet, a load of
+    // its own)
+    update_body<V>(G, W, x, r, p, q, blockIdx.y, blockIdx.x, nblk, it);
   }
 
   // every member's update in one launch: member m owns the blocks [upd_begin, upd_begin +
@@ -658,7 +875,7 @@ namespace lbfem_amd
     const int       k = member_of(T, n, int(blockIdx.x), [](const CgMember &m) { return m.upd_begin; });
     const CgMember &m = T[k];
     update_body<double>(m.G, m.W, m.x, m.r, m.p[(it + 1) % 2], m.q, blockIdx.y, int(blockIdx.x) - m.upd_begin,
-                        m.upd_blocks);
+                        m.upd_blocks, it);
   }
 
   // The nodal stages of the batched consistent-mass step, elementwise over each member's
@@ -1107,7 +1324,9 @@ namespace lbfem_amd
   int
   cg_update_blocks(const Geo &G, const int max_blocks, const int alpha_rows)
   {
-    return std::min(nodal_blocks(G), std::min(max_blocks - alpha_rows, 1024));
+    const int64_t n    = int64_t(G.np) * G.np;
+    const int64_t need = (n + int64_t(kThreads) * kUpdNodes - 1) / (int64_t(kThreads) * kUpdNodes);
+    return int(std::min<int64_t>(need, std::min(max_blocks - alpha_rows, 1024)));
   }
 
   int
@@ -1121,7 +1340,17 @@ namespace lbfem_amd
   update_launch(const Geo &G, const CgWork &W, const int it, V *x, V *r, const V *p, const V *q, hipStream_t s)
   {
     const int nb = cg_update_blocks(G, W.max_blocks, W.alpha_rows);
-    hipLaunchKernelGGL(k_cg_update<V>, dim3(nb, std::max(W.nv, 1)), dim3(kThreads), 0, s, G, W, x, r, p, q);
+    hipLaunchKernelGGL(k_cg_update<V>, dim3(nb, std::max(W.nv, 1)), dim3(kThreads), 0, s, G, W, x, r, p, q, it, nb);
+    if (W.fold_rows == 0) // else the next direction stage folds it in
+      hipLaunchKernelGGL((k_cg_finalize<3>), dim3(std::max(W.nv, 1)), dim3(kFinThreads), 0, s, W, nb, it,
+                         W.alpha_rows);
+    return hipGetLastError();
+  }
+
+  hipError_t
+  cg_update_finalize_launch(const Geo &G, const CgWork &W, const int it, hipStream_t s)
+  {
+    const int nb = cg_update_blocks(G, W.max_blocks, W.alpha_rows);
     hipLaunchKernelGGL((k_cg_finalize<3>), dim3(std::max(W.nv, 1)), dim3(kFinThreads), 0, s, W, nb, it, W.alpha_rows);
     return hipGetLastError();
   }
@@ -1352,9 +1581,8 @@ namespace lbfem_amd
   }
   hipError_t
   diagnostics_launch(const Geo &G, const double *f, const double *u_exact, double *u_prev, double *sums,
-                     double *partial, unsigned *counter, const int max_blocks, hipStream_t s)
+                     double *partial, const int max_blocks, hipStream_t s)
   {
-    (void)counter;
     int nb = std::min(nodal_blocks(G), std::min(max_blocks, 1024));
     hipLaunchKernelGGL(k_diagnostics<double>, dim3(nb), dim3(kThreads), 0, s, G, f, u_exact, 1., u_prev, partial,
                        max_blocks);
