@@ -1356,8 +1356,7 @@ namespace
             M[k].p[0]      = s.pb[0].p;
             M[k].p[1]      = s.pb[1].p;
             M[k].q         = s.q.p;
-            M[k].tiles_x   = (G.P + 1 + 31) / 32;
-            M[k].tiles_y   = (G.P + 1 + 15) / 16;
+            cg_operator_tiles(G, M[k].tiles_x, M[k].tiles_y);
             M[k].nt        = M[k].tiles_x * M[k].tiles_y;
             M[k].op_begin  = op_blocks;
             op_blocks += (M[k].nt + 7) & ~7;

@@ -80,8 +80,37 @@ namespace lbfem_amd
     const double *dinv;
   };
 
+  // a double moved across lanes by a DPP control (both halves; lanes of rows outside
+  // ROW_MASK read zero)
+  template <int CTRL, int ROW_MASK = 0xf>
+  __device__ __forceinline__ double
+  dpp_move(const double v)
+  {
+    const uint64_t b  = __builtin_bit_cast(uint64_t, v);
+    const int      lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), CTRL, ROW_MASK, 0xf, false);
+    const int      hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(b >> 32)), CTRL, ROW_MASK, 0xf, false);
+    return __builtin_bit_cast(double, uint64_t(uint32_t(lo)) | (uint64_t(uint32_t(hi)) << 32));
+  }
+
+  // The sum of a wavefront's 64 values, in lane 63, in a fixed order without LDS: pairs and
+  // quads (quad permutations), the four quads of each 16-lane row (row rotations by 4 and
+  // 8), then the rows (lane 15 into row 1 and lane 47 into row 3, then lane 31 into rows 2
+  // and 3)
+  __device__ __forceinline__ double
+  wave_sum(double s)
+  {
+    s += dpp_move<0xb1>(s);       // quad_perm [1, 0, 3, 2]
+    s += dpp_move<0x4e>(s);       // quad_perm [2, 3, 0, 1]
+    s += dpp_move<0x124>(s);      // row_ror 4
+    s += dpp_move<0x128>(s);      // row_ror 8
+    s += dpp_move<0x142, 0xa>(s); // row_bcast 15
+    s += dpp_move<0x143, 0xc>(s); // row_bcast 31
+    return s;
+  }
+  constexpr int kWaveSumLane = 63;
+
   // sum over a block of NT threads of up to K values per thread; the result is
-  // valid in thread 0 (a fixed order: wavefront shuffles, then the wavefronts)
+  // valid in thread 0 (a fixed order: wave_sum, then the wavefronts)
   template <int K, int NT>
   __device__ __forceinline__ void
   block_sum_n(double (&v)[K], double (*lds)[NT / 64])
@@ -90,11 +119,8 @@ namespace lbfem_amd
 #pragma unroll
     for (int k = 0; k < K; ++k)
       {
-        double s = v[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-          s += __shfl_down(s, off, 64);
-        if (lane == 0)
+        const double s = wave_sum(v[k]);
+        if (lane == kWaveSumLane)
           lds[k][wave] = s;
       }
     __syncthreads();
@@ -121,11 +147,8 @@ namespace lbfem_amd
 #pragma unroll
     for (int k = 0; k < K; ++k)
       {
-        double s = v[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-          s += __shfl_down(s, off, 64);
-        if (lane == 0)
+        const double s = wave_sum(v[k]);
+        if (lane == kWaveSumLane)
           lds[k][wave] = s;
       }
     __syncthreads();

@@ -214,6 +214,8 @@ namespace lbfem_amd
 
   // partial rows the CG direction stage writes (its blocks per vector)
   int cg_direction_rows(const Geo &G);
+  // the tensor operator's tiles per direction (64 - 2p owned columns, 16 owned rows each)
+  void cg_operator_tiles(const Geo &G, int &tiles_x, int &tiles_y);
   // the update stage's grid of a CG with the partial rows alpha_rows ahead of its own
   int cg_update_blocks(const Geo &G, int max_blocks, int alpha_rows);
 

@@ -306,18 +306,32 @@ namespace lbfem_amd
     return which == 0 ? a.M : which == 1 ? a.D : which == 2 ? a.K : a.DT;
   }
 
+  // The operator's tile: HX = 64 halo'd columns, one wavefront's lanes (lane l stages and
+  // reads column l of every row), TX = 64 - 2p owned columns, TY = 16 owned rows; wavefront
+  // w stages halo rows w, w + 4, ... and owns rows 4w .. 4w + 3. A row index is uniform over
+  // a wavefront and a column index fixed per lane, so the staging's address arithmetic is a
+  // few scalar operations per row and one multiply-add per lane (a flat halo index split
+  // into row and column per slot cost most of the kernel's instructions).
+  template <int p>
+  struct OpTile
+  {
+    static constexpr int HX = 64, TX = HX - 2 * p, TY = 16, HY = TY + 2 * p;
+    static constexpr int NWAVE = kThreads / 64, RO = TY / NWAVE, NR = (HY + NWAVE - 1) / NWAVE;
+    static_assert(TY % NWAVE == 0, "owned rows per wavefront");
+  };
+
   // The body of one operator block: block bid of the mesh's (XCD-ordered) tile blocks, vector v.
-  template <typename V, int p, int TX, int TY, int KIND, int OP>
+  template <typename V, int p, int KIND, int OP>
   __device__ __forceinline__ void
   operator_body(const Geo &G, const CgWork &W, const int it, const int tiles_x, const int tiles_y, const int bid,
                 const int v, const V *x, const V *b, const V *r_in, V *y, V *y2)
   {
-    using B                  = OpBands<OP>;
-    constexpr int  W_        = 2 * p + 1, HX = TX + 2 * p, HY = TY + 2 * p, NH = HX * HY;
-    constexpr int  TR        = kThreads / TX, RO = TY / TR;
-    constexpr bool LAP       = OP == kOpLaplace;
-    constexpr int  which[4]  = {0, OP == kOpLaplace ? 2 : 1, 2, 3}; // band array of each slot
-    static_assert(TY % TR == 0, "tile shape");
+    using B                 = OpBands<OP>;
+    using T                 = OpTile<p>;
+    constexpr int  W_       = 2 * p + 1, HX = T::HX, HY = T::HY, TX = T::TX, TY = T::TY;
+    constexpr int  RO       = T::RO, NR = T::NR, NWAVE = T::NWAVE;
+    constexpr bool LAP      = OP == kOpLaplace;
+    constexpr int  which[4] = {0, OP == kOpLaplace ? 2 : 1, 2, 3}; // band array of each slot
     __shared__ V      X[HY][HX];
     __shared__ double red[3][kThreads / 64];
 
@@ -325,7 +339,7 @@ namespace lbfem_amd
     // argument loads after it wait in turn: a chain of round trips): a padding block loads
     // the last tile's operand and leaves after the loads. The kernel arguments the loads
     // need are made available together first: their scalar loads are issued before the
-    // statement (a zero into `early`, which the thread index takes on, so it stays ahead of
+    // statement (a zero into `early`, which the lane index takes on, so it stays ahead of
     // the loads; no memory effect, so uniform loads after it may still be scalar) and waited
     // for once, not each behind the previous one's wait at its first use.
     int early;
@@ -342,7 +356,8 @@ namespace lbfem_amd
     const bool live = tile < nt;
     const int  tl   = live ? tile : nt - 1;
     const int  i0 = (tl % tiles_x) * TX, j0 = (tl / tiles_x) * TY;
-    const int  P = G.P, tid = int(threadIdx.x) + early;
+    const int  P = G.P, tid = int(threadIdx.x);
+    const int  lane = (tid & 63) + early, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const auto wall = [P](const int i, const int j) { return i == 0 || j == 0 || i == P || j == P; };
     // the staged node of halo coordinate i, -1 outside: walled, outside [0, P]; periodic, the
     // image in [0, P) of i in [-p, P + p] (a halo further out, past the last tile's nodes,
@@ -357,36 +372,33 @@ namespace lbfem_amd
     };
 
     const int64_t vb = int64_t(W.block[v]) * G.stride;
-    // Stage the operand (KIND 2: the new direction) of the halo'd block. A thread's NS slots
+    // Stage the operand (KIND 2: the new direction) of the halo'd block. A thread's NR rows
     // are loaded together, with the vector's state, before anything waits on them: one
-    // memory round trip instead of one per slot and one for the state (this kernel's time
-    // at small meshes is its chain of round trips, not its bytes). A converged vector's
-    // loads are in bounds and unused.
-    constexpr int NS = (NH + kThreads - 1) / kThreads;
+    // memory round trip (this kernel's time at small meshes is its chain of round trips,
+    // not its bytes). A converged vector's loads are in bounds and unused; so are those of a
+    // slot outside the mesh, at an in-mesh address.
     // KIND 2: the stopping state after iteration it - 1, read, or (fold) formed here from
     // iteration it - 2's and the update stage's partial rows of iteration it - 1, summed as
     // the finalisation sums them (thread t takes rows t, t + 256, ..., then the block)
     constexpr int kFoldPer = kMaxCgFoldRows / kThreads; // update rows per thread
     const bool    fold     = KIND == 2 && W.fold_rows > 0 && it > 1;
     const double  st_word  = KIND == 2 ? state_word(states_of(W, fold ? it - 2 : it - 1) + v) : 0.;
-    V             sx[NS], sr[NS];
-    double        sd[NS];
-    unsigned      in_mesh = 0; // per slot: holds a node's value (the rest stage zero)
-    // Every slot loads (an in-mesh address, its value discarded where the slot is outside the
-    // mesh): loads under divergent branches, each joined before the next, were what kept
-    // them in a chain.
+    int           hx       = halo(i0 - p + lane, G.ax.periodic); // this lane's column
+    const bool    col      = hx >= 0;
+    hx                     = std::max(hx, 0);
+    V             sx[NR], sr[NR];
+    double        sd[NR];
+    unsigned      in_mesh = 0; // per row slot: holds a node's value (the rest stage zero)
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NR; ++s)
       {
-        const int k  = std::min(tid + s * kThreads, NH - 1);
-        const int kx = k % HX, ky = k / HX;
-        int       gi = halo(i0 - p + kx, G.ax.periodic), gj = halo(j0 - p + ky, G.ay.periodic);
+        const int ky = wave + NWAVE * s; // uniform
+        int       hy = halo(j0 - p + std::min(ky, HY - 1), G.ay.periodic);
         // the Laplacian's wall entries are constrained to zero (read as zero by the cells)
-        const bool ok = tid + s * kThreads < NH && gi >= 0 && gj >= 0 && !(LAP && wall(gi, gj));
+        const bool ok = col && ky < HY && hy >= 0 && !(LAP && wall(hx, hy));
         in_mesh |= unsigned(ok) << s;
-        gi = std::max(gi, 0);
-        gj = std::max(gj, 0);
-        const int64_t idx = node_index(G, gi, gj);
+        hy                = std::max(hy, 0);
+        const int64_t idx = int64_t(hy) * G.np + hx;
         // KIND 2 at it == 1 has no old direction: the buffer is read (it is there) and its
         // values unused
         sx[s] = x[vb + idx];
@@ -398,8 +410,9 @@ namespace lbfem_amd
             sd[s] = G.dinv[idx];
           }
       }
-    const int ii = tid % TX, jr = (tid / TX) * RO;
-    const int gi = i0 + ii, gic = std::min(gi, P);
+    // the bands of this lane's owned column and of the wavefront's owned rows
+    const int jr = wave * RO;
+    const int gi = i0 + lane, gic = std::min(gi, P);
     V         xc[B::NB][W_], yc[RO][B::NB][W_];
 #pragma unroll
     for (int o = 0; o < W_; ++o)
@@ -473,10 +486,10 @@ namespace lbfem_amd
       }
     const V beta = V((KIND == 2 && it > 1) ? st.beta : 0.);
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NR; ++s)
       {
-        const int k = tid + s * kThreads;
-        if (k >= NH)
+        const int ky = wave + NWAVE * s;
+        if (ky >= HY)
           continue;
         V val = V(0);
         if (in_mesh >> s & 1u)
@@ -485,16 +498,17 @@ namespace lbfem_amd
             if constexpr (KIND == 2)
               {
                 const V z = V(sd[s]) * sr[s];
-                val            = it > 1 ? beta * sx[s] - z : -z;
+                val       = it > 1 ? beta * sx[s] - z : -z;
               }
           }
-        X[k / HX][k % HX] = val;
+        X[ky][lane] = val;
       }
     __syncthreads();
 
     // partial sums in scalars (an array indexed by a run-time vector number would live in scratch memory)
-    double part0 = 0., part1 = 0., part2 = 0.;
-    V      acc[RO];
+    const int ii    = std::min(lane, TX - 1); // lanes past the owned columns compute a copy, unused
+    double    part0 = 0., part1 = 0., part2 = 0.;
+    V         acc[RO];
 #pragma unroll
     for (int r = 0; r < RO; ++r)
       acc[r] = V(0);
@@ -563,7 +577,7 @@ namespace lbfem_amd
     for (int r = 0; r < RO; ++r)
       {
         const int gj = j0 + jr + r;
-        if (gi > P || gj > P)
+        if (lane >= TX || gi > P || gj > P)
           continue;
         const int64_t idx   = node_index(G, gi, gj);
         const bool    slave = is_slave(G, gi, gj) || (LAP && wall(gi, gj));
@@ -596,22 +610,29 @@ namespace lbfem_amd
             part0 += double(own) * double(a);
           }
       }
-    if constexpr (KIND != 0)
+    if constexpr (KIND == 1)
       {
         double pv[3] = {part0, part1, part2};
         block_sum<3>(pv, red);
         if (tid == 0)
-          for (int k = 0; k < (KIND == 1 ? 3 : 1); ++k)
+          for (int k = 0; k < 3; ++k)
             W.partial[size_t(tile) * kPart + 3 * v + k] = pv[k];
+      }
+    else if constexpr (KIND == 2)
+      {
+        double pv[1] = {part0};
+        block_sum<1>(pv, red);
+        if (tid == 0)
+          W.partial[size_t(tile) * kPart + 3 * v] = pv[0];
       }
   }
 
-  template <typename V, int p, int TX, int TY, int KIND, int OP>
+  template <typename V, int p, int KIND, int OP>
   __global__ void __launch_bounds__(kThreads)
   k_operator(const Geo G, const CgWork W, const int it, const int tiles_x, const int tiles_y, const V *x,
              const V *b, const V *r_in, V *y, V *y2)
   {
-    operator_body<V, p, TX, TY, KIND, OP>(G, W, it, tiles_x, tiles_y, blockIdx.x, blockIdx.y, x, b, r_in, y, y2);
+    operator_body<V, p, KIND, OP>(G, W, it, tiles_x, tiles_y, blockIdx.x, blockIdx.y, x, b, r_in, y, y2);
   }
 
   // the member of a batched launch that owns block b: the last whose first block is <= b
@@ -635,7 +656,7 @@ namespace lbfem_amd
   // [op_begin, op_begin + nb8) of the launch and runs exactly its own single-mesh blocks
   // (the CG start, KIND 1, on x, b, r; the direction, KIND 2, on the directions of
   // iteration it and q).
-  template <int p, int TX, int TY, int KIND, int OP>
+  template <int p, int KIND, int OP>
   __global__ void __launch_bounds__(kThreads)
   k_operator_batch(const CgMember *__restrict__ T, const int n, const int it)
   {
@@ -645,7 +666,7 @@ namespace lbfem_amd
       return;
     const double   *x = KIND == 1 ? m.x : m.p[it % 2];
     double         *y = KIND == 1 ? m.r : m.p[(it + 1) % 2];
-    operator_body<double, p, TX, TY, KIND, OP>(m.G, m.W, it, m.tiles_x, m.tiles_y, int(blockIdx.x) - m.op_begin,
+    operator_body<double, p, KIND, OP>(m.G, m.W, it, m.tiles_x, m.tiles_y, int(blockIdx.x) - m.op_begin,
                                                int(blockIdx.y), x, m.b, m.r, y, m.q);
   }
 
@@ -693,11 +714,8 @@ namespace lbfem_amd
 #pragma unroll
     for (int k = 0; k < nk; ++k)
       {
-        double t = acc[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-          t += __shfl_down(t, off, 64);
-        if (lane == 0)
+        const double t = wave_sum(acc[k]);
+        if (lane == kWaveSumLane)
           red[k][wave] = t;
       }
     __syncthreads();
@@ -1263,9 +1281,9 @@ namespace lbfem_amd
         else
           return hipErrorInvalidValue;
       }
-    constexpr int TX = 32, TY = 16;
-    const int     tx = (G.P + 1 + TX - 1) / TX, ty = (G.P + 1 + TY - 1) / TY;
-    const int     nt = tx * ty, nb8 = (nt + 7) & ~7;
+    int tx = 0, ty = 0;
+    cg_operator_tiles(G, tx, ty);
+    const int nt = tx * ty, nb8 = (nt + 7) & ~7;
     if (KIND != 0 && nt > W.max_blocks)
       return hipErrorInvalidValue;
     // the band set: the Laplacian's, or the streaming matrix's when a vector has a K_e term
@@ -1276,8 +1294,7 @@ namespace lbfem_amd
     const dim3 grid(nb8, W.nv), block(kThreads);
     const auto go = [&](auto pc, auto oc) {
       constexpr int P_ = decltype(pc)::value, OP_ = decltype(oc)::value;
-      hipLaunchKernelGGL((k_operator<V, P_, TX, TY, KIND, OP_>), grid, block, 0, s, G, W, it, tx, ty, x, b, r, y,
-                         y2);
+      hipLaunchKernelGGL((k_operator<V, P_, KIND, OP_>), grid, block, 0, s, G, W, it, tx, ty, x, b, r, y, y2);
     };
     const auto by_op = [&](auto pc) {
       if (op == kOpMass)
@@ -1312,13 +1329,24 @@ namespace lbfem_amd
     return hipGetLastError();
   }
 
+  static_assert(OpTile<1>::TX == 62 && OpTile<3>::TX == 58 && OpTile<2>::TY == 16, "cg_operator_tiles");
+
+  void
+  cg_operator_tiles(const Geo &G, int &tiles_x, int &tiles_y)
+  {
+    const int tx = 64 - 2 * G.p, ty = 16; // OpTile<p>::TX, TY
+    tiles_x      = (G.P + 1 + tx - 1) / tx;
+    tiles_y      = (G.P + 1 + ty - 1) / ty;
+  }
+
   int
   cg_direction_rows(const Geo &G)
   {
     if (G.gen.vert != nullptr)
       return quad_tile_count(G);
-    constexpr int TX = 32, TY = 16; // the tensor operator's tile (operator_launch)
-    return ((G.P + 1 + TX - 1) / TX) * ((G.P + 1 + TY - 1) / TY);
+    int tx = 0, ty = 0;
+    cg_operator_tiles(G, tx, ty);
+    return tx * ty;
   }
 
   int
@@ -1361,11 +1389,10 @@ namespace lbfem_amd
   operator_batch_launch(const int p, const int op, const CgMember *T, const int n, const int op_blocks, const int it,
                         hipStream_t s)
   {
-    constexpr int TX = 32, TY = 16;
-    const dim3    grid(op_blocks, NM), block(kThreads);
-    const auto    go = [&](auto pc, auto oc) {
+    const dim3 grid(op_blocks, NM), block(kThreads);
+    const auto go = [&](auto pc, auto oc) {
       constexpr int P_ = decltype(pc)::value, OP_ = decltype(oc)::value;
-      hipLaunchKernelGGL((k_operator_batch<P_, TX, TY, KIND, OP_>), grid, block, 0, s, T, n, it);
+      hipLaunchKernelGGL((k_operator_batch<P_, KIND, OP_>), grid, block, 0, s, T, n, it);
     };
     const auto by_op = [&](auto pc) {
       if (op == kOpMass)
