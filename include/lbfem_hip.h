@@ -166,8 +166,11 @@ typedef struct
   lbfem_mass_settings mass;
   int                 precision; /* LBFEM_FP64 | LBFEM_MIXED: FP32 populations stored as
                                     deviations f - w rho0, FP64 moments and equilibria;
-                                    with the consistent mass, FP32 inner CG solves refined
-                                    with FP64 residuals (lumped or CG, not Richardson) */
+                                    with the consistent mass, CG solves on FP32 increments
+                                    (FP64 scalars and sums) from the extrapolated FP32
+                                    increment, stopping at max(cg_tolerance, 1e-6) |b|
+                                    (the FP32 right-hand side's round-off level); lumped
+                                    or CG, not Richardson */
 } lbfem_scheme_settings;
 
 int lbfem_scheme_create(const lbfem_disc *disc, const lbfem_scheme_settings *settings, lbfem_scheme **scheme);
@@ -196,8 +199,8 @@ int lbfem_scheme_cg_iterations(const lbfem_scheme *scheme, int64_t *iterations);
 /* The CG steps of each solve of the last step (the SolverControl::last_step()
  * of the reference's per-population solves, mass.h:204-211): its[sw * 8 + a - 1]
  * for sweep sw (0, and 1 for split TG3) and population a, 0 where no CG ran
- * (lumped, Richardson, a population at rest in the sweep). The mixed form
- * reports the inner FP32 iterations summed over the refinements. */
+ * (lumped, Richardson, a population at rest in the sweep); the mixed form's
+ * are its FP32 solves'. */
 int lbfem_scheme_last_cg_iterations(const lbfem_scheme *scheme, int32_t *its);
 /* StageTimers (include/lbfem/timers.h:17-23): seconds in collision,
  * advection, mass, and the total of the timed steps; timing adds events. */

@@ -272,13 +272,11 @@ namespace lbfem_amd
                                  float *p_new, float *q, hipStream_t stream);
   hipError_t cg_update_launch(const Geo &G, const CgWork &W, int it, float *x, float *r, const float *p,
                               const float *q, hipStream_t stream);
-  // the mixed form's refinement: rhs to FP64, -r to FP32, x += d (d zeroed), and the nodal updates on FP32
-  // deviations from an FP64 increment
-  hipError_t to_double_launch(const Geo &G, const float *x, double *y, hipStream_t stream);
-  hipError_t negate_to_float_launch(const Geo &G, const double *x, float *y, hipStream_t stream);
-  hipError_t refine_add_launch(const Geo &G, double *x, float *d, unsigned mask, hipStream_t stream);
-  hipError_t add_increments_launch(const Geo &G, float *g, const double *x, unsigned mask, hipStream_t stream);
-  hipError_t pc_from_f_launch(const Geo &G, float *f, const double *incr, double theta, hipStream_t stream);
+  // the mixed form's consistent mass: the extrapolated start and the nodal updates on FP32 deviations from
+  // FP32 increments
+  hipError_t extrapolate_launch(const Geo &G, float *x, float *x_prev, unsigned mask, hipStream_t stream);
+  hipError_t add_increments_launch(const Geo &G, float *g, const float *x, unsigned mask, hipStream_t stream);
+  hipError_t pc_from_f_launch(const Geo &G, float *f, const float *incr, double theta, hipStream_t stream);
   // General quadrilaterals (quad_kernels.hip): the advection right-hand side
   // and the fused lumped step (TileMode semantics, no wall faces), and the mass
   // operator stages of the CG (KIND 0 product, 1 start, 2 direction).

@@ -179,8 +179,9 @@ namespace lbfem_amd
           f[(a + 1) * G.stride + k] += x[a * G.stride + k];
   }
 
+  template <typename V> // the FP64 form's increments, or the mixed form's FP32 ones
   __global__ void __launch_bounds__(kThreads)
-  k_extrapolate(const Geo G, double *x, double *x_prev, const unsigned mask)
+  k_extrapolate(const Geo G, V *x, V *x_prev, const unsigned mask)
   {
     const int64_t n = int64_t(G.np) * G.np;
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
@@ -188,8 +189,8 @@ namespace lbfem_amd
       for (int a = 0; a < NM; ++a)
         if (mask & (1u << a))
           {
-            const double xa = x[a * G.stride + k], xp = x_prev[a * G.stride + k];
-            x[a * G.stride + k]      = -xp + 2. * xa; // sadd(-1, 2, xa), then swap
+            const V xa = x[a * G.stride + k], xp = x_prev[a * G.stride + k];
+            x[a * G.stride + k]      = V(-double(xp) + 2. * double(xa)); // sadd(-1, 2, xa), then swap
             x_prev[a * G.stride + k] = xa;
           }
   }
@@ -1226,7 +1227,7 @@ namespace lbfem_amd
   hipError_t
   extrapolate_launch(const Geo &G, double *x, double *x_prev, const unsigned mask, hipStream_t s)
   {
-    hipLaunchKernelGGL(k_extrapolate, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, x_prev, mask);
+    hipLaunchKernelGGL(k_extrapolate<double>, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, x_prev, mask);
     return hipGetLastError();
   }
   hipError_t
@@ -1514,46 +1515,21 @@ namespace lbfem_amd
   }
 
   // ------------------------------------------------------------------ mixed-form helpers
-  // y = s x over NM blocks (the right-hand side to FP64, the negated residual to FP32)
-  template <typename A, typename B>
+  // Bardow: g_{a+1} += x_a on FP32 deviations from FP32 increments (the sum in FP64, rounded once)
   __global__ void __launch_bounds__(kThreads)
-  k_convert(const Geo G, const A *__restrict__ x, B *__restrict__ y, const double sgn)
-  {
-    const int64_t n = int64_t(NM) * G.stride;
-    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
-      y[k] = B(sgn * double(x[k]));
-  }
-
-  // x += d for the vectors in mask (the refinement's correction), d zeroed for the next inner solve
-  __global__ void __launch_bounds__(kThreads)
-  k_refine_add(const Geo G, double *__restrict__ x, float *__restrict__ d, const unsigned mask)
+  k_add_increments_mixed(const Geo G, float *g, const float *x, const unsigned mask)
   {
     const int64_t n = int64_t(G.np) * G.np;
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
 #pragma unroll
       for (int a = 0; a < NM; ++a)
         if (mask & (1u << a))
-          {
-            x[a * G.stride + k] += double(d[a * G.stride + k]);
-            d[a * G.stride + k] = 0.f;
-          }
-  }
-
-  // Bardow: g_{a+1} += x_a on FP32 deviations (the sum in FP64, rounded once)
-  __global__ void __launch_bounds__(kThreads)
-  k_add_increments_mixed(const Geo G, float *g, const double *x, const unsigned mask)
-  {
-    const int64_t n = int64_t(G.np) * G.np;
-    for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
-#pragma unroll
-      for (int a = 0; a < NM; ++a)
-        if (mask & (1u << a))
-          g[(a + 1) * G.stride + k] = float(double(g[(a + 1) * G.stride + k]) + x[a * G.stride + k]);
+          g[(a + 1) * G.stride + k] = float(double(g[(a + 1) * G.stride + k]) + double(x[a * G.stride + k]));
   }
 
   // Lee-Lin after the consistent-mass solve, on FP32 deviations: f <- PC(f, feq(f), x)
   __global__ void __launch_bounds__(kThreads)
-  k_pc_from_f_mixed(const Geo G, float *f, const double *incr, const double theta)
+  k_pc_from_f_mixed(const Geo G, float *f, const float *incr, const double theta)
   {
     const int64_t n = int64_t(G.np) * G.np;
     for (int64_t k = blockIdx.x * int64_t(kThreads) + threadIdx.x; k < n; k += int64_t(gridDim.x) * kThreads)
@@ -1565,7 +1541,7 @@ namespace lbfem_amd
           v[a] = f[a * G.stride + k];
 #pragma unroll
         for (int a = 0; a < NM; ++a)
-          x[a] = float(incr[a * G.stride + k]);
+          x[a] = incr[a * G.stride + k];
         double     wx = 0., wy = 0.;
         const bool wall = wall_of(G, i, j, wx, wy);
         equilibrium_node(v, fe, wall, wx, wy);
@@ -1577,31 +1553,19 @@ namespace lbfem_amd
   }
 
   hipError_t
-  to_double_launch(const Geo &G, const float *x, double *y, hipStream_t s)
+  extrapolate_launch(const Geo &G, float *x, float *x_prev, const unsigned mask, hipStream_t s)
   {
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, y, 1.);
+    hipLaunchKernelGGL(k_extrapolate<float>, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, x_prev, mask);
     return hipGetLastError();
   }
   hipError_t
-  negate_to_float_launch(const Geo &G, const double *x, float *y, hipStream_t s)
-  {
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, y, -1.);
-    return hipGetLastError();
-  }
-  hipError_t
-  refine_add_launch(const Geo &G, double *x, float *d, const unsigned mask, hipStream_t s)
-  {
-    hipLaunchKernelGGL(k_refine_add, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, x, d, mask);
-    return hipGetLastError();
-  }
-  hipError_t
-  add_increments_launch(const Geo &G, float *g, const double *x, const unsigned mask, hipStream_t s)
+  add_increments_launch(const Geo &G, float *g, const float *x, const unsigned mask, hipStream_t s)
   {
     hipLaunchKernelGGL(k_add_increments_mixed, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, g, x, mask);
     return hipGetLastError();
   }
   hipError_t
-  pc_from_f_launch(const Geo &G, float *f, const double *incr, const double theta, hipStream_t s)
+  pc_from_f_launch(const Geo &G, float *f, const float *incr, const double theta, hipStream_t s)
   {
     hipLaunchKernelGGL(k_pc_from_f_mixed, dim3(nodal_blocks(G)), dim3(kThreads), 0, s, G, f, incr, theta);
     return hipGetLastError();
