@@ -1258,7 +1258,7 @@ namespace
     g.n_nodes = 0;
     for (const lbfem_scheme *s : g.members)
       g.n_nodes += int64_t(s->d->G.np) * s->d->G.np;
-    g.variant = size_default_variant(g.n_nodes, false);
+    g.variant = batch_default_variant(g.n_nodes, false);
     int tx, ty;
     tile_dims(g.variant, tx, ty);
     const bool ll = g.scheme == LBFEM_LEELIN;
@@ -1594,7 +1594,7 @@ namespace
         it->mixed = s->mixed;
         it->grid  = b.shape_grid;
         it->variant =
-          b.shape_variant >= 0 ? b.shape_variant : size_default_variant(int64_t(s->d->G.np) * s->d->G.np, s->mixed);
+          b.shape_variant >= 0 ? b.shape_variant : batch_default_variant(int64_t(s->d->G.np) * s->d->G.np, s->mixed);
       }
     it->members.push_back(s);
   }
@@ -2597,7 +2597,7 @@ extern "C"
           int64_t nodes = 0;
           for (const lbfem_scheme *m : g.members)
             nodes += int64_t(m->d->G.np) * m->d->G.np;
-          g.variant = size_default_variant(nodes, g.mixed);
+          g.variant = batch_default_variant(nodes, g.mixed);
         }
       rebuild_tables(*b);
       for (auto &g : b->cg_groups)
@@ -2824,8 +2824,8 @@ extern "C"
   lbfem_batch_set_shape(lbfem_batch *b, const int variant, const int grid)
   {
     return guarded([&]() -> int {
-      if (!b || variant < -1 || variant >= kTileVariants || grid < 0)
-        return fail(LBFEM_EINVAL, "invalid tile variant or grid");
+      if (!b || variant < -1 || variant >= kTileLdsVariants || grid < 0)
+        return fail(LBFEM_EINVAL, "invalid tile variant or grid (a batch runs the LDS tile shapes 0..7)");
       HIP_CHECK(hipStreamSynchronize(b->stream));
       b->shape_variant = variant;
       b->shape_grid    = grid;
@@ -2859,7 +2859,7 @@ extern "C"
         {
           double best = 1e300;
           int    bv = g.variant, bg = g.grid;
-          for (int v = 0; v < kTileVariants; ++v)
+          for (int v = 0; v < kTileLdsVariants; ++v)
             for (const int grid : grids)
               {
                 // tile counts depend on the variant; launches at a fixed parity read one buffer and write the

@@ -164,6 +164,20 @@ namespace lbfem_amd
     __syncthreads();
   }
 
+  // global node line of a tile position, -1 outside a non-periodic axis. A periodic
+  // position lies in [-p, P + P): positions past P + p feed no owned node and map to
+  // any line inside.
+  __device__ __forceinline__ int
+  wrap_line(const int i, const int P, const int periodic)
+  {
+    if (periodic)
+      {
+        const int w = i < 0 ? i + P : (i >= P ? i - P : i);
+        return w < P ? w : P - 1;
+      }
+    return (i < 0 || i > P) ? -1 : i;
+  }
+
   __device__ __forceinline__ int64_t
   node_index(const Geo &G, const int i, const int j)
   {

@@ -24,7 +24,8 @@ namespace lbfem_amd
     double theta;             // Lee-Lin dt / lambda (schemes.h:97)
     int    collide;           // Bardow: collide on load
     int    remove_mass_flux;  // wall surface term option (advection.h:182-199)
-    int    tiles_x, tiles_y;  // set by the launcher
+    int    tiles_x, tiles_y;  // set by the launcher (the row-march kernel: strips, row segments)
+    int    rows;              // the row-march kernel: node rows per wavefront segment
   };
 
   // shapes of the tile kernel (all give bitwise identical results)
@@ -38,8 +39,22 @@ namespace lbfem_amd
     kTileWide40   = 5, // 40 x 8 nodes, 320 threads: 231 tiles (one per CU) on a 257^2 node grid
     kTileWide48   = 6, // 48 x 8 nodes, 384 threads
     kTileSplit2W40 = 7, // 40 x 8 nodes, 640 threads, split over 2 (about 1.5 node-waves per SIMD at 257^2)
-    kTileVariants = 8
+    kTileLdsVariants = 8, // the variants above stage the halo'd tile in LDS (the batch kernel's shapes)
+    // the row-march kernel (Q1 FP64; elsewhere these run as kTileWide): wavefront strips of 62
+    // columns marching up segments of 1, 4, 16 or 64 node rows
+    kMarch1       = 8,
+    kMarch4       = 9,
+    kMarch16      = 10,
+    kMarch64      = 11,
+    kTileVariants = 12
   };
+
+  // node rows per wavefront segment of a row-march variant
+  inline int
+  march_rows(const int variant)
+  {
+    return variant == kMarch1 ? 1 : variant == kMarch4 ? 4 : variant == kMarch16 ? 16 : 64;
+  }
 
   template <typename T, int p>
   hipError_t tile_launch_degree(int mode, int variant, const Geo &G, const TileParams &S, const T *fin, const T *feq,
@@ -142,12 +157,20 @@ namespace lbfem_amd
   // The variant used until a measurement picks one, by the node count of the launch: below
   // about four wide-tile wavefronts per SIMD of the device (256 CUs x 4 SIMDs) the 40 x 8 tile
   // split over two threads per node (the fastest measured shape at 257^2 nodes, Q1 and Q2,
-  // FP64 and FP32); above, the 32 x 8 tile in FP64 (20-30 % ahead of 32 x 16 at 1025^2 and
-  // 2049^2 nodes, Q1 and Q2, both schemes) and the 32 x 16 tile in FP32 (2 % ahead).
+  // FP64 and FP32); above, in FP64 the row-march kernel over 4-row segments (Q1: 18-27 %
+  // ahead of the 32 x 8 tile at 1025^2 and 2049^2 nodes, both schemes; on Q2/Q3 it runs as
+  // the 32 x 8 tile, which is 20-30 % ahead of 32 x 16 there) and in FP32 the 32 x 16 tile.
   inline int
   size_default_variant(const long long nodes, const bool fp32)
   {
-    return nodes < 64LL * 4 * 1024 ? kTileSplit2W40 : fp32 ? kTileTall : kTileWide;
+    return nodes < 64LL * 4 * 1024 ? kTileSplit2W40 : fp32 ? kTileTall : kMarch4;
+  }
+  // the same among the shapes a batch runs (the LDS tile shapes): the 32 x 8 tile in FP64
+  inline int
+  batch_default_variant(const long long nodes, const bool fp32)
+  {
+    const int v = size_default_variant(nodes, fp32);
+    return v >= kTileLdsVariants ? kTileWide : v;
   }
   inline int
   tile_default_variant(const Geo &G, const bool fp32)

@@ -1,0 +1,128 @@
+// The Q1 streaming operator in column form, shared by the tile kernel and the
+// row-march kernel so that both compute every increment by the same operation
+// sequence (bitwise identical results).
+//
+// On a cartesian Q1 mesh the advection right-hand side of population a at node
+// (i, j) is (advection.h:211-293, the Kronecker form of the cell integrals)
+//   r = ex sa (Dx (x) My) h + ey sa (Mx (x) Dy) h
+//     + c (ex^2 (Kx (x) My) + ey^2 (Mx (x) Ky) + ex ey (Dx (x) Dy^T + Dx^T (x) Dy)) f
+// with h = f (Bardow) or h = a f + b feq (Lee-Lin, sa = 1). It is evaluated in
+// two contractions. First, at each of the three columns x = i - 1 + k (k = 0, 1,
+// 2), the y bands of row j act on the column's three taps (rows j - 1, j, j + 1)
+// and the results are grouped by the x band they meet:
+//   U_D = ex sa My h + c ex ey Dy^T f,   U_M = ey sa Dy h + c ey^2 Ky f,
+//   U_K = c ex^2 My f,                   U_T = c ex ey Dy f.
+// Then w_k = Dx[k] U_D + Mx[k] U_M + Kx[k] U_K + Dx^T[k] U_T with the x bands of
+// column i, and r = (w_0 + w_1) + w_2. The column values depend only on the
+// column and the row, so a kernel that holds one column per lane computes them
+// once and passes w_0 and w_2 to the neighbouring lanes; the tile kernel
+// computes all three columns per node. Terms of a population that vanish by its
+// direction class (x: ey = 0, y: ex = 0, diagonal) are skipped the same way in
+// both.
+#pragma once
+
+#include "device.hpp"
+
+namespace lbfem_amd
+{
+  namespace q1
+  {
+    // fused multiply-add in the storage type (explicit, so that no contraction choice of
+    // the compiler can differ between the kernels)
+    __device__ __forceinline__ double
+    fma_t(const double a, const double b, const double c)
+    {
+      return __builtin_fma(a, b, c);
+    }
+    __device__ __forceinline__ float
+    fma_t(const float a, const float b, const float c)
+    {
+      return __builtin_fmaf(a, b, c);
+    }
+
+    // the direction and scheme coefficients of one population (uniform values)
+    template <typename T>
+    struct Coef
+    {
+      T    exsa, eysa, cexx, ceyy, cexy;
+      bool nx, ny;
+    };
+
+    template <typename T>
+    __device__ __forceinline__ Coef<T>
+    coef(const T ex, const T ey, const T sa, const T c)
+    {
+      Coef<T> k;
+      k.nx   = ex != T(0);
+      k.ny   = ey != T(0);
+      k.exsa = ex * sa;
+      k.eysa = ey * sa;
+      k.cexx = c * (ex * ex);
+      k.ceyy = c * (ey * ey);
+      k.cexy = c * (ex * ey);
+      return k;
+    }
+
+    // sum_o y[o] v[o], o = 0, 1, 2
+    template <typename T>
+    __device__ __forceinline__ T
+    chain(const T (&y)[3], const T (&v)[3])
+    {
+      return fma_t(y[2], v[2], fma_t(y[1], v[1], y[0] * v[0]));
+    }
+
+    template <typename T>
+    struct Cols
+    {
+      T D, M, K, T_;
+    };
+
+    // The column values of one population at one column from its taps fv (f) and hv
+    // (h; the same as fv for Bardow), yb[band][o] the y bands of the output row
+    // (bands: 0 M, 1 D, 2 K, 3 D^T).
+    template <typename T, bool LL>
+    __device__ __forceinline__ Cols<T>
+    columns(const T (&yb)[4][3], const T (&fv)[3], const T (&hv)[3], const Coef<T> &k)
+    {
+      Cols<T> u{T(0), T(0), T(0), T(0)};
+      if (k.nx)
+        {
+          const T yMh = chain(yb[0], hv);
+          const T yMf = LL ? chain(yb[0], fv) : yMh;
+          u.D         = k.exsa * yMh;
+          u.K         = k.cexx * yMf;
+        }
+      if (k.ny)
+        {
+          const T yDh = chain(yb[1], hv);
+          const T yKf = chain(yb[2], fv);
+          u.M         = fma_t(k.ceyy, yKf, k.eysa * yDh);
+          if (k.nx)
+            {
+              const T yTf = chain(yb[3], fv);
+              const T yDf = LL ? chain(yb[1], fv) : yDh;
+              u.D         = fma_t(k.cexy, yTf, u.D);
+              u.T_        = k.cexy * yDf;
+            }
+        }
+      return u;
+    }
+
+    // w = Dx u.D + Mx u.M + Kx u.K + Dx^T u.T with the x-band entries xD, xM, xK, xT of
+    // the output node at this column
+    template <typename T>
+    __device__ __forceinline__ T
+    xcontract(const T xM, const T xD, const T xK, const T xT, const Cols<T> &u, const Coef<T> &k)
+    {
+      if (!k.nx)
+        return fma_t(xM, u.M, T(0)); // not a bare product: no kernel may fuse it into the sum that follows
+      T w = xD * u.D;
+      if (k.ny)
+        w = fma_t(xM, u.M, w);
+      w = fma_t(xK, u.K, w);
+      if (k.ny)
+        w = fma_t(xT, u.T_, w);
+      return w;
+    }
+  } // namespace q1
+} // namespace lbfem_amd

@@ -32,6 +32,8 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "march_kernel.hpp"
+#include "q1_column.hpp"
 
 namespace lbfem_amd
 {
@@ -46,18 +48,10 @@ namespace lbfem_amd
       return ((const __attribute__((address_space(1))) double *)B)[i * (2 * p + 1) + o + p];
     }
 
-    // global node line of a tile position, -1 outside a non-periodic axis. A
-    // periodic position lies in [-p, P + TX + p): positions past P + p feed no
-    // owned node and map to any line inside.
     __device__ __forceinline__ int
     wrap(const int i, const int P, const int periodic)
     {
-      if (periodic)
-        {
-          const int w = i < 0 ? i + P : (i >= P ? i - P : i);
-          return w < P ? w : P - 1;
-        }
-      return (i < 0 || i > P) ? -1 : i;
+      return wrap_line(i, P, periodic);
     }
 
     // A tile variant: T the storage and arithmetic type of the populations
@@ -139,11 +133,15 @@ namespace lbfem_amd
       using T                   = typename C::T;
       constexpr int  p          = C::p, TX = C::TX, PS = C::PS;
       constexpr int  W          = C::W, HX = C::HX, HY = C::HY, NH = C::NH, NT = C::NT, R = C::R, NF = C::NF;
-      constexpr bool LL         = C::LL, STEP = C::STEP, WGT = C::WGT;
+      constexpr bool LL         = C::LL, STEP = C::STEP;
       // the mixed form applies the operator to f - f(node): every term annihilates
       // constants (its bands' row or column sums vanish), and FP32 then carries the
       // small differences instead of cancelling large values
       constexpr bool CEN        = sizeof(T) == 4;
+      // Q1: FP64 in the column form shared with the row-march kernel (q1_column.hpp),
+      // FP32 with per-node 3 x 3 weights
+      constexpr bool COL        = C::WGT && !CEN;
+      constexpr bool WGT        = C::WGT && CEN;
       constexpr int  PX         = C::PX;
       T(&F)[NF][HY][PX]         = sm.F;
       // the population buffers are device (global) memory: say so, so that the loads and
@@ -207,7 +205,7 @@ namespace lbfem_amd
                   fe[a] = gfeq[a * G.stride + node_index(G, gis[r], gjs[r])];
 #pragma unroll
               for (int a = 1; a < Q; ++a)
-                F[Q + a - 1][ky][kx] = a_c * v[r][a] + b_c * fe[a];
+                F[Q + a - 1][ky][kx] = q1::fma_t(a_c, v[r][a], b_c * fe[a]);
               if (r == 0)
 #pragma unroll
                 for (int a = 0; a < Q; ++a)
@@ -273,134 +271,167 @@ namespace lbfem_amd
       // run-time values: compile-time D2Q9 constants let the scheduler interleave the
       // barrier-free populations and spill registers. A population at rest in the sweep
       // gets r = 0 (streaming.h:118-122).
-      auto popA = [&](const int al, T &res) {
-        const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
-        const bool nx = ex != T(0), ny = ey != T(0);
-        res           = T(0);
-        const T(*Ff)[PX] = F[al];
-        const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
-        const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
-        const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
-        if constexpr (WGT)
-          {
-            if (!nx)
-              return; // y class: phase B
-            T hx = 0, fx = 0, fb = 0;
-            if (!ny)
-              {
-#pragma unroll
-                for (int c = 0; c < W; ++c)
-#pragma unroll
-                  for (int o = 0; o < W; ++o)
-                    {
-                      if (CEN && c == p && o == p)
-                        continue; // f - f(node) vanishes there
-                      const T fv = Ff[jj + o][ii + c] - cf;
-                      const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
-                      hx += wgx[c][o] * hv;
-                      fx += wkx[c][o] * fv;
-                    }
-                res = ex * hx + (ex * ex) * fx;
-              }
-            else
-              {
-#pragma unroll
-                for (int c = 0; c < W; ++c)
-#pragma unroll
-                  for (int o = 0; o < W; ++o)
-                    {
-                      if (CEN && c == p && o == p)
-                        continue;
-                      const T fv = Ff[jj + o][ii + c] - cf;
-                      const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
-                      hx += wgx[c][o] * hv;
-                      fx += wkx[c][o] * fv;
-                      fb += wbb[c][o] * fv;
-                    }
-                res = (ex * hx + (ex * ex) * fx) + (ex * ey) * fb;
-              }
-          }
+      // the column form (Q1, FP64): the whole increment of population al
+      auto col_pop = [&](const int al) -> T {
+        if constexpr (!COL)
+          return T(0);
         else
           {
-            if (!nx && !ny)
-              return;
-            // Q2, Q3: y bands over the columns' strips, then the column's x bands
-            T acc = 0;
-            if (nx && !ny)
+            const q1::Coef<T> k = q1::coef<T>(T(S.e[al][0]), T(S.e[al][1]), sa, c_c);
+            if (!k.nx && !k.ny)
+              return T(0);
+            const T(*Ff)[PX] = F[al];
+            const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
+            T wk[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
               {
-                const T wd = ex * sa, wk = c_c * ex * ex;
+                T fv[3], hv[3];
 #pragma unroll
-                for (int c = 0; c < W; ++c)
+                for (int o = 0; o < 3; ++o)
                   {
-                    T My = 0, Myh = 0;
-#pragma unroll
-                    for (int o = 0; o < W; ++o)
-                      {
-                        My += yc[0][o] * (Ff[jj + o][ii + c] - cf);
-                        if constexpr (LL)
-                          Myh += yc[0][o] * (Fh[jj + o][ii + c] - ch);
-                      }
-                    if constexpr (!LL)
-                      Myh = My;
-                    acc += (xc[1][c] * wd) * Myh + (xc[2][c] * wk) * My;
+                    fv[o] = Ff[jj + o][ii + c];
+                    hv[o] = LL ? Fh[jj + o][ii + c] : fv[o];
                   }
+                const q1::Cols<T> u = q1::columns<T, LL>(yc, fv, hv, k);
+                wk[c]               = q1::xcontract(xc[0][c], xc[1][c], xc[2][c], xc[3][c], u, k);
               }
-            else if (!nx && ny)
+            return (wk[0] + wk[1]) + wk[2];
+          }
+      };
+      auto popA = [&](const int al, T &res) {
+        if constexpr (COL)
+          res = col_pop(al);
+        else
+          {
+            const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
+            const bool nx = ex != T(0), ny = ey != T(0);
+            res           = T(0);
+            const T(*Ff)[PX] = F[al];
+            const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
+            const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
+            const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
+            if constexpr (WGT)
               {
-                const T wd = ey * sa, wk = c_c * ey * ey;
-                T       y1[W], y2[W];
-#pragma unroll
-                for (int o = 0; o < W; ++o)
+                if (!nx)
+                  return; // y class: phase B
+                T hx = 0, fx = 0, fb = 0;
+                if (!ny)
                   {
-                    y1[o] = yc[1][o] * wd;
-                    y2[o] = yc[2][o] * wk;
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
+    #pragma unroll
+                      for (int o = 0; o < W; ++o)
+                        {
+                          if (CEN && c == p && o == p)
+                            continue; // f - f(node) vanishes there
+                          const T fv = Ff[jj + o][ii + c] - cf;
+                          const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                          hx += wgx[c][o] * hv;
+                          fx += wkx[c][o] * fv;
+                        }
+                    res = ex * hx + (ex * ex) * fx;
                   }
-#pragma unroll
-                for (int c = 0; c < W; ++c)
+                else
                   {
-                    T s = 0;
-#pragma unroll
-                    for (int o = 0; o < W; ++o)
-                      {
-                        const T fv = Ff[jj + o][ii + c] - cf;
-                        const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
-                        s += y1[o] * hv + y2[o] * fv;
-                      }
-                    acc += xc[0][c] * s;
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
+    #pragma unroll
+                      for (int o = 0; o < W; ++o)
+                        {
+                          if (CEN && c == p && o == p)
+                            continue;
+                          const T fv = Ff[jj + o][ii + c] - cf;
+                          const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                          hx += wgx[c][o] * hv;
+                          fx += wkx[c][o] * fv;
+                          fb += wbb[c][o] * fv;
+                        }
+                    res = (ex * hx + (ex * ex) * fx) + (ex * ey) * fb;
                   }
               }
             else
               {
-                const T wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
-#pragma unroll
-                for (int c = 0; c < W; ++c)
+                if (!nx && !ny)
+                  return;
+                // Q2, Q3: y bands over the columns' strips, then the column's x bands
+                T acc = 0;
+                if (nx && !ny)
                   {
-                    T My = 0, Dy = 0, Ky = 0, DTy = 0, Myh = 0, Dyh = 0;
-#pragma unroll
+                    const T wd = ex * sa, wk = c_c * ex * ex;
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
+                      {
+                        T My = 0, Myh = 0;
+    #pragma unroll
+                        for (int o = 0; o < W; ++o)
+                          {
+                            My += yc[0][o] * (Ff[jj + o][ii + c] - cf);
+                            if constexpr (LL)
+                              Myh += yc[0][o] * (Fh[jj + o][ii + c] - ch);
+                          }
+                        if constexpr (!LL)
+                          Myh = My;
+                        acc += (xc[1][c] * wd) * Myh + (xc[2][c] * wk) * My;
+                      }
+                  }
+                else if (!nx && ny)
+                  {
+                    const T wd = ey * sa, wk = c_c * ey * ey;
+                    T       y1[W], y2[W];
+    #pragma unroll
                     for (int o = 0; o < W; ++o)
                       {
-                        const T fv = Ff[jj + o][ii + c] - cf;
-                        My += yc[0][o] * fv;
-                        Dy += yc[1][o] * fv;
-                        Ky += yc[2][o] * fv;
-                        DTy += yc[3][o] * fv;
-                        if constexpr (LL)
-                          {
-                            const T hv = Fh[jj + o][ii + c] - ch;
-                            Myh += yc[0][o] * hv;
-                            Dyh += yc[1][o] * hv;
-                          }
+                        y1[o] = yc[1][o] * wd;
+                        y2[o] = yc[2][o] * wk;
                       }
-                    if constexpr (!LL)
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
                       {
-                        Myh = My;
-                        Dyh = Dy;
+                        T s = 0;
+    #pragma unroll
+                        for (int o = 0; o < W; ++o)
+                          {
+                            const T fv = Ff[jj + o][ii + c] - cf;
+                            const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                            s += y1[o] * hv + y2[o] * fv;
+                          }
+                        acc += xc[0][c] * s;
                       }
-                    acc += xc[1][c] * (wdx * Myh + wx * DTy) + xc[2][c] * (wk * My) +
-                           xc[0][c] * (wdy * Dyh + wky * Ky) + xc[3][c] * (wx * Dy);
                   }
+                else
+                  {
+                    const T wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
+                      {
+                        T My = 0, Dy = 0, Ky = 0, DTy = 0, Myh = 0, Dyh = 0;
+    #pragma unroll
+                        for (int o = 0; o < W; ++o)
+                          {
+                            const T fv = Ff[jj + o][ii + c] - cf;
+                            My += yc[0][o] * fv;
+                            Dy += yc[1][o] * fv;
+                            Ky += yc[2][o] * fv;
+                            DTy += yc[3][o] * fv;
+                            if constexpr (LL)
+                              {
+                                const T hv = Fh[jj + o][ii + c] - ch;
+                                Myh += yc[0][o] * hv;
+                                Dyh += yc[1][o] * hv;
+                              }
+                          }
+                        if constexpr (!LL)
+                          {
+                            Myh = My;
+                            Dyh = Dy;
+                          }
+                        acc += xc[1][c] * (wdx * Myh + wx * DTy) + xc[2][c] * (wk * My) +
+                               xc[0][c] * (wdy * Dyh + wky * Ky) + xc[3][c] * (wx * Dy);
+                      }
+                  }
+                res = acc;
               }
-            res = acc;
           }
       };
       // phase B (Q1 only): adds the y parts with the weights sa M_x (x) D_y, c M_x (x) K_y;
@@ -454,13 +485,13 @@ namespace lbfem_amd
         // On Q1 the bands are read again here (wall nodes only), so that they need no
         // registers next to the weights.
         auto xb = [&](const int b, const int o) -> T {
-          if constexpr (WGT)
+          if constexpr (C::WGT)
             return T(band<p>(b == 0 ? G.ax.M : G.ax.D, gi, o - p));
           else
             return xc[b][o];
         };
         auto yb = [&](const int b, const int o) -> T {
-          if constexpr (WGT)
+          if constexpr (C::WGT)
             return T(band<p>(b == 0 ? G.ay.M : G.ay.D, gj, o - p));
           else
             return yc[b][o];
@@ -480,11 +511,11 @@ namespace lbfem_amd
                     T dn = 0;
 #pragma unroll
                     for (int l = 0; l <= p; ++l)
-                      dn += T(bn[l]) * Ff[r0 + l][ii + o];
-                    t += xb(1, o) * Ff[ty][ii + o];
-                    nrm += xb(0, o) * dn;
+                      dn = q1::fma_t(T(bn[l]), Ff[r0 + l][ii + o], dn);
+                    t   = q1::fma_t(xb(1, o), Ff[ty][ii + o], t);
+                    nrm = q1::fma_t(xb(0, o), dn, nrm);
                   }
-                fc += hdt2 * (lo ? -ey : ey) * (ex * t + ey * nrm);
+                fc = q1::fma_t(hdt2 * (lo ? -ey : ey), q1::fma_t(ex, t, ey * nrm), fc);
               }
             if (!G.ax.periodic && nx && (gi == 0 || gi == P))
               {
@@ -498,11 +529,11 @@ namespace lbfem_amd
                     T dn = 0;
 #pragma unroll
                     for (int l = 0; l <= p; ++l)
-                      dn += T(bn[l]) * Ff[jj + o][c0 + l];
-                    t += yb(1, o) * Ff[jj + o][tx];
-                    nrm += yb(0, o) * dn;
+                      dn = q1::fma_t(T(bn[l]), Ff[jj + o][c0 + l], dn);
+                    t   = q1::fma_t(yb(1, o), Ff[jj + o][tx], t);
+                    nrm = q1::fma_t(yb(0, o), dn, nrm);
                   }
-                fc += hdt2 * (lo ? -ex : ex) * (ex * nrm + ey * t);
+                fc = q1::fma_t(hdt2 * (lo ? -ex : ex), q1::fma_t(ex, nrm, ey * t), fc);
               }
           }
         return fc;
@@ -731,6 +762,18 @@ namespace lbfem_amd
           return launch_tile<tile::Cfg<T, p, 48, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileSplit2W40:
           return launch_tile<tile::Cfg<T, p, 40, 8, 2, MODE>>(G, S, fin, feq, out, rhs, stream);
+        case kMarch1:
+        case kMarch4:
+        case kMarch16:
+        case kMarch64:
+          if constexpr (p == 1 && sizeof(T) == 8)
+            {
+              if (standard_directions(S))
+                return launch_march<T, MODE>(march_rows(variant), G, S, fin, feq, out, rhs, stream);
+              return launch_tile<tile::Cfg<T, p, 32, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
+            }
+          else
+            return launch_tile<tile::Cfg<T, p, 32, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
         default:
           return hipErrorInvalidValue;
       }
