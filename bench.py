@@ -32,16 +32,14 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 Q = 9
 # kernel names of the tile variants (Cfg<T, p, TX, TY, PS, mode>), fused Lee-Lin step, Q1; variants 8-11
-# are the row-march kernel (FP64 Q1; elsewhere they run as variant 0)
+# are the row-march kernel (Q1; elsewhere they run as variant 0)
 TILE_SHAPES = ((32, 8, 1), (32, 16, 1), (32, 8, 2), (32, 8, 4), (16, 8, 4), (40, 8, 1), (48, 8, 1), (40, 8, 2))
 N_VARIANTS = 12
 
 
 def tile_kernel(variant, t="double"):
     if variant >= len(TILE_SHAPES):
-        if t == "double":
-            return "k_march<double, 1>"
-        variant = 0
+        return f"k_march<{t}, 1>"
     tx, ty, ps = TILE_SHAPES[variant]
     return f"k_tile<Cfg<{t}, 1, {tx}, {ty}, {ps}, 1> >"
 

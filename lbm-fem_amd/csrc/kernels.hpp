@@ -40,12 +40,12 @@ namespace lbfem_amd
     kTileWide48   = 6, // 48 x 8 nodes, 384 threads
     kTileSplit2W40 = 7, // 40 x 8 nodes, 640 threads, split over 2 (about 1.5 node-waves per SIMD at 257^2)
     kTileLdsVariants = 8, // the variants above stage the halo'd tile in LDS (the batch kernel's shapes)
-    // the row-march kernel (Q1 FP64; elsewhere these run as kTileWide): wavefront strips of 62
-    // columns marching up segments of 1, 4, 16 or 64 node rows
+    // the row-march kernel (Q1; elsewhere these run as kTileWide): wavefront strips of 62
+    // columns marching up segments of 1, 4, 8 or 16 node rows
     kMarch1       = 8,
     kMarch4       = 9,
-    kMarch16      = 10,
-    kMarch64      = 11,
+    kMarch8       = 10,
+    kMarch16      = 11,
     kTileVariants = 12
   };
 
@@ -53,7 +53,7 @@ namespace lbfem_amd
   inline int
   march_rows(const int variant)
   {
-    return variant == kMarch1 ? 1 : variant == kMarch4 ? 4 : variant == kMarch16 ? 16 : 64;
+    return variant == kMarch1 ? 1 : variant == kMarch4 ? 4 : variant == kMarch8 ? 8 : 16;
   }
 
   template <typename T, int p>
@@ -157,20 +157,25 @@ namespace lbfem_amd
   // The variant used until a measurement picks one, by the node count of the launch: below
   // about four wide-tile wavefronts per SIMD of the device (256 CUs x 4 SIMDs) the 40 x 8 tile
   // split over two threads per node (the fastest measured shape at 257^2 nodes, Q1 and Q2,
-  // FP64 and FP32); above, in FP64 the row-march kernel over 4-row segments (Q1: 18-27 %
-  // ahead of the 32 x 8 tile at 1025^2 and 2049^2 nodes, both schemes; on Q2/Q3 it runs as
-  // the 32 x 8 tile, which is 20-30 % ahead of 32 x 16 there) and in FP32 the 32 x 16 tile.
+  // FP64 and FP32); above, the row-march kernel with about 256 row segments per strip (the
+  // fastest height measured: 1 row at 513^2 nodes, 4 at 1025^2, 8 at 2049^2, 16 at 4097^2;
+  // Lee-Lin FP64 6 %, 20 %, 33 % and 42 % ahead of the 32 x 8 tile there, FP32 28-49 % ahead
+  // of 32 x 16 from 1025^2).
+  // On Q2/Q3 the march variants run as the 32 x 8 tile, 20-30 % ahead of 32 x 16 in FP64 and
+  // 2 % behind it in FP32.
   inline int
-  size_default_variant(const long long nodes, const bool fp32)
+  size_default_variant(const long long nodes, const bool)
   {
-    return nodes < 64LL * 4 * 1024 ? kTileSplit2W40 : fp32 ? kTileTall : kMarch4;
+    if (nodes < 64LL * 4 * 1024)
+      return kTileSplit2W40;
+    return nodes < 768LL * 768 ? kMarch1 : nodes < 1536LL * 1536 ? kMarch4 : nodes < 3072LL * 3072 ? kMarch8 : kMarch16;
   }
-  // the same among the shapes a batch runs (the LDS tile shapes): the 32 x 8 tile in FP64
+  // the same among the shapes a batch runs (the LDS tile shapes): 32 x 8 in FP64, 32 x 16 in FP32
   inline int
   batch_default_variant(const long long nodes, const bool fp32)
   {
     const int v = size_default_variant(nodes, fp32);
-    return v >= kTileLdsVariants ? kTileWide : v;
+    return v < kTileLdsVariants ? v : fp32 ? kTileTall : kTileWide;
   }
   inline int
   tile_default_variant(const Geo &G, const bool fp32)

@@ -145,8 +145,10 @@ namespace lbfem_amd
 
     const int lane = threadIdx.x & 63;
     const int wid  = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // XCD-aware block order (consecutive blocks of the list on one XCD), then the block's
-    // wavefronts on consecutive row segments of a strip
+    // XCD-aware block order (consecutive blocks of the list on one XCD); the wavefronts of the list
+    // run along a band of row segments, so that a block's wavefronts are neighbouring strips of one
+    // segment: they read the same rows at about the same time and share the halo columns' cache
+    // lines in L2
     const int nwave = S.tiles_x * S.tiles_y;
     const int nb    = (nwave + kWaves - 1) / kWaves;
     const int nb8   = (nb + 7) & ~7;
@@ -155,7 +157,7 @@ namespace lbfem_amd
     const int gw    = tb * kWaves + wid;
     if (tb >= nb || gw >= nwave)
       return;
-    const int strip = gw / S.tiles_y, seg = gw % S.tiles_y;
+    const int strip = gw % S.tiles_x, seg = gw / S.tiles_x;
     const int P = G.P, H = S.rows;
     const int jb = seg * H, je = min(jb + H, P + 1);
     const int x  = strip * kOut - 1 + lane; // this lane's column position
@@ -305,6 +307,8 @@ namespace lbfem_amd
       }
       const bool ywall = !G.ay.periodic && (j == 0 || j == P);
       const int  ln    = opaque(lane);
+      // FP32: the row sums of the M and D^T bands (the centred form's row terms)
+      const T mM = q1::row_sum(yb[0]), mT = q1::row_sum(yb[3]);
       T          xinc[NM], mp[NM / 2];
       // the D2Q9 directions as compile-time constants (the launcher runs this kernel for the
       // standard direction set only): the coefficients fold to +-sa, +-c, c, exactly the
@@ -321,11 +325,24 @@ namespace lbfem_amd
 #pragma unroll
               for (int o = 0; o < 3; ++o)
                 hv[o] = LL ? (o == 0 ? lo.h[al - 1] : o == 1 ? mid.h[al - 1] : hi.h[al - 1]) : fv[o];
-              const q1::Cols<T> u = q1::columns<T, LL>(yb, fv, hv, k);
-              const auto &xr      = xbl[wid];
-              const T w0 = q1::xcontract(xr[0][0][ln], xr[1][0][ln], xr[2][0][ln], xr[3][0][ln], u, k);
-              const T w1 = q1::xcontract(xr[0][1][ln], xr[1][1][ln], xr[2][1][ln], xr[3][1][ln], u, k);
-              const T w2 = q1::xcontract(xr[0][2][ln], xr[1][2][ln], xr[2][2][ln], xr[3][2][ln], u, k);
+              q1::Cols<T> u0, u1, u2;
+              if constexpr (sizeof(T) == 8)
+                u0 = u1 = u2 = q1::columns<T, LL>(yb, fv, hv, k);
+              else
+                {
+                  // centred (FP32): the column's contractions once, the row terms per fed node with
+                  // the neighbouring columns' centre values
+                  const q1::CCols<T> cc = q1::ccolumns<T, LL>(yb, fv, hv, k);
+                  const T            fR = from_above(fv[1]), fL = from_below(fv[1]);
+                  const T            hR = LL ? from_above(hv[1]) : fR, hL = LL ? from_below(hv[1]) : fL;
+                  u0 = q1::assemble<T, LL>(cc, mM, mT, hv[1] - hR, fv[1] - fR, true, k);
+                  u1 = q1::assemble<T, LL>(cc, mM, mT, T(0), T(0), false, k);
+                  u2 = q1::assemble<T, LL>(cc, mM, mT, hv[1] - hL, fv[1] - fL, true, k);
+                }
+              const auto &xr = xbl[wid];
+              const T w0 = q1::xcontract(xr[0][0][ln], xr[1][0][ln], xr[2][0][ln], xr[3][0][ln], u0, k);
+              const T w1 = q1::xcontract(xr[0][1][ln], xr[1][1][ln], xr[2][1][ln], xr[3][1][ln], u1, k);
+              const T w2 = q1::xcontract(xr[0][2][ln], xr[1][2][ln], xr[2][2][ln], xr[3][2][ln], u2, k);
               r                    = (from_below(w0) + w1) + from_above(w2);
             }
           // wall surface term (advection.h:177-199), as tile::body's face()
@@ -498,13 +515,17 @@ namespace lbfem_amd
     };
 
     // ------------------------------------------------------------------ the march
+    // start-up: the first three rows' loads go out together (one memory round trip, not three)
     R r0, r1, r2;
     T v[Q];
-    load(jb - 1, v);
-    prologue(jb - 1, v, r0);
-    load(jb, v);
-    prologue(jb, v, r1);
-    load(jb + 1, v);
+    {
+      T v0[Q], v1[Q];
+      load(jb - 1, v0);
+      load(jb, v1);
+      load(jb + 1, v);
+      prologue(jb - 1, v0, r0);
+      prologue(jb, v1, r1);
+    }
     // one row: the prologue of row j + 1 from its loads (issued during the previous row), the
     // operator of row j, the loads of row j + 2, the epilogue and stores of row j
     auto advance = [&](const int j, const R &lo, const R &mid, R &hi) {

@@ -18,7 +18,7 @@
 // once and passes w_0 and w_2 to the neighbouring lanes; the tile kernel
 // computes all three columns per node. Terms of a population that vanish by its
 // direction class (x: ey = 0, y: ex = 0, diagonal) are skipped the same way in
-// both.
+// both. The FP32 form (below) centres the taps to carry differences.
 #pragma once
 
 #include "device.hpp"
@@ -123,6 +123,90 @@ namespace lbfem_amd
       if (k.ny)
         w = fma_t(xT, u.T_, w);
       return w;
+    }
+
+    // ------------------------------------------------------------------ FP32 (centred) form
+    // The mixed form applies the operator to differences, so that FP32 carries the small
+    // variations of f instead of cancelling large values. Centred on the output node (i, j),
+    //   sum_o y[o] (v(x, j + o) - v(i, j)) = sum_o y[o] (v(x, j + o) - v(x, j)) + m (v(x, j) - v(i, j)),
+    // m = y[0] + y[1] + y[2]: the first part depends on the column only (shared by the three
+    // nodes a column feeds), the second is a row term. m is taken as 0 for the D and K bands
+    // (their rows sum to zero: every term of the operator then annihilates constants exactly)
+    // and formed from the entries for M and D^T (D^T rows sum to -1 and 1 on the bottom and
+    // top boundary rows, to 0 elsewhere).
+
+    // sum_o y[o] (v[o] - v[1])
+    template <typename T>
+    __device__ __forceinline__ T
+    chain_c(const T (&y)[3], const T (&v)[3])
+    {
+      return fma_t(y[2], v[2] - v[1], y[0] * (v[0] - v[1]));
+    }
+
+    // the column-centred contractions of one population at one column
+    template <typename T>
+    struct CCols
+    {
+      T Mh, Dh, Mf, Kf, Tf, Df;
+    };
+
+    template <typename T, bool LL>
+    __device__ __forceinline__ CCols<T>
+    ccolumns(const T (&yb)[4][3], const T (&fv)[3], const T (&hv)[3], const Coef<T> &k)
+    {
+      CCols<T> c{T(0), T(0), T(0), T(0), T(0), T(0)};
+      if (k.nx)
+        {
+          c.Mh = chain_c(yb[0], hv);
+          c.Mf = LL ? chain_c(yb[0], fv) : c.Mh;
+        }
+      if (k.ny)
+        {
+          c.Dh = chain_c(yb[1], hv);
+          c.Kf = chain_c(yb[2], fv);
+          if (k.nx)
+            {
+              c.Tf = chain_c(yb[3], fv);
+              c.Df = LL ? chain_c(yb[1], fv) : c.Dh;
+            }
+        }
+      return c;
+    }
+
+    // the column values for the node this column feeds at offset k: the row terms with dh =
+    // h(x, j) - h(i, j) and df = f(x, j) - f(i, j) (none at k = 1, where both vanish); mM and mT
+    // the row sums of the M and D^T bands
+    template <typename T, bool LL>
+    __device__ __forceinline__ Cols<T>
+    assemble(const CCols<T> &c, const T mM, const T mT, const T dh, const T df, const bool row, const Coef<T> &k)
+    {
+      Cols<T> u{T(0), T(0), T(0), T(0)};
+      if (k.nx)
+        {
+          const T yMh = row ? fma_t(mM, dh, c.Mh) : c.Mh;
+          const T yMf = LL ? (row ? fma_t(mM, df, c.Mf) : c.Mf) : yMh;
+          u.D         = k.exsa * yMh;
+          u.K         = k.cexx * yMf;
+        }
+      if (k.ny)
+        {
+          u.M = fma_t(k.ceyy, c.Kf, k.eysa * c.Dh);
+          if (k.nx)
+            {
+              const T yTf = (row && mT != T(0)) ? fma_t(mT, df, c.Tf) : c.Tf;
+              u.D         = fma_t(k.cexy, yTf, u.D);
+              u.T_        = k.cexy * c.Df;
+            }
+        }
+      return u;
+    }
+
+    // the row sum of a y band, as both kernels form it
+    template <typename T>
+    __device__ __forceinline__ T
+    row_sum(const T (&y)[3])
+    {
+      return (y[0] + y[1]) + y[2];
     }
   } // namespace q1
 } // namespace lbfem_amd

@@ -70,7 +70,7 @@ namespace lbfem_amd
       static constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
       static constexpr int  NF   = LL ? Q + NM : Q;      // f (or g*) and, for Lee-Lin, h of the moving populations
       static constexpr int  R    = (NH + NT - 1) / NT;   // prologue rounds
-      static constexpr bool WGT  = p == 1;               // the operator as 3 x 3 node weights
+      static constexpr bool WGT  = p == 1;               // Q1: the operator in the column form
       // LDS row pitch of the staged planes. The LDS serves 128 bytes a cycle (LB lanes of T);
       // where a row of TX threads does not end on a multiple of LB lanes, a group of LB lanes
       // spans two rows and hits the same banks twice unless the pitch is congruent to TX
@@ -138,10 +138,8 @@ namespace lbfem_amd
       // constants (its bands' row or column sums vanish), and FP32 then carries the
       // small differences instead of cancelling large values
       constexpr bool CEN        = sizeof(T) == 4;
-      // Q1: FP64 in the column form shared with the row-march kernel (q1_column.hpp),
-      // FP32 with per-node 3 x 3 weights
-      constexpr bool COL        = C::WGT && !CEN;
-      constexpr bool WGT        = C::WGT && CEN;
+      // Q1: the column form shared with the row-march kernel (q1_column.hpp), centred in FP32
+      constexpr bool COL        = C::WGT;
       constexpr int  PX         = C::PX;
       T(&F)[NF][HY][PX]         = sm.F;
       // the population buffers are device (global) memory: say so, so that the loads and
@@ -238,31 +236,6 @@ namespace lbfem_amd
           yc[2][o]      = iy ? T(band<p>(G.ay.K, gj, o - p)) : T(0);
           yc[3][o]      = iy ? T(band<p>(G.ay.DT, gj, o - p)) : T(0);
         }
-      // Q1: the node's weights of the first-order terms (sa D_x (x) M_y, sa M_x (x) D_y) and
-      // of the second-order terms (c K_x (x) M_y, c M_x (x) K_y, c (D_x (x) D_y^T + D_x^T (x) D_y)),
-      // [x offset][y offset], in two phases so that at most three sets are live: phase A
-      // the x-class populations whole and the diagonal ones' x and cross parts, phase B
-      // the y-class populations and the diagonal ones' y parts
-      constexpr int WW = WGT ? W : 1;
-      T             wgx[WW][WW], wkx[WW][WW], wbb[WW][WW], xm[WW], yd[WW], yk[WW];
-      if constexpr (WGT)
-        {
-#pragma unroll
-          for (int c = 0; c < W; ++c)
-            {
-              const T xd = sa * xc[1][c], xk = c_c * xc[2][c], xcd = c_c * xc[1][c], xct = c_c * xc[3][c];
-#pragma unroll
-              for (int o = 0; o < W; ++o)
-                {
-                  wgx[c][o] = xd * yc[0][o];
-                  wkx[c][o] = xk * yc[0][o];
-                  wbb[c][o] = xcd * yc[3][o] + xct * yc[1][o];
-                }
-              xm[c] = xc[0][c];
-              yd[c] = sa * yc[1][c];
-              yk[c] = c_c * yc[2][c];
-            }
-        }
       __syncthreads();
 
       // ---------------------------------------------------------- populations 1..8
@@ -271,7 +244,7 @@ namespace lbfem_amd
       // run-time values: compile-time D2Q9 constants let the scheduler interleave the
       // barrier-free populations and spill registers. A population at rest in the sweep
       // gets r = 0 (streaming.h:118-122).
-      // the column form (Q1, FP64): the whole increment of population al
+      // the column form (Q1): the whole increment of population al
       auto col_pop = [&](const int al) -> T {
         if constexpr (!COL)
           return T(0);
@@ -282,6 +255,10 @@ namespace lbfem_amd
               return T(0);
             const T(*Ff)[PX] = F[al];
             const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
+            // FP32: the centre values of the node's row, and the row sums of its M and D^T bands
+            const T f0c = CEN ? Ff[jj + 1][ii + 1] : T(0);
+            const T h0c = (CEN && LL) ? Fh[jj + 1][ii + 1] : f0c;
+            const T mM = CEN ? q1::row_sum(yc[0]) : T(0), mT = CEN ? q1::row_sum(yc[3]) : T(0);
             T wk[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -293,8 +270,13 @@ namespace lbfem_amd
                     fv[o] = Ff[jj + o][ii + c];
                     hv[o] = LL ? Fh[jj + o][ii + c] : fv[o];
                   }
-                const q1::Cols<T> u = q1::columns<T, LL>(yc, fv, hv, k);
-                wk[c]               = q1::xcontract(xc[0][c], xc[1][c], xc[2][c], xc[3][c], u, k);
+                q1::Cols<T> u;
+                if constexpr (CEN)
+                  u = q1::assemble<T, LL>(q1::ccolumns<T, LL>(yc, fv, hv, k), mM, mT, hv[1] - h0c, fv[1] - f0c,
+                                          c != 1, k);
+                else
+                  u = q1::columns<T, LL>(yc, fv, hv, k);
+                wk[c] = q1::xcontract(xc[0][c], xc[1][c], xc[2][c], xc[3][c], u, k);
               }
             return (wk[0] + wk[1]) + wk[2];
           }
@@ -311,46 +293,6 @@ namespace lbfem_amd
             const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
             const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
             const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
-            if constexpr (WGT)
-              {
-                if (!nx)
-                  return; // y class: phase B
-                T hx = 0, fx = 0, fb = 0;
-                if (!ny)
-                  {
-    #pragma unroll
-                    for (int c = 0; c < W; ++c)
-    #pragma unroll
-                      for (int o = 0; o < W; ++o)
-                        {
-                          if (CEN && c == p && o == p)
-                            continue; // f - f(node) vanishes there
-                          const T fv = Ff[jj + o][ii + c] - cf;
-                          const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
-                          hx += wgx[c][o] * hv;
-                          fx += wkx[c][o] * fv;
-                        }
-                    res = ex * hx + (ex * ex) * fx;
-                  }
-                else
-                  {
-    #pragma unroll
-                    for (int c = 0; c < W; ++c)
-    #pragma unroll
-                      for (int o = 0; o < W; ++o)
-                        {
-                          if (CEN && c == p && o == p)
-                            continue;
-                          const T fv = Ff[jj + o][ii + c] - cf;
-                          const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
-                          hx += wgx[c][o] * hv;
-                          fx += wkx[c][o] * fv;
-                          fb += wbb[c][o] * fv;
-                        }
-                    res = (ex * hx + (ex * ex) * fx) + (ex * ey) * fb;
-                  }
-              }
-            else
               {
                 if (!nx && !ny)
                   return;
@@ -434,46 +376,6 @@ namespace lbfem_amd
               }
           }
       };
-      // phase B (Q1 only): adds the y parts with the weights sa M_x (x) D_y, c M_x (x) K_y;
-      // res holds phase A's value
-      T wgy[WW][WW], wky[WW][WW];
-      auto popB = [&](const int al, T &res) {
-        if constexpr (WGT)
-          {
-            const T ey = T(S.e[al][1]);
-            if (ey == T(0))
-              return;
-            const T(*Ff)[PX] = F[al];
-            const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
-            const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
-            const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
-            T       hy = 0, fy = 0;
-#pragma unroll
-            for (int c = 0; c < W; ++c)
-#pragma unroll
-              for (int o = 0; o < W; ++o)
-                {
-                  if (CEN && c == p && o == p)
-                    continue;
-                  const T fv = Ff[jj + o][ii + c] - cf;
-                  const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
-                  hy += wgy[c][o] * hv;
-                  fy += wky[c][o] * fv;
-                }
-            res = res + (ey * hy + (ey * ey) * fy);
-          }
-      };
-      auto weights_b = [&]() {
-        if constexpr (WGT)
-#pragma unroll
-          for (int c = 0; c < W; ++c)
-#pragma unroll
-            for (int o = 0; o < W; ++o)
-              {
-                wgy[c][o] = xm[c] * yd[o];
-                wky[c][o] = xm[c] * yk[o];
-              }
-      };
       // the wall surface term of population al at the owned node
       auto face = [&](const int al) -> T {
         const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
@@ -545,11 +447,9 @@ namespace lbfem_amd
 #pragma unroll
           for (int al = 1; al < Q; ++al)
             popA(al, xinc[al - 1]);
-          weights_b();
 #pragma unroll
           for (int al = 1; al < Q; ++al)
             {
-              popB(al, xinc[al - 1]);
               const T fc   = face(al);
               xinc[al - 1] = xinc[al - 1] + fc;
               if (al & 1)
@@ -564,12 +464,10 @@ namespace lbfem_amd
 #pragma unroll
           for (int k = 0; k < C::PPP; ++k)
             popA(q * C::PPP + k + 1, res[k]); // uniform over the wavefront
-          weights_b();
 #pragma unroll
           for (int k = 0; k < C::PPP; ++k)
             {
               const int al = q * C::PPP + k + 1;
-              popB(al, res[k]);
               const T fc           = face(al);
               sm.X[al - 1][jj][ii] = res[k] + fc;
               if ((k & 1) == 0)
@@ -764,9 +662,9 @@ namespace lbfem_amd
           return launch_tile<tile::Cfg<T, p, 40, 8, 2, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kMarch1:
         case kMarch4:
+        case kMarch8:
         case kMarch16:
-        case kMarch64:
-          if constexpr (p == 1 && sizeof(T) == 8)
+          if constexpr (p == 1)
             {
               if (standard_directions(S))
                 return launch_march<T, MODE>(march_rows(variant), G, S, fin, feq, out, rhs, stream);
