@@ -18,6 +18,10 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
     # packed FP32 arithmetic disabled for all device code of this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fused only within one source expression: every kernel that evaluates the same
+    # expressions rounds them the same way (all tile shapes, the row-march kernel and the batched
+    # kernels give bitwise identical populations), whatever code surrounds them
+    "-ffp-contract=on",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 

@@ -1141,6 +1141,9 @@ namespace
     int                        begin = 0;
     int                        tx, ty;
     tile_dims(g.variant, tx, ty);
+    // Q1 groups at a march variant list each member's wavefronts: 62-column strips by row segments
+    const bool march = g.p == 1 && g.variant >= kTileLdsVariants;
+    const int  rows  = march ? march_rows(g.variant) : 0;
     g.n_nodes = 0;
     for (size_t k = 0; k < g.members.size(); ++k)
       {
@@ -1150,8 +1153,9 @@ namespace
         E[k].G           = G;
         E[k].S           = tile_params(s, s.sweeps[0], ll);
         E[k].S.collide   = 1;
-        E[k].S.tiles_x   = (G.P + 1 + tx - 1) / tx;
-        E[k].S.tiles_y   = (G.P + 1 + ty - 1) / ty;
+        E[k].S.tiles_x   = march ? (G.P + 1 + kMarchOut - 1) / kMarchOut : (G.P + 1 + tx - 1) / tx;
+        E[k].S.tiles_y   = march ? (G.P + 1 + rows - 1) / rows : (G.P + 1 + ty - 1) / ty;
+        E[k].S.rows      = rows;
         if constexpr (sizeof(T) == 4)
           {
             E[k].f[0] = s.g32[0].p;
@@ -1168,7 +1172,8 @@ namespace
         g.n_nodes += int64_t(G.np) * G.np;
       }
     g.n_tiles            = begin;
-    const int nb8        = (g.n_tiles + 7) & ~7;
+    // one block per tile, or per four wavefronts of the march list
+    const int nb8        = ((march ? (g.n_tiles + 3) / 4 : g.n_tiles) + 7) & ~7;
     const int grid       = g.grid > 0 ? (g.grid + 7) & ~7 : std::max(8, nb8);
     if (grid != g.launch_grid)
       b.drop_graphs();
@@ -1594,7 +1599,8 @@ namespace
         it->mixed = s->mixed;
         it->grid  = b.shape_grid;
         it->variant =
-          b.shape_variant >= 0 ? b.shape_variant : batch_default_variant(int64_t(s->d->G.np) * s->d->G.np, s->mixed);
+          b.shape_variant >= 0 ? b.shape_variant
+                             : batch_default_variant(int64_t(s->d->G.np) * s->d->G.np, s->mixed, s->d->G.p);
       }
     it->members.push_back(s);
   }
@@ -2591,13 +2597,14 @@ extern "C"
       HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       for (int k = 0; k < n; ++k)
         join(*b, schemes[k]);
-      // default shape: the variant the size-based rule picks for the group's total node count
+      // default shape: the row-march kernel on Q1 groups, else the tile shape the size-based rule
+      // picks for the group's total node count
       for (auto &g : b->groups)
         {
           int64_t nodes = 0;
           for (const lbfem_scheme *m : g.members)
             nodes += int64_t(m->d->G.np) * m->d->G.np;
-          g.variant = batch_default_variant(nodes, g.mixed);
+          g.variant = batch_default_variant(nodes, g.mixed, g.p);
         }
       rebuild_tables(*b);
       for (auto &g : b->cg_groups)
@@ -2824,8 +2831,8 @@ extern "C"
   lbfem_batch_set_shape(lbfem_batch *b, const int variant, const int grid)
   {
     return guarded([&]() -> int {
-      if (!b || variant < -1 || variant >= kTileLdsVariants || grid < 0)
-        return fail(LBFEM_EINVAL, "invalid tile variant or grid (a batch runs the LDS tile shapes 0..7)");
+      if (!b || variant < -1 || variant >= kTileVariants || grid < 0)
+        return fail(LBFEM_EINVAL, "invalid tile variant or grid");
       HIP_CHECK(hipStreamSynchronize(b->stream));
       b->shape_variant = variant;
       b->shape_grid    = grid;
@@ -2859,7 +2866,7 @@ extern "C"
         {
           double best = 1e300;
           int    bv = g.variant, bg = g.grid;
-          for (int v = 0; v < kTileLdsVariants; ++v)
+          for (int v = 0; v < kTileVariants; ++v)
             for (const int grid : grids)
               {
                 // tile counts depend on the variant; launches at a fixed parity read one buffer and write the

@@ -49,6 +49,8 @@ namespace lbfem_amd
     kTileVariants = 12
   };
 
+  constexpr int kMarchOut = 62; // output columns per wavefront strip of the row-march kernel
+
   // node rows per wavefront segment of a row-march variant
   inline int
   march_rows(const int variant)
@@ -170,10 +172,14 @@ namespace lbfem_amd
       return kTileSplit2W40;
     return nodes < 768LL * 768 ? kMarch1 : nodes < 1536LL * 1536 ? kMarch4 : nodes < 3072LL * 3072 ? kMarch8 : kMarch16;
   }
-  // the same among the shapes a batch runs (the LDS tile shapes): 32 x 8 in FP64, 32 x 16 in FP32
+  // The default shape of a batch group: on Q1 the row-march kernel over 4-row segments (on the
+  // 96-instance seeded batch 19 % ahead of the best tile shape in FP32, 9 % in FP64); on Q2/Q3
+  // and for the consistent-mass groups' advection (tile lists only) the tile shapes above.
   inline int
-  batch_default_variant(const long long nodes, const bool fp32)
+  batch_default_variant(const long long nodes, const bool fp32, const int p = 0)
   {
+    if (p == 1)
+      return kMarch4;
     const int v = size_default_variant(nodes, fp32);
     return v < kTileLdsVariants ? v : fp32 ? kTileTall : kTileWide;
   }

@@ -37,7 +37,7 @@ namespace lbfem_amd
 {
   namespace march
   {
-    constexpr int kOut   = 62; // output columns per wavefront
+    constexpr int kOut   = kMarchOut; // output columns per wavefront
     constexpr int kWaves = 4;  // wavefronts per block (independent; no barrier)
 
     // a value from the lane below (lane l receives lane l - 1's; lane 0 its own)
@@ -125,12 +125,26 @@ namespace lbfem_amd
     }
   } // namespace march
 
-  template <typename T, int MODE>
-  __global__ void __launch_bounds__(64 * march::kWaves)
-  k_march(const Geo G, const TileParams S, const T *__restrict__ fin, const T *__restrict__ feq_in,
-          T *__restrict__ out, T *__restrict__ rhs_out)
+  namespace march
   {
-    using namespace march;
+    // the per-wavefront LDS of the kernel: a row's own f_0 and (Lee-Lin) equilibria by row parity,
+    // and the lane's x-band entries (read per row: no registers held)
+    template <typename T, bool LL>
+    struct Lds
+    {
+      static constexpr int NX = LL ? Q + 1 : 1;
+      T                    ext[kWaves][2][NX][64];
+      T                    xbl[kWaves][4][3][64];
+    };
+
+    // one wavefront's march: wavefront gw of the mesh's list (strip gw mod S.tiles_x, row segment
+    // gw / S.tiles_x of S.rows rows)
+    template <typename T, int MODE>
+    __device__ __forceinline__ void
+    body(const Geo &G, const TileParams &S, const T *__restrict__ fin, const T *__restrict__ feq_in,
+         T *__restrict__ out, T *__restrict__ rhs_out, const int gw, const int wid,
+         Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin> &lds)
+  {
     constexpr bool LL   = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
     constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
     constexpr int  NX   = LL ? Q + 1 : 1; // per-lane extras of a row: f_0 and (Lee-Lin) the equilibria
@@ -140,23 +154,10 @@ namespace lbfem_amd
     const GT *const gfeq = (const GT *)feq_in;
     GT *const       gout = (GT *)out;
     GT *const       grhs = (GT *)rhs_out;
-    __shared__ T    ext[kWaves][2][NX][64];
-    __shared__ T    xbl[kWaves][4][3][64]; // the lane's x-band entries (read per row: no registers held)
+    auto           &ext  = lds.ext;
+    auto           &xbl  = lds.xbl;
 
-    const int lane = threadIdx.x & 63;
-    const int wid  = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // XCD-aware block order (consecutive blocks of the list on one XCD); the wavefronts of the list
-    // run along a band of row segments, so that a block's wavefronts are neighbouring strips of one
-    // segment: they read the same rows at about the same time and share the halo columns' cache
-    // lines in L2
-    const int nwave = S.tiles_x * S.tiles_y;
-    const int nb    = (nwave + kWaves - 1) / kWaves;
-    const int nb8   = (nb + 7) & ~7;
-    const int bid   = blockIdx.x;
-    const int tb    = (bid & 7) * (nb8 >> 3) + (bid >> 3);
-    const int gw    = tb * kWaves + wid;
-    if (tb >= nb || gw >= nwave)
-      return;
+    const int lane  = threadIdx.x & 63;
     const int strip = gw % S.tiles_x, seg = gw / S.tiles_x;
     const int P = G.P, H = S.rows;
     const int jb = seg * H, je = min(jb + H, P + 1);
@@ -546,8 +547,65 @@ namespace lbfem_amd
         advance(j + 2, r2, r0, r1);
       }
   }
+  } // namespace march
 
-  // Launch of the row-march kernel over the mesh with H rows per wavefront segment.
+  // One mesh: XCD-aware block order (consecutive blocks of the list on one XCD); the wavefronts
+  // of the list run along a band of row segments, so that a block's wavefronts are neighbouring
+  // strips of one segment: they read the same rows at about the same time and share the halo
+  // columns' cache lines in L2.
+  template <typename T, int MODE>
+  __global__ void __launch_bounds__(64 * march::kWaves)
+  k_march(const Geo G, const TileParams S, const T *__restrict__ fin, const T *__restrict__ feq_in,
+          T *__restrict__ out, T *__restrict__ rhs_out)
+  {
+    using namespace march;
+    __shared__ Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin> lds;
+    const int wid   = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwave = S.tiles_x * S.tiles_y;
+    const int nb    = (nwave + kWaves - 1) / kWaves;
+    const int nb8   = (nb + 7) & ~7;
+    const int bid   = blockIdx.x;
+    const int tb    = (bid & 7) * (nb8 >> 3) + (bid >> 3);
+    const int gw    = tb * kWaves + wid;
+    if (tb >= nb || gw >= nwave)
+      return;
+    body<T, MODE>(G, S, fin, feq_in, out, rhs_out, gw, wid, lds);
+  }
+
+  // Many meshes of one degree and form (a batch group): the wavefronts of all members form one
+  // list (member k owns waves [tile_begin_k, tile_begin_k+1), its S.tiles_x strips by S.tiles_y
+  // row segments of S.rows rows); wavefront w of the grid takes list entries w, w + 4 gridDim.x,
+  // ... in the XCD-aware block order, so that any grid covers the list and a captured launch keeps
+  // working after members join or leave. Each member flips its own population buffers by parity.
+  template <typename T, int MODE>
+  __global__ void __launch_bounds__(64 * march::kWaves)
+  k_march_batch(const BatchEntry<T> *__restrict__ E, const BatchCounts *__restrict__ counts, const int parity)
+  {
+    using namespace march;
+    constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
+    __shared__ Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin> lds;
+    const int wid    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int n_inst = counts->n_inst, n_waves = counts->n_tiles;
+    const int nb8    = gridDim.x; // a multiple of 8
+    const int tb     = (blockIdx.x & 7) * (nb8 >> 3) + (blockIdx.x >> 3);
+    for (int gw = tb * kWaves + wid; gw < n_waves; gw += nb8 * kWaves)
+      {
+        int lo = 0, hi = n_inst - 1; // the member: last tile_begin <= gw
+        while (lo < hi)
+          {
+            const int mid = (lo + hi + 1) >> 1;
+            if (E[mid].tile_begin <= gw)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+        const BatchEntry<T> &e   = E[lo];
+        const int            cur = (e.par + parity) & 1;
+        T                   *out = (STEP || e.S.collide) ? e.f[cur ^ 1] : nullptr;
+        body<T, MODE>(e.G, e.S, e.f[cur], nullptr, out, e.rhs, gw - e.tile_begin, wid, lds);
+      }
+  }
+
   // the sweep's directions are the D2Q9 set (the row-march kernel has them built in)
   inline bool
   standard_directions(const TileParams &S)
@@ -558,6 +616,7 @@ namespace lbfem_amd
     return true;
   }
 
+  // Launch of the row-march kernel over the mesh with H rows per wavefront segment.
   template <typename T, int MODE>
   hipError_t
   launch_march(const int rows, const Geo &G, TileParams S, const T *fin, const T *feq, T *out, T *rhs,
@@ -570,6 +629,17 @@ namespace lbfem_amd
     const int nb   = (nw + march::kWaves - 1) / march::kWaves;
     const int nb8  = (nb + 7) & ~7;
     hipLaunchKernelGGL((k_march<T, MODE>), dim3(nb8), dim3(64 * march::kWaves), 0, stream, G, S, fin, feq, out, rhs);
+    return hipGetLastError();
+  }
+
+  // grid: blocks of the batch launch (a multiple of 8; its wavefronts stride over the list)
+  template <typename T, int MODE>
+  hipError_t
+  launch_march_batch(const BatchEntry<T> *table, const BatchCounts *counts, const int parity, const int grid,
+                     hipStream_t stream)
+  {
+    hipLaunchKernelGGL((k_march_batch<T, MODE>), dim3(grid), dim3(64 * march::kWaves), 0, stream, table, counts,
+                       parity);
     return hipGetLastError();
   }
 } // namespace lbfem_amd

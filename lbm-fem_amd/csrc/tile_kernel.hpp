@@ -700,6 +700,15 @@ namespace lbfem_amd
           return launch_batch<tile::Cfg<T, p, 48, 8, 1, MODE>>(table, counts, parity, grid, stream);
         case kTileSplit2W40:
           return launch_batch<tile::Cfg<T, p, 40, 8, 2, MODE>>(table, counts, parity, grid, stream);
+        case kMarch1:
+        case kMarch4:
+        case kMarch8:
+        case kMarch16:
+          // the table holds each member's march decomposition (Q1) or its 32 x 8 tiles
+          if constexpr (p == 1)
+            return launch_march_batch<T, MODE>(table, counts, parity, grid, stream);
+          else
+            return launch_batch<tile::Cfg<T, p, 32, 8, 1, MODE>>(table, counts, parity, grid, stream);
         default:
           return hipErrorInvalidValue;
       }

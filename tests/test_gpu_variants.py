@@ -34,6 +34,11 @@ CASES = [
     ("tgv", 11, 2, A.BARDOW, A.MASS_LUMPED, A.MIXED),
     ("cavity", 21, 1, A.LEELIN, A.MASS_CG, A.FP64),
     ("cavity", 10, 2, A.BARDOW, A.MASS_CG, A.FP64),
+    # periodic meshes with Lee-Lin (the slave lines' predictor-corrector), both precisions
+    ("tgv", 20, 1, A.LEELIN, A.MASS_LUMPED, A.FP64),
+    ("tgv", 20, 1, A.LEELIN, A.MASS_LUMPED, A.MIXED),
+    ("couette", 70, 1, A.LEELIN, A.MASS_LUMPED, A.FP64),
+    ("couette", 70, 1, A.BARDOW, A.MASS_LUMPED, A.MIXED),
 ]
 
 
