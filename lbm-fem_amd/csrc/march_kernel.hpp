@@ -274,7 +274,7 @@ namespace lbfem_amd
 #pragma unroll
             for (int a = 0; a < Q; ++a)
               fe[a] = T(0);
-          else if (feq_in == nullptr)
+          else if (STEP || feq_in == nullptr) // the step modes compute the equilibria (given: advection only)
             equilibrium_node(v, fe, wall, wx, wy);
           else
 #pragma unroll
@@ -307,10 +307,15 @@ namespace lbfem_amd
             yb[b][o] = T(band_uniform(ybp[b], j, o, P));
       }
       const bool ywall = !G.ay.periodic && (j == 0 || j == P);
-      const int  ln    = opaque(lane);
       // FP32: the row sums of the M and D^T bands (the centred form's row terms)
+      const int  ln    = opaque(lane);
       const T mM = q1::row_sum(yb[0]), mT = q1::row_sum(yb[3]);
-      T          xinc[NM], mp[NM / 2];
+      T          xinc[NM];
+      // the next row's loads go out before the operator: in flight for the whole operator and
+      // epilogue (their registers are those the prologue of this row's successor consumed), and
+      // ahead of this row's stores, so that waiting for them does not wait for the stores (the
+      // vector memory counter counts both)
+      between();
       // the D2Q9 directions as compile-time constants (the launcher runs this kernel for the
       // standard direction set only): the coefficients fold to +-sa, +-c, c, exactly the
       // products the tile kernel forms at run time, and no per-population value stays live
@@ -346,68 +351,83 @@ namespace lbfem_amd
               const T w2 = q1::xcontract(xr[0][2][ln], xr[1][2][ln], xr[2][2][ln], xr[3][2][ln], u2, k);
               r                    = (from_below(w0) + w1) + from_above(w2);
             }
-          // wall surface term (advection.h:177-199), as tile::body's face()
-          T          fc = T(0);
-          const T    ex = T(ex_of(al)), ey = T(ey_of(al));
-          if (ywall && k.ny)
-            {
-              const bool    lo_w = j == 0;
-              const double *bn   = lo_w ? G.ay.b0 : G.ay.b1;
-              const T       r0v  = lo_w ? mid.f[al - 1] : lo.f[al - 1];
-              const T       r1v  = lo_w ? hi.f[al - 1] : mid.f[al - 1];
-              T             dn   = q1::fma_t(T(bn[0]), r0v, T(0));
-              dn                 = q1::fma_t(T(bn[1]), r1v, dn);
-              const T rv[3] = {from_below(mid.f[al - 1]), mid.f[al - 1], from_above(mid.f[al - 1])};
-              const T dv[3] = {from_below(dn), dn, from_above(dn)};
-              T       t = T(0), nrm = T(0);
-#pragma unroll
-              for (int o = 0; o < 3; ++o)
-                {
-                  t   = q1::fma_t(T(band_at(G.ax.D, opaque(x), o, P)), rv[o], t);
-                  nrm = q1::fma_t(T(band_at(G.ax.M, opaque(x), o, P)), dv[o], nrm);
-                }
-              fc = q1::fma_t(hdt2 * (lo_w ? -ey : ey), q1::fma_t(ex, t, ey * nrm), fc);
-            }
-          if (xwall_wave && k.nx)
-            {
-              // lanes at x = 0 take column 1 from the lane above, lanes at x = P column P - 1 from below
-              const bool lo_w = x == 0;
-              const T    fr[3] = {lo.f[al - 1], mid.f[al - 1], hi.f[al - 1]};
-              T          nb[3];
-#pragma unroll
-              for (int o = 0; o < 3; ++o)
-                {
-                  const T up = from_above(fr[o]), dw = from_below(fr[o]);
-                  nb[o]      = lo_w ? up : dw;
-                }
-              if (xwall_strip && outl)
-                {
-                  const double *bn = lo_w ? G.ax.b0 : G.ax.b1;
-                  T             t = T(0), nrm = T(0);
-#pragma unroll
-                  for (int o = 0; o < 3; ++o)
-                    {
-                      const T c0 = lo_w ? fr[o] : nb[o], c1 = lo_w ? nb[o] : fr[o];
-                      T       dn = q1::fma_t(T(bn[0]), c0, T(0));
-                      dn         = q1::fma_t(T(bn[1]), c1, dn);
-                      t          = q1::fma_t(yb[1][o], fr[o], t);
-                      nrm        = q1::fma_t(yb[0][o], dn, nrm);
-                    }
-                  fc = q1::fma_t(hdt2 * (lo_w ? -ex : ex), q1::fma_t(ex, nrm, ey * t), fc);
-                }
-            }
-          xinc[al - 1] = r + fc;
-          if (al & 1)
-            mp[(al - 1) >> 1] = fc;
-          else
-            mp[(al - 1) >> 1] += fc;
+          xinc[al - 1] = r;
         });
 
-      const T mflux = ((mp[0] + mp[1]) + mp[2]) + mp[3];
-      // the row's stencil values are consumed: the next loads go out here, before this row's stores,
-      // so that waiting for them later does not wait for the stores (the vector memory counter
-      // counts both), and while the oldest row's registers are free
-      between();
+      // The wall surface terms (advection.h:177-199), as tile::body's face(), in a pass of their own
+      // over the wall rows and the strips that hold a wall column (uniform branch): xinc = r + fc
+      // for every population (fc = 0 where the term vanishes), the same sums as one pass; the wall
+      // mass flux ((m_0 + m_1) + m_2) + m_3, m_k = fc(2k + 1) + fc(2k + 2). Out of the operator loop,
+      // the face code adds no registers to the operator's.
+      T mflux = T(0);
+      if (ywall || xwall_wave)
+        {
+          T mp = T(0);
+          for_pops<1, Q>([&](auto I) {
+            constexpr int     al = decltype(I)::value;
+            const q1::Coef<T> k  = q1::coef<T>(T(ex_of(al)), T(ey_of(al)), sa, c_c);
+            T                 fc = T(0);
+            const T    ex = T(ex_of(al)), ey = T(ey_of(al));
+            if (ywall && k.ny)
+              {
+                const bool    lo_w = j == 0;
+                const double *bn   = lo_w ? G.ay.b0 : G.ay.b1;
+                const T       r0v  = lo_w ? mid.f[al - 1] : lo.f[al - 1];
+                const T       r1v  = lo_w ? hi.f[al - 1] : mid.f[al - 1];
+                T             dn   = q1::fma_t(T(bn[0]), r0v, T(0));
+                dn                 = q1::fma_t(T(bn[1]), r1v, dn);
+                const T rv[3] = {from_below(mid.f[al - 1]), mid.f[al - 1], from_above(mid.f[al - 1])};
+                const T dv[3] = {from_below(dn), dn, from_above(dn)};
+                T       t = T(0), nrm = T(0);
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+                  {
+                    t   = q1::fma_t(T(band_at(G.ax.D, opaque(x), o, P)), rv[o], t);
+                    nrm = q1::fma_t(T(band_at(G.ax.M, opaque(x), o, P)), dv[o], nrm);
+                  }
+                fc = q1::fma_t(hdt2 * (lo_w ? -ey : ey), q1::fma_t(ex, t, ey * nrm), fc);
+              }
+            if (xwall_wave && k.nx)
+              {
+                // lanes at x = 0 take column 1 from the lane above, lanes at x = P column P - 1 from below
+                const bool lo_w = x == 0;
+                const T    fr[3] = {lo.f[al - 1], mid.f[al - 1], hi.f[al - 1]};
+                T          nb[3];
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+                  {
+                    const T up = from_above(fr[o]), dw = from_below(fr[o]);
+                    nb[o]      = lo_w ? up : dw;
+                  }
+                if (xwall_strip && outl)
+                  {
+                    const double *bn = lo_w ? G.ax.b0 : G.ax.b1;
+                    T             t = T(0), nrm = T(0);
+#pragma unroll
+                    for (int o = 0; o < 3; ++o)
+                      {
+                        const T c0 = lo_w ? fr[o] : nb[o], c1 = lo_w ? nb[o] : fr[o];
+                        T       dn = q1::fma_t(T(bn[0]), c0, T(0));
+                        dn         = q1::fma_t(T(bn[1]), c1, dn);
+                        t          = q1::fma_t(yb[1][o], fr[o], t);
+                        nrm        = q1::fma_t(yb[0][o], dn, nrm);
+                      }
+                    fc = q1::fma_t(hdt2 * (lo_w ? -ex : ex), q1::fma_t(ex, nrm, ey * t), fc);
+                  }
+              }
+            xinc[al - 1] = xinc[al - 1] + fc;
+            if constexpr (al & 1)
+              mp = fc;
+            else if constexpr (al == 2)
+              mflux = mp + fc;
+            else
+              mflux += mp + fc;
+          });
+        }
+      else
+#pragma unroll
+        for (int a = 0; a < NM; ++a)
+          xinc[a] = xinc[a] + T(0); // r + fc with fc = 0, as above
 
       // ---------------------------------------------------------------- epilogue (tile::body's)
       const int      gi = x, gj = j;
