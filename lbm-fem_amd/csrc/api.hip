@@ -1140,6 +1140,15 @@ namespace
     std::vector<BatchEntry<T>> E(g.members.size());
     int                        begin = 0;
     int                        tx, ty;
+    // a member's population block of 4 GiB or more puts the group on the LDS tile kernel
+    if (g.p == 1 && g.variant >= kTileLdsVariants)
+      for (const lbfem_scheme *s : g.members)
+        if (!march_fits(s->d->G, sizeof(T)))
+          {
+            g.variant = kTileWide;
+            b.drop_graphs();
+            break;
+          }
     tile_dims(g.variant, tx, ty);
     // Q1 groups at a march variant list each member's wavefronts: 62-column strips by row segments
     const bool march = g.p == 1 && g.variant >= kTileLdsVariants;

@@ -51,6 +51,14 @@ namespace lbfem_amd
 
   constexpr int kMarchOut = 62; // output columns per wavefront strip of the row-march kernel
 
+  // the row-march kernel addresses a population block (Q vectors of the mesh's stride) with
+  // 32-bit byte offsets: it runs on blocks under 4 GiB (the LDS tile kernel takes the rest)
+  inline bool
+  march_fits(const Geo &G, const size_t elem_bytes)
+  {
+    return uint64_t(Q) * uint64_t(G.stride) * uint64_t(elem_bytes) < (uint64_t(1) << 32);
+  }
+
   // node rows per wavefront segment of a row-march variant
   inline int
   march_rows(const int variant)

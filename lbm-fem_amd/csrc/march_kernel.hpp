@@ -16,9 +16,11 @@
 // and (Lee-Lin) equilibria wait in a small per-lane LDS slot until the row's
 // epilogue, and the lane's x-band entries sit in LDS (read per row).
 //
-// Memory: the next row's loads are issued between a row's operator and its
-// epilogue (in flight during the epilogue and stores, in the registers of the row
-// just dropped). Nothing else in the row loop is a vector load on every row:
+// Memory: the next row's loads are issued before a row's operator (in flight
+// during the operator, the epilogue and the stores, in the registers the row's
+// successor's prologue consumed). The wall surface terms run in a pass of their
+// own on wall rows and wall strips, so that the operator's register peak does not
+// include them. Nothing else in the row loop is a vector load on every row:
 // per-lane constants are read once with unconditional loads, row-uniform lookups
 // (y bands, 1 / lumped mass, side-wall tables) are scalar loads, and the per-lane
 // bottom/top wall lookups run on those two rows only, so that no wait for a
@@ -123,6 +125,37 @@ namespace lbfem_amd
       asm volatile("" : "+v"(v));
       return v;
     }
+
+    // Buffer access to a block of population vectors: the lane's node byte offset (voffset) plus
+    // the population's block offset a * stride (soffset, uniform), bounded by the block's size
+    // (an access past it reads zero instead of faulting). The launchers run this kernel for
+    // blocks under 4 GiB.
+    using Rsrc = __amdgpu_buffer_rsrc_t;
+    __device__ __forceinline__ Rsrc
+    block_rsrc(const void *base, const uint32_t bytes)
+    {
+      return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, int(bytes), 0x00020000);
+    }
+    template <typename T>
+    __device__ __forceinline__ T
+    bload(const Rsrc r, const uint32_t voff, const uint32_t soff)
+    {
+      if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+      else
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    }
+    __device__ __forceinline__ void
+    bstore(const double v, const Rsrc r, const uint32_t voff, const uint32_t soff)
+    {
+      using V2 = unsigned __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, soff, 0);
+    }
+    __device__ __forceinline__ void
+    bstore(const float v, const Rsrc r, const uint32_t voff, const uint32_t soff)
+    {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    }
   } // namespace march
 
   namespace march
@@ -149,11 +182,6 @@ namespace lbfem_amd
     constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
     constexpr int  NX   = LL ? Q + 1 : 1; // per-lane extras of a row: f_0 and (Lee-Lin) the equilibria
     using R             = Row<T, LL>;
-    using GT            = __attribute__((address_space(1))) T;
-    const GT *const gfin = (const GT *)fin;
-    const GT *const gfeq = (const GT *)feq_in;
-    GT *const       gout = (GT *)out;
-    GT *const       grhs = (GT *)rhs_out;
     auto           &ext  = lds.ext;
     auto           &xbl  = lds.xbl;
 
@@ -187,14 +215,11 @@ namespace lbfem_amd
     }
     // 1 / lumped mass of the lane's column (output lanes: 0 <= x <= P)
     const double mlx = ((const __attribute__((address_space(1))) double *)G.ax.ml_inv)[min(max(x, 0), P)];
-    // node addresses: the population block's base (uniform) plus the node's 32-bit byte offset
-    // (the launcher runs this kernel on meshes of fewer than 2^29 nodes)
-    auto pop_ptr = [&](auto *base, const int a, const uint32_t off) {
-      using B = std::remove_pointer_t<decltype(base)>;
-      using C = std::conditional_t<std::is_const_v<B>, const __attribute__((address_space(1))) char,
-                                   __attribute__((address_space(1))) char>;
-      return (B *)((C *)(base + a * G.stride) + off);
-    };
+    // the population blocks (Q vectors of fin, feq and out, NM of rhs) as buffer resources; a
+    // population's soffset is a * sb
+    const uint32_t sb   = uint32_t(G.stride) * uint32_t(sizeof(T));
+    const Rsrc     r_in = block_rsrc(fin, Q * sb), r_eq = block_rsrc(feq_in, Q * sb);
+    const Rsrc     r_out = block_rsrc(out, Q * sb), r_rhs = block_rsrc(rhs_out, NM * sb);
     // the wall columns x = 0 and x = P (non-periodic x) and the strips that hold them
     const bool xwall_strip = !G.ax.periodic && (x == 0 || x == P);
     const bool xwall_wave  = !G.ax.periodic && (strip == 0 || strip * kOut + kOut > P);
@@ -207,7 +232,7 @@ namespace lbfem_amd
       const uint32_t off = (xw >= 0 && yw >= 0) ? uint32_t(yw * G.np + xw) * uint32_t(sizeof(T)) : 0u;
 #pragma unroll
       for (int a = 0; a < Q; ++a)
-        v[a] = *pop_ptr(gfin, a, off);
+        v[a] = bload<T>(r_in, off, a * sb);
     };
     // wall_of(G, i, jj) (collision.h:145-165) with the row index jj uniform over the wavefront: the
     // side walls' table entries and velocities are scalar loads; the bottom and top walls' per-lane
@@ -279,7 +304,7 @@ namespace lbfem_amd
           else
 #pragma unroll
             for (int a = 0; a < Q; ++a)
-              fe[a] = *pop_ptr(gfeq, a, uint32_t(yw * G.np + xw) * uint32_t(sizeof(T)));
+              fe[a] = bload<T>(r_eq, uint32_t(yw * G.np + xw) * uint32_t(sizeof(T)), a * sb);
 #pragma unroll
           for (int a = 1; a < Q; ++a)
             row.h[a - 1] = q1::fma_t(a_c, v[a], b_c * fe[a]);
@@ -467,17 +492,17 @@ namespace lbfem_amd
                   own[a] += xinc[a - 1];
 #pragma unroll
               for (int a = 0; a < Q; ++a)
-                *pop_ptr(gout, a, off) = own[a];
+                bstore(own[a], r_out, off, a * sb);
             }
           else
             {
 #pragma unroll
               for (int a = 0; a < NM; ++a)
-                *pop_ptr(grhs, a, off) = xinc[a];
+                bstore(xinc[a], r_rhs, off, a * sb);
               if (out != nullptr)
 #pragma unroll
                 for (int a = 0; a < Q; ++a)
-                  *pop_ptr(gout, a, off) = own[a];
+                  bstore(own[a], r_out, off, a * sb);
             }
         }
       // Periodic slaves (the lines x = P, y = P of a periodic axis), after the row's stores and in
@@ -493,7 +518,7 @@ namespace lbfem_amd
             T          own[Q];
 #pragma unroll
             for (int a = 0; a < Q; ++a)
-              own[a] = *pop_ptr(gfin, a, off);
+              own[a] = bload<T>(r_in, off, a * sb);
             if constexpr (!LL)
               if (S.collide)
                 collide_node(own, S.omega, wall, wx, wy);
@@ -520,17 +545,17 @@ namespace lbfem_amd
                     own[a] += xinc[a - 1];
 #pragma unroll
                 for (int a = 0; a < Q; ++a)
-                  *pop_ptr(gout, a, off) = own[a];
+                  bstore(own[a], r_out, off, a * sb);
               }
             else
               {
 #pragma unroll
                 for (int a = 0; a < NM; ++a)
-                  *pop_ptr(grhs, a, off) = T(0);
+                  bstore(T(0), r_rhs, off, a * sb);
                 if (out != nullptr)
 #pragma unroll
                   for (int a = 0; a < Q; ++a)
-                    *pop_ptr(gout, a, off) = own[a];
+                    bstore(own[a], r_out, off, a * sb);
               }
           }
     };

@@ -666,7 +666,7 @@ namespace lbfem_amd
         case kMarch16:
           if constexpr (p == 1)
             {
-              if (standard_directions(S))
+              if (standard_directions(S) && march_fits(G, sizeof(T)))
                 return launch_march<T, MODE>(march_rows(variant), G, S, fin, feq, out, rhs, stream);
               return launch_tile<tile::Cfg<T, p, 32, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
             }
