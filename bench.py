@@ -177,7 +177,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-general", action="store_true")
-    ap.add_argument("--precision-steps", type=int, default=100)
+    ap.add_argument("--precision-steps", type=int, default=5000)
     ap.add_argument("--profile-headline", action="store_true")
     ap.add_argument("--profile-cells", type=int, default=256)
     ap.add_argument("--profile-general", action="store_true")
@@ -201,8 +201,10 @@ def main():
         disc, s = make_scheme(A, N, 1, Re=1000.0 * N / 256, tune=False)  # dt / lambda 2.88 on every mesh size
         log(f"profile-headline: tile variant {s.tile_variant} ({TILE_KERNEL[s.tile_variant]})")
         timed(torch, s, K, W)
+        tk = kernel_time(torch, s, K)
         gbs, ok = hbm_probe(torch, A, nbytes=256 << 20, reps=3)
-        log(f"profile-headline: {K + W} steps of cavity Q1 {N}^2, copy probe 3 x 256 MiB at {gbs:.0f} GB/s (ok={ok})")
+        log(f"profile-headline: {2 * K + W} steps of cavity Q1 {N}^2 (HIP-event kernel time {1e6 * tk:.2f} us over the "
+            f"last {K}), copy probe 3 x 256 MiB at {gbs:.0f} GB/s (ok={ok})")
         return
 
     if args.profile_general:
@@ -339,20 +341,20 @@ def main():
                                 "workload": "cavity Q1 2048^2 cells (4 198 401 nodes), Lee-Lin, lumped, Re 8000, FP64"}
             del s3, d3
 
-    # ---- precision: TGV (1,1) on Q1 1024^2, Bardow, consistent mass (CG 1e-10), FP64 and mixed (FP32 inner CG
-    # refined with FP64 residuals); L2 velocity error of the mixed run vs FP64 and both vs the analytic decay
+    # ---- precision: TGV (1,1) on Q1 1024^2 and Q2 512^2, Bardow, consistent mass (CG 1e-10), FP64 and mixed (FP32
+    # populations and CG vectors, FP64 scalars and sums); L2 velocity error of the mixed run vs FP64 and both vs the
+    # analytic decay
     precision = None
     if not args.no_precision:
         steps_p = args.precision_steps
-        precision = {"workload": "Taylor-Green (1,1), Bardow, consistent mass CG 1e-10, CFL 0.25, Re 100, "
+        precision = {"workload": "Taylor-Green (1,1), U0 0.01, Bardow, consistent mass CG 1e-10, CFL 0.25, Re 100, "
                                  f"{steps_p} steps after 5; Q1 1024^2 and Q2 512^2 cells"}
         for label, N, p in (("q1_1024", 1024, 1), ("q2_512", 512, 2)):
             runs = {}
-            # mixed_tol1e-6: FP32 storage puts round-off of about 1e-5 |b| into the right-hand side at this size,
-            # which a 1e-10 solve resolves at a cost of many iterations and no accuracy
-            for prec, name, tol in ((A.FP64, "fp64", 1e-10), (A.MIXED, "mixed", 1e-10),
-                                    (A.MIXED, "mixed_tol1e-6", 1e-6)):
-                tc = A.test_cases.make("tgv", mach=0.1, modes=(1, 1))
+            # the mixed form stops its FP32 solves at the round-off level of its right-hand side (a floor that
+            # scales with the nodes per mesh line: 9.7e-6 |b| on these meshes) whatever the tolerance asked
+            for prec, name, tol in ((A.FP64, "fp64", 1e-10), (A.MIXED, "mixed", 1e-10)):
+                tc = A.test_cases.make("tgv", mach=0.01 * 3 ** 0.5, modes=(1, 1))  # U0 = 0.01 (lattice units)
                 tc.set_reynolds(100.0)
                 dp = A.Discretization(tc, N, p)
                 sp = A.Bardow(dp, mass=A.MASS_CG, cg_tolerance=tol, precision=prec)
@@ -379,8 +381,6 @@ def main():
                                                     / (w[indep] * (ur ** 2 + vr ** 2)[indep]).sum()))
             row = {"l2_velocity_mixed_vs_fp64": l2(runs["mixed"]["u"], runs["mixed"]["v"], runs["fp64"]["u"],
                                                    runs["fp64"]["v"]),
-                   "l2_velocity_mixed_tol1e-6_vs_fp64": l2(runs["mixed_tol1e-6"]["u"], runs["mixed_tol1e-6"]["v"],
-                                                           runs["fp64"]["u"], runs["fp64"]["v"]),
                    "l2_velocity_vs_analytic": {k: l2(r["u"], r["v"], ue, ve) for k, r in runs.items()}}
             for k, r in runs.items():
                 row[k] = {x: r[x] for x in ("ms_per_step", "MLUPS", "cg_its_per_solve")}

@@ -168,9 +168,10 @@ typedef struct
                                     deviations f - w rho0, FP64 moments and equilibria;
                                     with the consistent mass, CG solves on FP32 increments
                                     (FP64 scalars and sums) from the extrapolated FP32
-                                    increment, stopping at max(cg_tolerance, 1e-6) |b|
-                                    (the FP32 right-hand side's round-off level); lumped
-                                    or CG, not Richardson */
+                                    increment, stopping at max(cg_tolerance, floor) |b|,
+                                    floor = max(1e-6, 2^-24 n / (2 pi)) for n nodes per
+                                    mesh line (the FP32 right-hand side's round-off
+                                    level); lumped or CG, not Richardson */
 } lbfem_scheme_settings;
 
 int lbfem_scheme_create(const lbfem_disc *disc, const lbfem_scheme_settings *settings, lbfem_scheme **scheme);

@@ -771,14 +771,22 @@ namespace
     return LBFEM_OK;
   }
 
-  // The mixed form's consistent-mass solves stop at max(cg_tolerance, kMixedCgTolerance) |b|:
-  // the right-hand side comes from FP32 deviations and carries their round-off, which a tighter
-  // solve resolves at the cost of iterations and no accuracy, and which FP32 vectors do not get
-  // below reliably. Measured against FP64 solves to 1e-10 (relative L2 velocity): 1e-6 keeps
-  // the mixed form within 1e-6 on the Taylor-Green and cavity meshes of 20^2 to 48^2 cells
-  // (600 steps) and within 1e-5 on 1024^2 / 512^2 over 5000 steps; 1e-5 and 1e-4 lose up to
-  // 9e-6 and 7e-5 on the small meshes.
-  constexpr double kMixedCgTolerance = 1e-6;
+  // The mixed form's consistent-mass solves stop at max(cg_tolerance, mixed_cg_floor) |b|, the
+  // floor max(1e-6, 2^-24 np / (2 pi)) with np nodes per mesh line: the right-hand side comes from
+  // FP32 deviations and carries their round-off, about 2^-24 |g| per value while |b| is about
+  // |g| 2 pi / np times smaller (a difference over one mesh spacing of values that vary over the
+  // domain), so a solve below that level resolves the round-off at a cost of iterations and no
+  // accuracy. Measured against FP64 solves to 1e-10 (relative L2 velocity, Taylor-Green U = 0.01,
+  // Re 100, CFL 0.25, 5000 steps): Q1 1024^2 at a floor of 1e-6 6.0 iterations per solve, 0.713 ms
+  // per step, 1.1e-6; at 1e-5 4.1 iterations, 0.538 ms, 6.0e-7 (FP64: 2.65, 0.667 ms); Q2 512^2
+  // 4.5 / 0.630 ms / 4.9e-6 and 2.9 / 0.483 ms / 3.5e-6 (FP64: 1.68, 0.563 ms). On meshes of 20^2
+  // to 48^2 cells (600 steps) fixed floors of 1e-5 and 1e-4 lost up to 9e-6 and 7e-5 where 1e-6
+  // keeps 1e-6: the floor scales with the round-off, 1e-6 up to 105 nodes per line, 9.7e-6 at 1025.
+  inline double
+  mixed_cg_floor(const Geo &G)
+  {
+    return std::max(1e-6, 0x1p-24 * double(G.np) / (2. * M_PI));
+  }
 
   // the mixed form's increments of one sweep: FP32 advection (collision fused) into an FP32
   // right-hand side, then the consistent-mass solves on FP32 vectors (FP64 scalars and sums)
@@ -807,7 +815,7 @@ namespace
     if (s.extrapolate)
       HIP_CHECK(extrapolate_launch(G, xs, s.x32[1].p, mask, s.stream));
     CgWork W  = cg_work(d, s.state32.p, s.state32.p + NM, s.partial.p, s.set.mass);
-    W.tol_rel = std::max(s.set.mass.cg_tolerance, kMixedCgTolerance);
+    W.tol_rel = std::max(s.set.mass.cg_tolerance, mixed_cg_floor(G));
     W.nv      = 0;
     for (int a = 0; a < NM; ++a)
       if (mask & (1u << a))
