@@ -46,7 +46,7 @@ def tile_kernel(variant, t="double"):
 
 TILE_KERNEL = {v: tile_kernel(v) for v in range(N_VARIANTS)}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_tile_cavity_q1_256.txt")
-PMC_SUMMARY_HBM = os.path.join(ROOT, "profiles", "pmc_k_tile_cavity_q1_2048.txt")
+PMC_SUMMARY_HBM = os.path.join(ROOT, "profiles", "pmc_k_march_cavity_q1_2048.txt")
 
 
 def log(*a):
@@ -335,7 +335,7 @@ def main():
                 roofline_hbm = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(gbs / HBM_PEAK_GBS, 4),
                                 "traffic": pmc_traffic(kern, PMC_SUMMARY_HBM),
-                                "traffic_source": "lookup: profiles/pmc_k_tile_cavity_q1_2048.txt (an earlier "
+                                "traffic_source": "lookup: profiles/pmc_k_march_cavity_q1_2048.txt (an earlier "
                                                   "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE run of this leg)",
                                 "kernel": kern, "kernel_us": round(1e6 * tk, 2), "algorithmic_bytes_per_launch": alg3,
                                 "workload": "cavity Q1 2048^2 cells (4 198 401 nodes), Lee-Lin, lumped, Re 8000, FP64"}

@@ -1556,8 +1556,8 @@ namespace
     if (s->set.mass.type != LBFEM_MASS_CG)
       return fail(LBFEM_EINVAL, "a batch steps the lumped or the consistent (CG) mass, not Richardson");
     if (s->mixed)
-      return fail(LBFEM_EINVAL, "a batch steps consistent-mass schemes in FP64 (the mixed form's refined solves "
-                                "are stepped alone)");
+      return fail(LBFEM_EINVAL, "a batch steps consistent-mass schemes in FP64 (a mixed consistent-mass scheme is "
+                                "stepped alone)");
     return LBFEM_OK;
   }
 
