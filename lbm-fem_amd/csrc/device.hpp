@@ -216,42 +216,29 @@ namespace lbfem_amd
     uy               = my * inv;
   }
 
-  // e_a . u for the D2Q9 directions
-  template <int a>
-  __device__ __forceinline__ double
-  e_dot(const double ux, const double uy)
-  {
-    constexpr int sx = ex_of(a), sy = ey_of(a);
-    if constexpr (sx == 0 && sy == 0)
-      return 0.;
-    else if constexpr (sy == 0)
-      return sx > 0 ? ux : -ux;
-    else if constexpr (sx == 0)
-      return sy > 0 ? uy : -uy;
-    else
-      return (sx > 0 ? ux : -ux) + (sy > 0 ? uy : -uy);
-  }
-
-  // Second-order equilibrium of population a (include/lbfem/d2q9.h:44-55).
-  template <int a>
-  __device__ __forceinline__ double
-  feq(const double rho, const double ux, const double uy, const double uu)
-  {
-    const double eu = e_dot<a>(ux, uy);
-    return w_of(a) * rho * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * uu);
-  }
-
-  template <int a = 0>
+  // The second-order equilibria (include/lbfem/d2q9.h:44-55), w_a rho (1 + 3 e_a.u + 4.5 (e_a.u)^2 -
+  // 1.5 u^2), formed per pair of opposite directions a, a': with c = e_a.u = -e_a'.u and
+  // A = w rho (1 - 1.5 u^2 + 4.5 c^2), the pair is A + 3 w rho c and A - 3 w rho c (the same
+  // polynomial in about half the operations of one evaluation per population). The pairs of
+  // this numbering: (1, 5) c = ux, (3, 7) c = uy, (2, 6) c = ux + uy, (4, 8) c = uy - ux.
   __device__ __forceinline__ void
   equilibrium_all(const double rho, const double ux, const double uy, double *out)
   {
     const double uu = ux * ux + uy * uy;
-    if constexpr (a < Q)
-      {
-        out[a] = feq<a>(rho, ux, uy, uu);
-        if constexpr (a + 1 < Q)
-          equilibrium_all<a + 1>(rho, ux, uy, out);
-      }
+    const double c0 = 1. - 1.5 * uu;
+    const double r1 = w_of(1) * rho, r2 = w_of(2) * rho;
+    const double t1 = 3. * r1, t2 = 3. * r2;
+    const auto   pair = [&](const double c, const double r, const double t, const int a, const int b) {
+      const double A = r * (c0 + 4.5 * c * c);
+      const double B = t * c;
+      out[a]         = A + B;
+      out[b]         = A - B;
+    };
+    out[0] = w_of(0) * rho * c0;
+    pair(ux, r1, t1, 1, 5);
+    pair(uy, r1, t1, 3, 7);
+    pair(ux + uy, r2, t2, 2, 6);
+    pair(uy - ux, r2, t2, 4, 8);
   }
 
   // BGK collision of Bardow's transformed populations at one node
@@ -332,26 +319,38 @@ namespace lbfem_amd
   // evaluated in FP64 and rounded to FP32 once: evaluated in FP32 its rounding
   // errors are correlated across the nine populations and from step to step on a
   // slowly varying flow, and the step drifts coherently (1e-5 relative velocity
-  // error after 1000 Taylor-Green steps, against 1e-7 with this form).
-  template <int a>
-  __device__ __forceinline__ float
-  feq_dev(const double drho, const double rho, const double ux, const double uy, const double uu)
+  // error after 1000 Taylor-Green steps, against 1e-7 with this form). The nine
+  // are paired as in equilibrium_all: with c = e_a.u, A = w drho + w rho (4.5 c^2 -
+  // 1.5 u^2) and the pair A +- 3 w rho c.
+  __device__ __forceinline__ void
+  equilibrium_dev_fp64(const double drho, const double rho, const double ux, const double uy, double *out)
   {
-    const double eu = e_dot<a>(ux, uy);
-    return float(w_of(a) * (drho + rho * (3. * eu + 4.5 * eu * eu - 1.5 * uu)));
+    const double uu = ux * ux + uy * uy;
+    const double c0 = -1.5 * uu;
+    const double r1 = w_of(1) * rho, r2 = w_of(2) * rho;
+    const double d1 = w_of(1) * drho, d2 = w_of(2) * drho;
+    const double t1 = 3. * r1, t2 = 3. * r2;
+    const auto   pair = [&](const double c, const double r, const double d, const double t, const int a, const int b) {
+      const double A = d + r * (c0 + 4.5 * c * c);
+      const double B = t * c;
+      out[a]         = A + B;
+      out[b]         = A - B;
+    };
+    out[0] = w_of(0) * (drho + rho * c0);
+    pair(ux, r1, d1, t1, 1, 5);
+    pair(uy, r1, d1, t1, 3, 7);
+    pair(ux + uy, r2, d2, t2, 2, 6);
+    pair(uy - ux, r2, d2, t2, 4, 8);
   }
 
-  template <int a = 0>
   __device__ __forceinline__ void
   equilibrium_dev_all(const double drho, const double rho, const double ux, const double uy, float *out)
   {
-    const double uu = ux * ux + uy * uy;
-    if constexpr (a < Q)
-      {
-        out[a] = feq_dev<a>(drho, rho, ux, uy, uu);
-        if constexpr (a + 1 < Q)
-          equilibrium_dev_all<a + 1>(drho, rho, ux, uy, out);
-      }
+    double e[Q];
+    equilibrium_dev_fp64(drho, rho, ux, uy, e);
+#pragma unroll
+    for (int a = 0; a < Q; ++a)
+      out[a] = float(e[a]);
   }
 
   // BGK collision of deviations (collision.h:152-168): equilibria as above, the
@@ -387,18 +386,15 @@ namespace lbfem_amd
   // Lee & Lin predictor-corrector on deviations (collision.h:119-145): affine in
   // f, feq and x with the rest state fixed, so the same formulas apply
   // f_a = fh_a + theta (feq_a(fh) - fe_a), the corrector in FP64, rounded once
-  template <int a = 0>
   __device__ __forceinline__ void
   pc_correct(float *f, const double *fh, const float *fe, const double theta, const double drho, const double rho,
-             const double ux, const double uy, const double uu)
+             const double ux, const double uy)
   {
-    if constexpr (a < Q)
-      {
-        const double eu = e_dot<a>(ux, uy);
-        const double eh = w_of(a) * (drho + rho * (3. * eu + 4.5 * eu * eu - 1.5 * uu));
-        f[a]            = float(fh[a] + theta * (eh - double(fe[a])));
-        pc_correct<a + 1>(f, fh, fe, theta, drho, rho, ux, uy, uu);
-      }
+    double eh[Q];
+    equilibrium_dev_fp64(drho, rho, ux, uy, eh);
+#pragma unroll
+    for (int a = 0; a < Q; ++a)
+      f[a] = float(fh[a] + theta * (eh[a] - double(fe[a])));
   }
 
   // Lee & Lin predictor-corrector on deviations (collision.h:119-145), in FP64
@@ -426,6 +422,6 @@ namespace lbfem_amd
         ux = wx;
         uy = wy;
       }
-    pc_correct(f, fh, fe, theta, drho, rho, ux, uy, ux * ux + uy * uy);
+    pc_correct(f, fh, fe, theta, drho, rho, ux, uy);
   }
 } // namespace lbfem_amd
