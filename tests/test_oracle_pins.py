@@ -72,6 +72,94 @@ def test_element_matrices_match_published_fractions(p):
         assert np.allclose(got, want, rtol=0, atol=1e-14 * max(1.0, np.abs(want).max())), name
 
 
+class _Q5:
+    """Exact numbers a + b sqrt(5), a and b rational (the field of the Q3 Gauss-Lobatto nodes)."""
+
+    def __init__(self, a, b=0):
+        self.a, self.b = Fraction(a), Fraction(b)
+
+    def __add__(self, o):
+        o = o if isinstance(o, _Q5) else _Q5(o)
+        return _Q5(self.a + o.a, self.b + o.b)
+
+    def __sub__(self, o):
+        o = o if isinstance(o, _Q5) else _Q5(o)
+        return _Q5(self.a - o.a, self.b - o.b)
+
+    def __mul__(self, o):
+        o = o if isinstance(o, _Q5) else _Q5(o)
+        return _Q5(self.a * o.a + 5 * self.b * o.b, self.a * o.b + self.b * o.a)
+
+    def inv(self):
+        n = self.a * self.a - 5 * self.b * self.b
+        return _Q5(self.a / n, -self.b / n)
+
+    def __float__(self):
+        return float(self.a) + float(self.b) * 5 ** 0.5
+
+
+def _poly_mul(p, q):
+    out = [_Q5(0)] * (len(p) + len(q) - 1)
+    for i, a in enumerate(p):
+        for j, b in enumerate(q):
+            out[i + j] = out[i + j] + a * b
+    return out
+
+
+def _exact_q3_1d():
+    """The Q3 Lagrange basis on the Gauss-Lobatto nodes 0, (1 - 1/sqrt5)/2, (1 + 1/sqrt5)/2, 1 of
+    [0, 1], built and integrated exactly: M[i][j] = int phi_i phi_j, C[i][j] = int phi_i phi_j',
+    K[i][j] = int phi_i' phi_j' (polynomial coefficients in Q(sqrt5), int x^k = 1 / (k + 1))."""
+    r = _Q5(0, Fraction(1, 10))  # 1 / (2 sqrt5) = sqrt5 / 10
+    nodes = [_Q5(0), _Q5(Fraction(1, 2)) - r, _Q5(Fraction(1, 2)) + r, _Q5(1)]
+    basis = []
+    for i, xi in enumerate(nodes):
+        poly = [_Q5(1)]
+        for j, xj in enumerate(nodes):
+            if j != i:
+                d = (xi - xj).inv()
+                poly = _poly_mul(poly, [xj * d * -1, d])  # (x - x_j) / (x_i - x_j)
+        basis.append(poly)
+    deriv = [[c * k for k, c in enumerate(poly)][1:] for poly in basis]
+
+    def integral(poly):
+        total = _Q5(0)
+        for k, c in enumerate(poly):
+            total = total + c * Fraction(1, k + 1)
+        return total
+
+    n = len(nodes)
+    M = [[float(integral(_poly_mul(basis[i], basis[j]))) for j in range(n)] for i in range(n)]
+    C = [[float(integral(_poly_mul(basis[i], deriv[j]))) for j in range(n)] for i in range(n)]
+    K = [[float(integral(_poly_mul(deriv[i], deriv[j]))) for j in range(n)] for i in range(n)]
+    return [float(x) for x in nodes], np.array(M), np.array(C), np.array(K)
+
+
+def test_q3_element_matrices_match_exact_integrals():
+    """Q3 (include/lbfem/discretization.h: FE_Q(3) on Gauss-Lobatto support points, cell integrals of
+    mass.h / advection.h): the oracle's element matrices, formed by quadrature in floating point, against
+    the exact integrals of the Lagrange basis on the nodes 0, (1 -+ 1/sqrt5)/2, 1, in closed form in
+    Q(sqrt5). A wrong node set, an under-integrating rule or a transposed derivative fails; the 2-D
+    element matrices are the tensor products with the cell size h and the direction e."""
+    x, M1, C1, K1 = _exact_q3_1d()
+    assert np.allclose(O.gll_nodes(3), x, rtol=0, atol=1e-15)
+    # the closed forms known for this basis: the row sums of M are the GLL weights (1/12, 5/12, 5/12, 1/12),
+    # C's rows sum to 0 (derivative of a constant) and its end rows are -1/2 and 1/2 after symmetrisation
+    assert np.allclose(M1.sum(1), [1 / 12, 5 / 12, 5 / 12, 1 / 12], atol=1e-15)
+    assert np.allclose(C1.sum(1), 0, atol=1e-15)
+    assert np.allclose(C1 + C1.T, np.diag([-1, 0, 0, 1]), atol=1e-15)
+    h = 0.37
+    for e in ((1.0, 0.0), (1.0, 1.0), (0.3, -0.8)):
+        M, Cm, K = O.element_matrices(3, h, e)
+        # node k = iy * 4 + ix, as the oracle numbers the cell's nodes
+        want_M = h * h * np.kron(M1, M1)
+        want_C = h * (e[0] * np.kron(M1, C1) + e[1] * np.kron(C1, M1))
+        want_K = (e[0] ** 2 * np.kron(M1, K1) + e[1] ** 2 * np.kron(K1, M1)
+                  + e[0] * e[1] * (np.kron(C1.T, C1) + np.kron(C1, C1.T)))
+        for name, got, want in (("M", M, want_M), ("C", Cm, want_C), ("K", K, want_K)):
+            assert np.allclose(got, want, rtol=0, atol=1e-14 * max(1.0, np.abs(want).max())), (name, e)
+
+
 def _assembled_row(disc, apply, i0, j0):
     """Row of the assembled operator at node (i0, j0): A[c, j] = (A e_j)[c]."""
     npd = disc.np1
