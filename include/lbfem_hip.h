@@ -213,10 +213,11 @@ int lbfem_scheme_stage_times(const lbfem_scheme *scheme, double *seconds4);
  * 256 threads, 1: 32 x 16 tiles of 512, 2 and 3: 32 x 8 tiles with a node's
  * populations split over 2 or 4 threads, 4: 16 x 8 tiles split over 4, 5 and
  * 6: 40 x 8 and 48 x 8 tiles of 320 and 384 threads, 7: 40 x 8 tiles split over
- * 2 threads per node (640 threads); 8-11: the row-march kernel (Q1 meshes;
- * wavefronts own 62-column strips and march up segments of 1, 4, 8 or 16 node
- * rows with the stencil rows in registers; elsewhere variant 0); -1 restores the
- * size-based default). Every variant gives bitwise identical populations. */
+ * 2 threads per node (640 threads); 8-11: the row-march kernel (Q1 meshes, and
+ * Q2 meshes of the mixed form; wavefronts own strips of 64 - 2p columns and
+ * march up segments of 1, 4, 8 or 16 node rows with the stencil rows in
+ * registers; elsewhere variant 0); -1 restores the size-based default). Every
+ * variant gives bitwise identical populations. */
 int lbfem_scheme_set_tile_variant(lbfem_scheme *scheme, int variant);
 int lbfem_scheme_get_tile_variant(const lbfem_scheme *scheme, int *variant);
 /* Times every variant of the scheme's tile kernel on the scheme's own buffers

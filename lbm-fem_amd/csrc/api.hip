@@ -1148,8 +1148,11 @@ namespace
     std::vector<BatchEntry<T>> E(g.members.size());
     int                        begin = 0;
     int                        tx, ty;
-    // a member's population block of 4 GiB or more puts the group on the LDS tile kernel
-    if (g.p == 1 && g.variant >= kTileLdsVariants)
+    // Q1 and Q2 groups at a march variant list each member's wavefronts (strips of march_out(p)
+    // columns by row segments), where the row-march kernel runs for the group's form; a member's
+    // population block of 4 GiB or more puts the group on the LDS tile kernel
+    const bool marchable = march_runs(g.p, sizeof(T) == 4);
+    if (marchable && g.variant >= kTileLdsVariants)
       for (const lbfem_scheme *s : g.members)
         if (!march_fits(s->d->G, sizeof(T)))
           {
@@ -1158,8 +1161,7 @@ namespace
             break;
           }
     tile_dims(g.variant, tx, ty);
-    // Q1 groups at a march variant list each member's wavefronts: 62-column strips by row segments
-    const bool march = g.p == 1 && g.variant >= kTileLdsVariants;
+    const bool march = marchable && g.variant >= kTileLdsVariants;
     const int  rows  = march ? march_rows(g.variant) : 0;
     g.n_nodes = 0;
     for (size_t k = 0; k < g.members.size(); ++k)
@@ -1170,7 +1172,7 @@ namespace
         E[k].G           = G;
         E[k].S           = tile_params(s, s.sweeps[0], ll);
         E[k].S.collide   = 1;
-        E[k].S.tiles_x   = march ? (G.P + 1 + kMarchOut - 1) / kMarchOut : (G.P + 1 + tx - 1) / tx;
+        E[k].S.tiles_x   = march ? (G.P + 1 + march_out(g.p) - 1) / march_out(g.p) : (G.P + 1 + tx - 1) / tx;
         E[k].S.tiles_y   = march ? (G.P + 1 + rows - 1) / rows : (G.P + 1 + ty - 1) / ty;
         E[k].S.rows      = rows;
         if constexpr (sizeof(T) == 4)

@@ -1,31 +1,31 @@
-// The Q1 streaming operator in column form, shared by the tile kernel and the
+// The streaming operator in column form, shared by the tile kernel and the
 // row-march kernel so that both compute every increment by the same operation
-// sequence (bitwise identical results).
+// sequence (bitwise identical results), for any degree p (W = 2p + 1 band entries).
 //
-// On a cartesian Q1 mesh the advection right-hand side of population a at node
+// On a cartesian Q_p mesh the advection right-hand side of population a at node
 // (i, j) is (advection.h:211-293, the Kronecker form of the cell integrals)
 //   r = ex sa (Dx (x) My) h + ey sa (Mx (x) Dy) h
 //     + c (ex^2 (Kx (x) My) + ey^2 (Mx (x) Ky) + ex ey (Dx (x) Dy^T + Dx^T (x) Dy)) f
 // with h = f (Bardow) or h = a f + b feq (Lee-Lin, sa = 1). It is evaluated in
-// two contractions. First, at each of the three columns x = i - 1 + k (k = 0, 1,
-// 2), the y bands of row j act on the column's three taps (rows j - 1, j, j + 1)
-// and the results are grouped by the x band they meet:
+// two contractions. First, at each of the W columns x = i - p + k (k = 0 .. 2p),
+// the y bands of row j act on the column's W taps (rows j - p .. j + p) and the
+// results are grouped by the x band they meet:
 //   U_D = ex sa My h + c ex ey Dy^T f,   U_M = ey sa Dy h + c ey^2 Ky f,
 //   U_K = c ex^2 My f,                   U_T = c ex ey Dy f.
 // Then w_k = Dx[k] U_D + Mx[k] U_M + Kx[k] U_K + Dx^T[k] U_T with the x bands of
-// column i, and r = (w_0 + w_1) + w_2. The column values depend only on the
-// column and the row, so a kernel that holds one column per lane computes them
-// once and passes w_0 and w_2 to the neighbouring lanes; the tile kernel
-// computes all three columns per node. Terms of a population that vanish by its
-// direction class (x: ey = 0, y: ex = 0, diagonal) are skipped the same way in
-// both. The FP32 form (below) centres the taps to carry differences.
+// column i, and r = ((w_0 + w_1) + ...) + w_2p. The column values depend only on
+// the column and the row, so a kernel that holds one column per lane computes
+// them once and passes the w_k to the neighbouring lanes; the tile kernel computes
+// all W columns per node. Terms of a population that vanish by its direction
+// class (x: ey = 0, y: ex = 0, diagonal) are skipped the same way in both. The
+// FP32 form (below) centres the taps to carry differences.
 #pragma once
 
 #include "device.hpp"
 
 namespace lbfem_amd
 {
-  namespace q1
+  namespace col
   {
     // fused multiply-add in the storage type (explicit, so that no contraction choice of
     // the compiler can differ between the kernels)
@@ -63,12 +63,16 @@ namespace lbfem_amd
       return k;
     }
 
-    // sum_o y[o] v[o], o = 0, 1, 2
-    template <typename T>
+    // sum_o y[o] v[o], o = 0 .. W - 1, in that order
+    template <typename T, int W>
     __device__ __forceinline__ T
-    chain(const T (&y)[3], const T (&v)[3])
+    chain(const T (&y)[W], const T (&v)[W])
     {
-      return fma_t(y[2], v[2], fma_t(y[1], v[1], y[0] * v[0]));
+      T s = y[0] * v[0];
+#pragma unroll
+      for (int o = 1; o < W; ++o)
+        s = fma_t(y[o], v[o], s);
+      return s;
     }
 
     template <typename T>
@@ -80,9 +84,9 @@ namespace lbfem_amd
     // The column values of one population at one column from its taps fv (f) and hv
     // (h; the same as fv for Bardow), yb[band][o] the y bands of the output row
     // (bands: 0 M, 1 D, 2 K, 3 D^T).
-    template <typename T, bool LL>
+    template <typename T, bool LL, int W>
     __device__ __forceinline__ Cols<T>
-    columns(const T (&yb)[4][3], const T (&fv)[3], const T (&hv)[3], const Coef<T> &k)
+    columns(const T (&yb)[4][W], const T (&fv)[W], const T (&hv)[W], const Coef<T> &k)
     {
       Cols<T> u{T(0), T(0), T(0), T(0)};
       if (k.nx)
@@ -128,19 +132,25 @@ namespace lbfem_amd
     // ------------------------------------------------------------------ FP32 (centred) form
     // The mixed form applies the operator to differences, so that FP32 carries the small
     // variations of f instead of cancelling large values. Centred on the output node (i, j),
-    //   sum_o y[o] (v(x, j + o) - v(i, j)) = sum_o y[o] (v(x, j + o) - v(x, j)) + m (v(x, j) - v(i, j)),
-    // m = y[0] + y[1] + y[2]: the first part depends on the column only (shared by the three
-    // nodes a column feeds), the second is a row term. m is taken as 0 for the D and K bands
-    // (their rows sum to zero: every term of the operator then annihilates constants exactly)
-    // and formed from the entries for M and D^T (D^T rows sum to -1 and 1 on the bottom and
-    // top boundary rows, to 0 elsewhere).
+    //   sum_o y[o] (v(x, j - p + o) - v(i, j)) = sum_o y[o] (v(x, j - p + o) - v(x, j)) + m (v(x, j) - v(i, j)),
+    // m = sum_o y[o]: the first part depends on the column only (shared by the W nodes a column
+    // feeds), the second is a row term. m is taken as 0 for the D and K bands (their rows sum to
+    // zero: every term of the operator then annihilates constants exactly) and formed from the
+    // entries for M and D^T (D^T rows sum to -1 and 1 on the bottom and top boundary rows, to 0
+    // elsewhere).
 
-    // sum_o y[o] (v[o] - v[1])
-    template <typename T>
+    // sum_{o != p} y[o] (v[o] - v[p]), o in increasing order
+    template <typename T, int W>
     __device__ __forceinline__ T
-    chain_c(const T (&y)[3], const T (&v)[3])
+    chain_c(const T (&y)[W], const T (&v)[W])
     {
-      return fma_t(y[2], v[2] - v[1], y[0] * (v[0] - v[1]));
+      constexpr int p = W / 2;
+      T             s = y[0] * (v[0] - v[p]);
+#pragma unroll
+      for (int o = 1; o < W; ++o)
+        if (o != p)
+          s = fma_t(y[o], v[o] - v[p], s);
+      return s;
     }
 
     // the column-centred contractions of one population at one column
@@ -150,9 +160,9 @@ namespace lbfem_amd
       T Mh, Dh, Mf, Kf, Tf, Df;
     };
 
-    template <typename T, bool LL>
+    template <typename T, bool LL, int W>
     __device__ __forceinline__ CCols<T>
-    ccolumns(const T (&yb)[4][3], const T (&fv)[3], const T (&hv)[3], const Coef<T> &k)
+    ccolumns(const T (&yb)[4][W], const T (&fv)[W], const T (&hv)[W], const Coef<T> &k)
     {
       CCols<T> c{T(0), T(0), T(0), T(0), T(0), T(0)};
       if (k.nx)
@@ -174,8 +184,8 @@ namespace lbfem_amd
     }
 
     // the column values for the node this column feeds at offset k: the row terms with dh =
-    // h(x, j) - h(i, j) and df = f(x, j) - f(i, j) (none at k = 1, where both vanish); mM and mT
-    // the row sums of the M and D^T bands
+    // h(x, j) - h(i, j) and df = f(x, j) - f(i, j) (none at the node's own column, where both
+    // vanish); mM and mT the row sums of the M and D^T bands
     template <typename T, bool LL>
     __device__ __forceinline__ Cols<T>
     assemble(const CCols<T> &c, const T mM, const T mT, const T dh, const T df, const bool row, const Coef<T> &k)
@@ -201,12 +211,16 @@ namespace lbfem_amd
       return u;
     }
 
-    // the row sum of a y band, as both kernels form it
-    template <typename T>
+    // the row sum of a y band, in order, as every kernel forms it
+    template <typename T, int W>
     __device__ __forceinline__ T
-    row_sum(const T (&y)[3])
+    row_sum(const T (&y)[W])
     {
-      return (y[0] + y[1]) + y[2];
+      T s = y[0];
+#pragma unroll
+      for (int o = 1; o < W; ++o)
+        s = s + y[o];
+      return s;
     }
-  } // namespace q1
+  } // namespace col
 } // namespace lbfem_amd

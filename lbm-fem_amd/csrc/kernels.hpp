@@ -49,7 +49,21 @@ namespace lbfem_amd
     kTileVariants = 12
   };
 
-  constexpr int kMarchOut = 62; // output columns per wavefront strip of the row-march kernel
+  // output columns per wavefront strip of the row-march kernel on Q_p (p lanes of halo each side)
+  constexpr int
+  march_out(const int p)
+  {
+    return 64 - 2 * p;
+  }
+
+  // The row-march kernel runs on Q1 and, where its five-row register window fits beside two
+  // wavefronts per SIMD, on Q2 (the mixed form's FP32 populations); other forms run the LDS tile
+  // kernel under every variant (the same results bit for bit).
+  constexpr bool
+  march_runs(const int p, const bool fp32)
+  {
+    return p == 1 || (p == 2 && fp32);
+  }
 
   // the row-march kernel addresses a population block (Q vectors of the mesh's stride) with
   // 32-bit byte offsets: it runs on blocks under 4 GiB (the LDS tile kernel takes the rest)
@@ -174,19 +188,23 @@ namespace lbfem_amd
   // On Q2/Q3 the march variants run as the 32 x 8 tile, 20-30 % ahead of 32 x 16 in FP64 and
   // 2 % behind it in FP32.
   inline int
-  size_default_variant(const long long nodes, const bool)
+  size_default_variant(const long long nodes, const bool, const int p = 1)
   {
     if (nodes < 64LL * 4 * 1024)
       return kTileSplit2W40;
+    // Q2 (FP32) below 768^2 nodes: the tile kernel (513^2: 26 us against 30-34 for the row march)
+    if (p == 2 && nodes < 768LL * 768)
+      return kTileWide;
     return nodes < 768LL * 768 ? kMarch1 : nodes < 1536LL * 1536 ? kMarch4 : nodes < 3072LL * 3072 ? kMarch8 : kMarch16;
   }
-  // The default shape of a batch group: on Q1 the row-march kernel over 4-row segments (on the
-  // 96-instance seeded batch 19 % ahead of the best tile shape in FP32, 9 % in FP64); on Q2/Q3
-  // and for the consistent-mass groups' advection (tile lists only) the tile shapes above.
+  // The default shape of a batch group: where the row-march kernel runs (march_runs), over 4-row
+  // segments (on the 96-instance seeded batch 19 % ahead of the best tile shape in FP32 on Q1,
+  // 9 % in FP64); elsewhere, and for the consistent-mass groups' advection (tile lists only,
+  // p = 0 here), the tile shapes above.
   inline int
   batch_default_variant(const long long nodes, const bool fp32, const int p = 0)
   {
-    if (p == 1)
+    if (p > 0 && march_runs(p, fp32))
       return kMarch4;
     const int v = size_default_variant(nodes, fp32);
     return v < kTileLdsVariants ? v : fp32 ? kTileTall : kTileWide;
@@ -194,7 +212,7 @@ namespace lbfem_amd
   inline int
   tile_default_variant(const Geo &G, const bool fp32)
   {
-    return size_default_variant((long long)(G.P + 1) * (G.P + 1), fp32);
+    return size_default_variant((long long)(G.P + 1) * (G.P + 1), fp32, G.p);
   }
   // nodal operations over every node (collision.h:78-168)
   hipError_t collide_launch(const Geo &G, double *g, double omega, hipStream_t stream);

@@ -1,18 +1,18 @@
-// The row-march kernel of the Q1 streaming step: the same fused step (or
-// advection right-hand side) as the tile kernel, computed without staging the
-// populations in LDS.
+// The row-march kernel of the streaming step on Q1 and Q2 meshes: the same fused
+// step (or advection right-hand side) as the tile kernel, computed without staging
+// the populations in LDS.
 //
-// A wavefront owns a strip of 62 node columns (lanes 1..62; lanes 0 and 63 hold
-// the neighbouring columns) and a segment of H node rows, and marches up the
-// segment one row at a time. Each lane keeps its column's three rows j - 1, j,
-// j + 1 of the moving populations (f, and h = a f + b feq for Lee-Lin) in
-// registers; each new row's nodal prologue (collision or equilibria) is done once
-// per node and the oldest row dropped. The operator is the column form of
-// q1_column.hpp: each lane evaluates the y contractions of its own column once,
-// forms the contributions w_0, w_1, w_2 of that column to the nodes x + 1, x,
-// x - 1, and the lanes exchange w_0 and w_2 by wavefront DPP shifts, so every node
-// gets r = (w_0 + w_1) + w_2 bit for bit as the tile kernel computes it. The wall
-// surface terms take their neighbouring columns the same way. The row's own f_0
+// A wavefront owns a strip of 64 - 2p node columns (lanes p .. 63 - p; the p lanes
+// on either side hold the neighbouring columns) and a segment of H node rows, and
+// marches up the segment one row at a time. Each lane keeps its column's 2p + 1
+// rows j - p .. j + p of the moving populations (f, and h = a f + b feq for
+// Lee-Lin) in registers; each new row's nodal prologue (collision or equilibria)
+// is done once per node and the oldest row dropped. The operator is the column
+// form of column.hpp: each lane evaluates the y contractions of its own column
+// once, forms the contributions w_0 .. w_2p of that column to the nodes x + p ..
+// x - p, and the lanes exchange them by wavefront DPP shifts, so every node gets
+// r = ((w_0 + w_1) + ...) + w_2p bit for bit as the tile kernel computes it. The
+// wall surface terms take their neighbouring columns the same way. A row's own f_0
 // and (Lee-Lin) equilibria wait in a small per-lane LDS slot until the row's
 // epilogue, and the lane's x-band entries sit in LDS (read per row).
 //
@@ -33,14 +33,13 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
-#include "q1_column.hpp"
+#include "column.hpp"
 
 namespace lbfem_amd
 {
   namespace march
   {
-    constexpr int kOut   = kMarchOut; // output columns per wavefront
-    constexpr int kWaves = 4;  // wavefronts per block (independent; no barrier)
+    constexpr int kWaves = 4; // wavefronts per block (independent; no barrier)
 
     // a value from the lane below (lane l receives lane l - 1's; lane 0 its own)
     __device__ __forceinline__ double
@@ -75,6 +74,20 @@ namespace lbfem_amd
       return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));
     }
 
+    // a value from the lane N below (N > 0: lane l receives lane l - N's) or above (N < 0: lane
+    // l - N's), one lane per step; lanes at the wavefront's edge receive what they hold
+    template <int N, typename V>
+    __device__ __forceinline__ V
+    shifted(const V v)
+    {
+      if constexpr (N > 0)
+        return shifted<N - 1>(from_below(v));
+      else if constexpr (N < 0)
+        return shifted<N + 1>(from_above(v));
+      else
+        return v;
+    }
+
     // one row of a lane's column: the moving populations f (g* for Bardow) and, for
     // Lee-Lin, h
     template <typename T, bool LL>
@@ -97,18 +110,20 @@ namespace lbfem_amd
         }
     }
 
-    // band entry k of row i of a 1-D operator (0 outside the rows 0..P)
+    // band entry k of row i of a 1-D operator of W entries per row (0 outside the rows 0..P)
+    template <int W>
     __device__ __forceinline__ double
     band_at(const double *B, const int i, const int k, const int P)
     {
-      return (i >= 0 && i <= P) ? ((const __attribute__((address_space(1))) double *)B)[i * 3 + k] : 0.;
+      return (i >= 0 && i <= P) ? ((const __attribute__((address_space(1))) double *)B)[i * W + k] : 0.;
     }
     // the same for a row index uniform over the wavefront: a scalar load (the bands are
     // read-only during a launch)
+    template <int W>
     __device__ __forceinline__ double
     band_uniform(const double *B, const int i, const int k, const int P)
     {
-      return (i >= 0 && i <= P) ? ((const __attribute__((address_space(4))) double *)B)[i * 3 + k] : 0.;
+      return (i >= 0 && i <= P) ? ((const __attribute__((address_space(4))) double *)B)[i * W + k] : 0.;
     }
     // *p for an address uniform over the wavefront: a scalar load (read-only data)
     template <typename V>
@@ -160,27 +175,30 @@ namespace lbfem_amd
 
   namespace march
   {
-    // the per-wavefront LDS of the kernel: a row's own f_0 and (Lee-Lin) equilibria by row parity,
-    // and the lane's x-band entries (read per row: no registers held)
-    template <typename T, bool LL>
+    // the per-wavefront LDS of the kernel: a row's own f_0 and (Lee-Lin) equilibria, for the p + 1
+    // rows between a row's prologue and its epilogue, and the lane's x-band entries (read per row:
+    // no registers held)
+    template <typename T, bool LL, int p>
     struct Lds
     {
       static constexpr int NX = LL ? Q + 1 : 1;
-      T                    ext[kWaves][2][NX][64];
-      T                    xbl[kWaves][4][3][64];
+      T                    ext[kWaves][p + 1][NX][64];
+      T                    xbl[kWaves][4][2 * p + 1][64];
     };
 
     // one wavefront's march: wavefront gw of the mesh's list (strip gw mod S.tiles_x, row segment
     // gw / S.tiles_x of S.rows rows)
-    template <typename T, int MODE>
+    template <typename T, int MODE, int p>
     __device__ __forceinline__ void
     body(const Geo &G, const TileParams &S, const T *__restrict__ fin, const T *__restrict__ feq_in,
          T *__restrict__ out, T *__restrict__ rhs_out, const int gw, const int wid,
-         Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin> &lds)
+         Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin, p> &lds)
   {
     constexpr bool LL   = (MODE == kStepLeeLin || MODE == kAdvectLeeLin);
     constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
     constexpr int  NX   = LL ? Q + 1 : 1; // per-lane extras of a row: f_0 and (Lee-Lin) the equilibria
+    constexpr int  W    = 2 * p + 1;     // band entries, window rows
+    constexpr int  KO   = march_out(p);  // output columns per strip
     using R             = Row<T, LL>;
     auto           &ext  = lds.ext;
     auto           &xbl  = lds.xbl;
@@ -189,29 +207,29 @@ namespace lbfem_amd
     const int strip = gw % S.tiles_x, seg = gw / S.tiles_x;
     const int P = G.P, H = S.rows;
     const int jb = seg * H, je = min(jb + H, P + 1);
-    const int x  = strip * kOut - 1 + lane; // this lane's column position
+    const int x  = strip * KO - p + lane; // this lane's column position
     const int xw = wrap_line(x, P, G.ax.periodic);
-    const bool outl = lane >= 1 && lane <= kOut && x <= P;
+    const bool outl = lane >= p && lane < p + KO && x <= P;
 
     const T a_c = T(S.a), b_c = T(S.b), c_c = T(S.c), hdt2 = T(0.5 * S.dt * S.dt);
     const T sa  = LL ? T(1) : a_c;
     // Per-lane values that do not change over the rows, read once with unconditional loads at
     // clamped indices: a conditional (divergent) vector load in the row loop would make the
     // wavefront wait for every memory operation in flight, the next row's loads included.
-    // x bands: entry k of output node x + 1 - k (the nodes this column feeds), bands 0 M, 1 D, 2 K, 3 D^T
+    // x bands: entry k of output node x + p - k (the nodes this column feeds), bands 0 M, 1 D, 2 K, 3 D^T
     {
       const double *xb[4] = {G.ax.M, G.ax.D, G.ax.K, G.ax.DT};
-      double        bv[4][3];
+      double        bv[4][W];
 #pragma unroll
       for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          bv[b][k] = ((const __attribute__((address_space(1))) double *)xb[b])[min(max(x + 1 - k, 0), P) * 3 + k];
+        for (int k = 0; k < W; ++k)
+          bv[b][k] = ((const __attribute__((address_space(1))) double *)xb[b])[min(max(x + p - k, 0), P) * W + k];
 #pragma unroll
       for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          xbl[wid][b][k][lane] = (x + 1 - k >= 0 && x + 1 - k <= P) ? T(bv[b][k]) : T(0);
+        for (int k = 0; k < W; ++k)
+          xbl[wid][b][k][lane] = (x + p - k >= 0 && x + p - k <= P) ? T(bv[b][k]) : T(0);
     }
     // 1 / lumped mass of the lane's column (output lanes: 0 <= x <= P)
     const double mlx = ((const __attribute__((address_space(1))) double *)G.ax.ml_inv)[min(max(x, 0), P)];
@@ -222,7 +240,9 @@ namespace lbfem_amd
     const Rsrc     r_out = block_rsrc(out, Q * sb), r_rhs = block_rsrc(rhs_out, NM * sb);
     // the wall columns x = 0 and x = P (non-periodic x) and the strips that hold them
     const bool xwall_strip = !G.ax.periodic && (x == 0 || x == P);
-    const bool xwall_wave  = !G.ax.periodic && (strip == 0 || strip * kOut + kOut > P);
+    const bool xwall_wave  = !G.ax.periodic && (strip == 0 || strip * KO + KO > P);
+    // the LDS slot of a row's extras: p + 1 rows live between a row's prologue and its epilogue
+    auto slot = [](const int tr) { return (tr + p + 1) % (p + 1); };
 
     // ------------------------------------------------------------------ loads and prologue
     // the loads of row tr (node (0, 0) stands in outside a non-periodic axis; the prologue
@@ -287,7 +307,7 @@ namespace lbfem_amd
       const bool ok = xw >= 0 && yw >= 0;
       double     wx = 0., wy = 0.;
       const bool wall = yw >= 0 && wall_at(max(xw, 0), yw, wx, wy) && ok;
-      T(&e)[NX][64]   = ext[wid][tr & 1];
+      T(&e)[NX][64]   = ext[wid][slot(tr)];
       if (!ok)
 #pragma unroll
         for (int a = 0; a < Q; ++a)
@@ -307,7 +327,7 @@ namespace lbfem_amd
               fe[a] = bload<T>(r_eq, uint32_t(yw * G.np + xw) * uint32_t(sizeof(T)), a * sb);
 #pragma unroll
           for (int a = 1; a < Q; ++a)
-            row.h[a - 1] = q1::fma_t(a_c, v[a], b_c * fe[a]);
+            row.h[a - 1] = col::fma_t(a_c, v[a], b_c * fe[a]);
 #pragma unroll
           for (int a = 0; a < Q; ++a)
             e[1 + a][lane] = fe[a];
@@ -321,20 +341,23 @@ namespace lbfem_amd
     };
 
     // ------------------------------------------------------------------ one output row
-    auto output = [&](const int j, const R &lo, const R &mid, const R &hi, auto &&between) {
-      T yb[4][3];
+    // row j from the window rw, whose slot (B + o) mod W holds row j - p + o
+    auto output = [&](auto Bc, const int j, const R (&rw)[W], auto &&between) {
+      constexpr int B   = decltype(Bc)::value;
+      const R      &mid = rw[(B + p) % W];
+      T             yb[4][W];
       {
         const double *ybp[4] = {G.ay.M, G.ay.D, G.ay.K, G.ay.DT};
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-          for (int o = 0; o < 3; ++o)
-            yb[b][o] = T(band_uniform(ybp[b], j, o, P));
+          for (int o = 0; o < W; ++o)
+            yb[b][o] = T(band_uniform<W>(ybp[b], j, o, P));
       }
       const bool ywall = !G.ay.periodic && (j == 0 || j == P);
       // FP32: the row sums of the M and D^T bands (the centred form's row terms)
       const int  ln    = opaque(lane);
-      const T mM = q1::row_sum(yb[0]), mT = q1::row_sum(yb[3]);
+      const T mM = col::row_sum(yb[0]), mT = col::row_sum(yb[3]);
       T          xinc[NM];
       // the next row's loads go out before the operator: in flight for the whole operator and
       // epilogue (their registers are those the prologue of this row's successor consumed), and
@@ -346,35 +369,49 @@ namespace lbfem_amd
       // products the tile kernel forms at run time, and no per-population value stays live
       // across the rows
       for_pops<1, Q>([&](auto I) {
-          constexpr int     al = decltype(I)::value;
-          const q1::Coef<T> k  = q1::coef<T>(T(ex_of(al)), T(ey_of(al)), sa, c_c);
-          T                 r  = T(0);
+          constexpr int      al = decltype(I)::value;
+          const col::Coef<T> k  = col::coef<T>(T(ex_of(al)), T(ey_of(al)), sa, c_c);
+          T                  r  = T(0);
           if (k.nx || k.ny)
             {
-              const T fv[3] = {lo.f[al - 1], mid.f[al - 1], hi.f[al - 1]};
-              T       hv[3];
+              T fv[W], hv[W];
 #pragma unroll
-              for (int o = 0; o < 3; ++o)
-                hv[o] = LL ? (o == 0 ? lo.h[al - 1] : o == 1 ? mid.h[al - 1] : hi.h[al - 1]) : fv[o];
-              q1::Cols<T> u0, u1, u2;
-              if constexpr (sizeof(T) == 8)
-                u0 = u1 = u2 = q1::columns<T, LL>(yb, fv, hv, k);
-              else
+              for (int o = 0; o < W; ++o)
                 {
-                  // centred (FP32): the column's contractions once, the row terms per fed node with
-                  // the neighbouring columns' centre values
-                  const q1::CCols<T> cc = q1::ccolumns<T, LL>(yb, fv, hv, k);
-                  const T            fR = from_above(fv[1]), fL = from_below(fv[1]);
-                  const T            hR = LL ? from_above(hv[1]) : fR, hL = LL ? from_below(hv[1]) : fL;
-                  u0 = q1::assemble<T, LL>(cc, mM, mT, hv[1] - hR, fv[1] - fR, true, k);
-                  u1 = q1::assemble<T, LL>(cc, mM, mT, T(0), T(0), false, k);
-                  u2 = q1::assemble<T, LL>(cc, mM, mT, hv[1] - hL, fv[1] - fL, true, k);
+                  fv[o] = rw[(B + o) % W].f[al - 1];
+                  hv[o] = LL ? rw[(B + o) % W].h[al - 1] : fv[o];
                 }
               const auto &xr = xbl[wid];
-              const T w0 = q1::xcontract(xr[0][0][ln], xr[1][0][ln], xr[2][0][ln], xr[3][0][ln], u0, k);
-              const T w1 = q1::xcontract(xr[0][1][ln], xr[1][1][ln], xr[2][1][ln], xr[3][1][ln], u1, k);
-              const T w2 = q1::xcontract(xr[0][2][ln], xr[1][2][ln], xr[2][2][ln], xr[3][2][ln], u2, k);
-              r                    = (from_below(w0) + w1) + from_above(w2);
+              // w_kk: this column's contribution to the node x + p - kk, then moved to that node's
+              // lane and summed in the order of kk
+              // FP64: the column values, once; FP32: the column-centred contractions, once
+              col::Cols<T>  u64{};
+              col::CCols<T> cc{};
+              if constexpr (sizeof(T) == 8)
+                u64 = col::columns<T, LL, W>(yb, fv, hv, k);
+              else
+                cc = col::ccolumns<T, LL, W>(yb, fv, hv, k);
+              for_pops<0, W>([&](auto KK) {
+                constexpr int kk = decltype(KK)::value;
+                col::Cols<T>  u;
+                if constexpr (sizeof(T) == 8)
+                  u = u64;
+                else
+                  {
+                    // centred (FP32): the row terms with the fed node's centre values
+                    if constexpr (kk == p)
+                      u = col::assemble<T, LL>(cc, mM, mT, T(0), T(0), false, k);
+                    else
+                      {
+                        const T fC = shifted<kk - p>(fv[p]);
+                        const T hC = LL ? shifted<kk - p>(hv[p]) : fC;
+                        u          = col::assemble<T, LL>(cc, mM, mT, hv[p] - hC, fv[p] - fC, true, k);
+                      }
+                  }
+                const T w  = col::xcontract(xr[0][kk][ln], xr[1][kk][ln], xr[2][kk][ln], xr[3][kk][ln], u, k);
+                const T wm = shifted<p - kk>(w);
+                r          = kk == 0 ? wm : r + wm;
+              });
             }
           xinc[al - 1] = r;
         });
@@ -389,55 +426,65 @@ namespace lbfem_amd
         {
           T mp = T(0);
           for_pops<1, Q>([&](auto I) {
-            constexpr int     al = decltype(I)::value;
-            const q1::Coef<T> k  = q1::coef<T>(T(ex_of(al)), T(ey_of(al)), sa, c_c);
-            T                 fc = T(0);
-            const T    ex = T(ex_of(al)), ey = T(ey_of(al));
+            constexpr int      al = decltype(I)::value;
+            const col::Coef<T> k  = col::coef<T>(T(ex_of(al)), T(ey_of(al)), sa, c_c);
+            T                  fc = T(0);
+            const T            ex = T(ex_of(al)), ey = T(ey_of(al));
             if (ywall && k.ny)
               {
+                // the wall-normal derivative over rows j .. j + p (bottom) or j - p .. j (top), at
+                // each of the W columns around the node
                 const bool    lo_w = j == 0;
                 const double *bn   = lo_w ? G.ay.b0 : G.ay.b1;
-                const T       r0v  = lo_w ? mid.f[al - 1] : lo.f[al - 1];
-                const T       r1v  = lo_w ? hi.f[al - 1] : mid.f[al - 1];
-                T             dn   = q1::fma_t(T(bn[0]), r0v, T(0));
-                dn                 = q1::fma_t(T(bn[1]), r1v, dn);
-                const T rv[3] = {from_below(mid.f[al - 1]), mid.f[al - 1], from_above(mid.f[al - 1])};
-                const T dv[3] = {from_below(dn), dn, from_above(dn)};
-                T       t = T(0), nrm = T(0);
+                T             dn   = T(0);
 #pragma unroll
-                for (int o = 0; o < 3; ++o)
+                for (int l = 0; l <= p; ++l)
+                  dn = col::fma_t(T(bn[l]), lo_w ? rw[(B + p + l) % W].f[al - 1] : rw[(B + l) % W].f[al - 1], dn);
+                T rv[W], dv[W];
+                for_pops<0, W>([&](auto O) {
+                  constexpr int o = decltype(O)::value;
+                  rv[o]           = shifted<p - o>(mid.f[al - 1]);
+                  dv[o]           = shifted<p - o>(dn);
+                });
+                T t = T(0), nrm = T(0);
+#pragma unroll
+                for (int o = 0; o < W; ++o)
                   {
-                    t   = q1::fma_t(T(band_at(G.ax.D, opaque(x), o, P)), rv[o], t);
-                    nrm = q1::fma_t(T(band_at(G.ax.M, opaque(x), o, P)), dv[o], nrm);
+                    t   = col::fma_t(T(band_at<W>(G.ax.D, opaque(x), o, P)), rv[o], t);
+                    nrm = col::fma_t(T(band_at<W>(G.ax.M, opaque(x), o, P)), dv[o], nrm);
                   }
-                fc = q1::fma_t(hdt2 * (lo_w ? -ey : ey), q1::fma_t(ex, t, ey * nrm), fc);
+                fc = col::fma_t(hdt2 * (lo_w ? -ey : ey), col::fma_t(ex, t, ey * nrm), fc);
               }
             if (xwall_wave && k.nx)
               {
-                // lanes at x = 0 take column 1 from the lane above, lanes at x = P column P - 1 from below
+                // lanes at x = 0 take columns 1 .. p from the lanes above, lanes at x = P columns
+                // P - p .. P - 1 from below: column c_l = x + l (x = 0) or x - p + l (x = P)
                 const bool lo_w = x == 0;
-                const T    fr[3] = {lo.f[al - 1], mid.f[al - 1], hi.f[al - 1]};
-                T          nb[3];
-#pragma unroll
-                for (int o = 0; o < 3; ++o)
-                  {
-                    const T up = from_above(fr[o]), dw = from_below(fr[o]);
-                    nb[o]      = lo_w ? up : dw;
-                  }
+                T          cl[W][p + 1];
+                for_pops<0, W>([&](auto O) {
+                  constexpr int o = decltype(O)::value;
+                  const T       f = rw[(B + o) % W].f[al - 1];
+                  for_pops<0, p + 1>([&](auto L) {
+                    constexpr int l = decltype(L)::value;
+                    const T       up = shifted<-l>(f), dw = shifted<p - l>(f);
+                    cl[o][l]         = lo_w ? up : dw;
+                  });
+                });
                 if (xwall_strip && outl)
                   {
                     const double *bn = lo_w ? G.ax.b0 : G.ax.b1;
                     T             t = T(0), nrm = T(0);
 #pragma unroll
-                    for (int o = 0; o < 3; ++o)
+                    for (int o = 0; o < W; ++o)
                       {
-                        const T c0 = lo_w ? fr[o] : nb[o], c1 = lo_w ? nb[o] : fr[o];
-                        T       dn = q1::fma_t(T(bn[0]), c0, T(0));
-                        dn         = q1::fma_t(T(bn[1]), c1, dn);
-                        t          = q1::fma_t(yb[1][o], fr[o], t);
-                        nrm        = q1::fma_t(yb[0][o], dn, nrm);
+                        T dn = T(0);
+#pragma unroll
+                        for (int l = 0; l <= p; ++l)
+                          dn = col::fma_t(T(bn[l]), cl[o][l], dn);
+                        t   = col::fma_t(yb[1][o], rw[(B + o) % W].f[al - 1], t);
+                        nrm = col::fma_t(yb[0][o], dn, nrm);
                       }
-                    fc = q1::fma_t(hdt2 * (lo_w ? -ex : ex), q1::fma_t(ex, nrm, ey * t), fc);
+                    fc = col::fma_t(hdt2 * (lo_w ? -ex : ex), col::fma_t(ex, nrm, ey * t), fc);
                   }
               }
             xinc[al - 1] = xinc[al - 1] + fc;
@@ -463,7 +510,7 @@ namespace lbfem_amd
           double     wx = 0., wy = 0.;
           const bool wall = wall_at(gi, gj, wx, wy);
           T          own[Q];
-          const T(&e)[NX][64] = ext[wid][j & 1];
+          const T(&e)[NX][64] = ext[wid][slot(j)];
           own[0]              = e[0][lane];
 #pragma unroll
           for (int a = 1; a < Q; ++a)
@@ -510,7 +557,7 @@ namespace lbfem_amd
       // them no increment (M_L^-1 is 0 on constrained rows, so the tile kernel's increment times
       // 0 is an exact zero) and the right-hand side is 0, so only nodal work on their own entries
       // remains: the collision (Bardow), the predictor-corrector with zero increments (Lee-Lin)
-      if ((G.ay.periodic && j == P) || (G.ax.periodic && strip * kOut + kOut > P - 1))
+      if ((G.ay.periodic && j == P) || (G.ax.periodic && strip * KO + KO > P - 1))
         if (outl && slave)
           {
             double     wx = 0., wy = 0.;
@@ -561,35 +608,54 @@ namespace lbfem_amd
     };
 
     // ------------------------------------------------------------------ the march
-    // start-up: the first three rows' loads go out together (one memory round trip, not three)
-    R r0, r1, r2;
+    // start-up: the first W rows' loads go out together (one memory round trip, not W)
+    R rw[W];
     T v[Q];
     {
-      T v0[Q], v1[Q];
-      load(jb - 1, v0);
-      load(jb, v1);
-      load(jb + 1, v);
-      prologue(jb - 1, v0, r0);
-      prologue(jb, v1, r1);
+      T vs[W - 1][Q];
+#pragma unroll
+      for (int o = 0; o < W - 1; ++o)
+        load(jb - p + o, vs[o]);
+      load(jb + p, v);
+#pragma unroll
+      for (int o = 0; o < W - 1; ++o)
+        prologue(jb - p + o, vs[o], rw[o]);
     }
-    // one row: the prologue of row j + 1 from its loads (issued during the previous row), the
-    // operator of row j, the loads of row j + 2, the epilogue and stores of row j
-    auto advance = [&](const int j, const R &lo, const R &mid, R &hi) {
-      prologue(j + 1, v, hi);
-      output(j, lo, mid, hi, [&]() {
+    // one row (window base B): the prologue of row j + p from its loads (issued during the
+    // previous row), the operator of row j, the loads of row j + p + 1, the epilogue and stores
+    // of row j
+    auto advance = [&](auto Bc, const int j) {
+      constexpr int B = decltype(Bc)::value;
+      prologue(j + p, v, rw[(B + W - 1) % W]);
+      output(Bc, j, rw, [&]() {
         if (j + 1 < je)
-          load(j + 2, v);
+          load(j + p + 1, v);
       });
     };
-    for (int j = jb; j < je; j += 3)
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    static_assert(W == 3 || W == 5, "the row-march kernel runs on Q1 and Q2");
+    for (int j = jb; j < je; j += W)
       {
-        advance(j, r0, r1, r2);
+        advance(I0{}, j);
         if (j + 1 >= je)
           break;
-        advance(j + 1, r1, r2, r0);
+        advance(I1{}, j + 1);
         if (j + 2 >= je)
           break;
-        advance(j + 2, r2, r0, r1);
+        advance(I2{}, j + 2);
+        if constexpr (W == 5)
+          {
+            if (j + 3 >= je)
+              break;
+            advance(I3{}, j + 3);
+            if (j + 4 >= je)
+              break;
+            advance(I4{}, j + 4);
+          }
       }
   }
   } // namespace march
@@ -598,13 +664,13 @@ namespace lbfem_amd
   // of the list run along a band of row segments, so that a block's wavefronts are neighbouring
   // strips of one segment: they read the same rows at about the same time and share the halo
   // columns' cache lines in L2.
-  template <typename T, int MODE>
+  template <typename T, int MODE, int p>
   __global__ void __launch_bounds__(64 * march::kWaves)
   k_march(const Geo G, const TileParams S, const T *__restrict__ fin, const T *__restrict__ feq_in,
           T *__restrict__ out, T *__restrict__ rhs_out)
   {
     using namespace march;
-    __shared__ Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin> lds;
+    __shared__ Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin, p> lds;
     const int wid   = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwave = S.tiles_x * S.tiles_y;
     const int nb    = (nwave + kWaves - 1) / kWaves;
@@ -614,7 +680,7 @@ namespace lbfem_amd
     const int gw    = tb * kWaves + wid;
     if (tb >= nb || gw >= nwave)
       return;
-    body<T, MODE>(G, S, fin, feq_in, out, rhs_out, gw, wid, lds);
+    body<T, MODE, p>(G, S, fin, feq_in, out, rhs_out, gw, wid, lds);
   }
 
   // Many meshes of one degree and form (a batch group): the wavefronts of all members form one
@@ -622,13 +688,13 @@ namespace lbfem_amd
   // row segments of S.rows rows); wavefront w of the grid takes list entries w, w + 4 gridDim.x,
   // ... in the XCD-aware block order, so that any grid covers the list and a captured launch keeps
   // working after members join or leave. Each member flips its own population buffers by parity.
-  template <typename T, int MODE>
+  template <typename T, int MODE, int p>
   __global__ void __launch_bounds__(64 * march::kWaves)
   k_march_batch(const BatchEntry<T> *__restrict__ E, const BatchCounts *__restrict__ counts, const int parity)
   {
     using namespace march;
     constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
-    __shared__ Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin> lds;
+    __shared__ Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin, p> lds;
     const int wid    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n_inst = counts->n_inst, n_waves = counts->n_tiles;
     const int nb8    = gridDim.x; // a multiple of 8
@@ -647,7 +713,7 @@ namespace lbfem_amd
         const BatchEntry<T> &e   = E[lo];
         const int            cur = (e.par + parity) & 1;
         T                   *out = (STEP || e.S.collide) ? e.f[cur ^ 1] : nullptr;
-        body<T, MODE>(e.G, e.S, e.f[cur], nullptr, out, e.rhs, gw - e.tile_begin, wid, lds);
+        body<T, MODE, p>(e.G, e.S, e.f[cur], nullptr, out, e.rhs, gw - e.tile_begin, wid, lds);
       }
   }
 
@@ -662,28 +728,29 @@ namespace lbfem_amd
   }
 
   // Launch of the row-march kernel over the mesh with H rows per wavefront segment.
-  template <typename T, int MODE>
+  template <typename T, int MODE, int p>
   hipError_t
   launch_march(const int rows, const Geo &G, TileParams S, const T *fin, const T *feq, T *out, T *rhs,
                hipStream_t stream)
   {
     S.rows         = rows;
-    S.tiles_x      = (G.P + 1 + march::kOut - 1) / march::kOut; // strips
+    S.tiles_x      = (G.P + 1 + march_out(p) - 1) / march_out(p); // strips
     S.tiles_y      = (G.P + 1 + rows - 1) / rows;               // row segments
     const int nw   = S.tiles_x * S.tiles_y;
     const int nb   = (nw + march::kWaves - 1) / march::kWaves;
     const int nb8  = (nb + 7) & ~7;
-    hipLaunchKernelGGL((k_march<T, MODE>), dim3(nb8), dim3(64 * march::kWaves), 0, stream, G, S, fin, feq, out, rhs);
+    hipLaunchKernelGGL((k_march<T, MODE, p>), dim3(nb8), dim3(64 * march::kWaves), 0, stream, G, S, fin, feq, out,
+                       rhs);
     return hipGetLastError();
   }
 
   // grid: blocks of the batch launch (a multiple of 8; its wavefronts stride over the list)
-  template <typename T, int MODE>
+  template <typename T, int MODE, int p>
   hipError_t
   launch_march_batch(const BatchEntry<T> *table, const BatchCounts *counts, const int parity, const int grid,
                      hipStream_t stream)
   {
-    hipLaunchKernelGGL((k_march_batch<T, MODE>), dim3(grid), dim3(64 * march::kWaves), 0, stream, table, counts,
+    hipLaunchKernelGGL((k_march_batch<T, MODE, p>), dim3(grid), dim3(64 * march::kWaves), 0, stream, table, counts,
                        parity);
     return hipGetLastError();
   }

@@ -33,7 +33,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "march_kernel.hpp"
-#include "q1_column.hpp"
+#include "column.hpp"
 
 namespace lbfem_amd
 {
@@ -70,7 +70,7 @@ namespace lbfem_amd
       static constexpr bool STEP = (MODE == kStepBardow || MODE == kStepLeeLin);
       static constexpr int  NF   = LL ? Q + NM : Q;      // f (or g*) and, for Lee-Lin, h of the moving populations
       static constexpr int  R    = (NH + NT - 1) / NT;   // prologue rounds
-      static constexpr bool WGT  = p == 1;               // Q1: the operator in the column form
+      static constexpr bool WGT  = p == 1;               // Q1: the face code reads the bands again (registers)
       // LDS row pitch of the staged planes. The LDS serves 128 bytes a cycle (LB lanes of T);
       // where a row of TX threads does not end on a multiple of LB lanes, a group of LB lanes
       // spans two rows and hits the same banks twice unless the pitch is congruent to TX
@@ -138,8 +138,6 @@ namespace lbfem_amd
       // constants (its bands' row or column sums vanish), and FP32 then carries the
       // small differences instead of cancelling large values
       constexpr bool CEN        = sizeof(T) == 4;
-      // Q1: the column form shared with the row-march kernel (q1_column.hpp), centred in FP32
-      constexpr bool COL        = C::WGT;
       constexpr int  PX         = C::PX;
       T(&F)[NF][HY][PX]         = sm.F;
       // the population buffers are device (global) memory: say so, so that the loads and
@@ -203,7 +201,7 @@ namespace lbfem_amd
                   fe[a] = gfeq[a * G.stride + node_index(G, gis[r], gjs[r])];
 #pragma unroll
               for (int a = 1; a < Q; ++a)
-                F[Q + a - 1][ky][kx] = q1::fma_t(a_c, v[r][a], b_c * fe[a]);
+                F[Q + a - 1][ky][kx] = col::fma_t(a_c, v[r][a], b_c * fe[a]);
               if (r == 0)
 #pragma unroll
                 for (int a = 0; a < Q; ++a)
@@ -244,137 +242,40 @@ namespace lbfem_amd
       // run-time values: compile-time D2Q9 constants let the scheduler interleave the
       // barrier-free populations and spill registers. A population at rest in the sweep
       // gets r = 0 (streaming.h:118-122).
-      // the column form (Q1): the whole increment of population al
-      auto col_pop = [&](const int al) -> T {
-        if constexpr (!COL)
-          return T(0);
-        else
-          {
-            const q1::Coef<T> k = q1::coef<T>(T(S.e[al][0]), T(S.e[al][1]), sa, c_c);
-            if (!k.nx && !k.ny)
-              return T(0);
-            const T(*Ff)[PX] = F[al];
-            const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
-            // FP32: the centre values of the node's row, and the row sums of its M and D^T bands
-            const T f0c = CEN ? Ff[jj + 1][ii + 1] : T(0);
-            const T h0c = (CEN && LL) ? Fh[jj + 1][ii + 1] : f0c;
-            const T mM = CEN ? q1::row_sum(yc[0]) : T(0), mT = CEN ? q1::row_sum(yc[3]) : T(0);
-            T wk[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              {
-                T fv[3], hv[3];
-#pragma unroll
-                for (int o = 0; o < 3; ++o)
-                  {
-                    fv[o] = Ff[jj + o][ii + c];
-                    hv[o] = LL ? Fh[jj + o][ii + c] : fv[o];
-                  }
-                q1::Cols<T> u;
-                if constexpr (CEN)
-                  u = q1::assemble<T, LL>(q1::ccolumns<T, LL>(yc, fv, hv, k), mM, mT, hv[1] - h0c, fv[1] - f0c,
-                                          c != 1, k);
-                else
-                  u = q1::columns<T, LL>(yc, fv, hv, k);
-                wk[c] = q1::xcontract(xc[0][c], xc[1][c], xc[2][c], xc[3][c], u, k);
-              }
-            return (wk[0] + wk[1]) + wk[2];
-          }
-      };
+      // the column form shared with the row-march kernel (column.hpp), centred in FP32: the whole
+      // increment of population al
       auto popA = [&](const int al, T &res) {
-        if constexpr (COL)
-          res = col_pop(al);
-        else
+        const col::Coef<T> k = col::coef<T>(T(S.e[al][0]), T(S.e[al][1]), sa, c_c);
+        res                  = T(0);
+        if (!k.nx && !k.ny)
+          return;
+        const T(*Ff)[PX] = F[al];
+        const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
+        // FP32: the centre values of the node's row, and the row sums of its M and D^T bands
+        const T f0c = CEN ? Ff[jj + p][ii + p] : T(0);
+        const T h0c = (CEN && LL) ? Fh[jj + p][ii + p] : f0c;
+        const T mM = CEN ? col::row_sum(yc[0]) : T(0), mT = CEN ? col::row_sum(yc[3]) : T(0);
+        T       acc = T(0);
+#pragma unroll
+        for (int c = 0; c < W; ++c)
           {
-            const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
-            const bool nx = ex != T(0), ny = ey != T(0);
-            res           = T(0);
-            const T(*Ff)[PX] = F[al];
-            const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
-            const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
-            const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
+            T fv[W], hv[W];
+#pragma unroll
+            for (int o = 0; o < W; ++o)
               {
-                if (!nx && !ny)
-                  return;
-                // Q2, Q3: y bands over the columns' strips, then the column's x bands
-                T acc = 0;
-                if (nx && !ny)
-                  {
-                    const T wd = ex * sa, wk = c_c * ex * ex;
-    #pragma unroll
-                    for (int c = 0; c < W; ++c)
-                      {
-                        T My = 0, Myh = 0;
-    #pragma unroll
-                        for (int o = 0; o < W; ++o)
-                          {
-                            My += yc[0][o] * (Ff[jj + o][ii + c] - cf);
-                            if constexpr (LL)
-                              Myh += yc[0][o] * (Fh[jj + o][ii + c] - ch);
-                          }
-                        if constexpr (!LL)
-                          Myh = My;
-                        acc += (xc[1][c] * wd) * Myh + (xc[2][c] * wk) * My;
-                      }
-                  }
-                else if (!nx && ny)
-                  {
-                    const T wd = ey * sa, wk = c_c * ey * ey;
-                    T       y1[W], y2[W];
-    #pragma unroll
-                    for (int o = 0; o < W; ++o)
-                      {
-                        y1[o] = yc[1][o] * wd;
-                        y2[o] = yc[2][o] * wk;
-                      }
-    #pragma unroll
-                    for (int c = 0; c < W; ++c)
-                      {
-                        T s = 0;
-    #pragma unroll
-                        for (int o = 0; o < W; ++o)
-                          {
-                            const T fv = Ff[jj + o][ii + c] - cf;
-                            const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
-                            s += y1[o] * hv + y2[o] * fv;
-                          }
-                        acc += xc[0][c] * s;
-                      }
-                  }
-                else
-                  {
-                    const T wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
-    #pragma unroll
-                    for (int c = 0; c < W; ++c)
-                      {
-                        T My = 0, Dy = 0, Ky = 0, DTy = 0, Myh = 0, Dyh = 0;
-    #pragma unroll
-                        for (int o = 0; o < W; ++o)
-                          {
-                            const T fv = Ff[jj + o][ii + c] - cf;
-                            My += yc[0][o] * fv;
-                            Dy += yc[1][o] * fv;
-                            Ky += yc[2][o] * fv;
-                            DTy += yc[3][o] * fv;
-                            if constexpr (LL)
-                              {
-                                const T hv = Fh[jj + o][ii + c] - ch;
-                                Myh += yc[0][o] * hv;
-                                Dyh += yc[1][o] * hv;
-                              }
-                          }
-                        if constexpr (!LL)
-                          {
-                            Myh = My;
-                            Dyh = Dy;
-                          }
-                        acc += xc[1][c] * (wdx * Myh + wx * DTy) + xc[2][c] * (wk * My) +
-                               xc[0][c] * (wdy * Dyh + wky * Ky) + xc[3][c] * (wx * Dy);
-                      }
-                  }
-                res = acc;
+                fv[o] = Ff[jj + o][ii + c];
+                hv[o] = LL ? Fh[jj + o][ii + c] : fv[o];
               }
+            col::Cols<T> u;
+            if constexpr (CEN)
+              u = col::assemble<T, LL>(col::ccolumns<T, LL>(yc, fv, hv, k), mM, mT, hv[p] - h0c, fv[p] - f0c, c != p,
+                                       k);
+            else
+              u = col::columns<T, LL>(yc, fv, hv, k);
+            const T w = col::xcontract(xc[0][c], xc[1][c], xc[2][c], xc[3][c], u, k);
+            acc       = c == 0 ? w : acc + w; // ((w_0 + w_1) + ...) + w_2p
           }
+        res = acc;
       };
       // the wall surface term of population al at the owned node
       auto face = [&](const int al) -> T {
@@ -413,11 +314,11 @@ namespace lbfem_amd
                     T dn = 0;
 #pragma unroll
                     for (int l = 0; l <= p; ++l)
-                      dn = q1::fma_t(T(bn[l]), Ff[r0 + l][ii + o], dn);
-                    t   = q1::fma_t(xb(1, o), Ff[ty][ii + o], t);
-                    nrm = q1::fma_t(xb(0, o), dn, nrm);
+                      dn = col::fma_t(T(bn[l]), Ff[r0 + l][ii + o], dn);
+                    t   = col::fma_t(xb(1, o), Ff[ty][ii + o], t);
+                    nrm = col::fma_t(xb(0, o), dn, nrm);
                   }
-                fc = q1::fma_t(hdt2 * (lo ? -ey : ey), q1::fma_t(ex, t, ey * nrm), fc);
+                fc = col::fma_t(hdt2 * (lo ? -ey : ey), col::fma_t(ex, t, ey * nrm), fc);
               }
             if (!G.ax.periodic && nx && (gi == 0 || gi == P))
               {
@@ -431,11 +332,11 @@ namespace lbfem_amd
                     T dn = 0;
 #pragma unroll
                     for (int l = 0; l <= p; ++l)
-                      dn = q1::fma_t(T(bn[l]), Ff[jj + o][c0 + l], dn);
-                    t   = q1::fma_t(yb(1, o), Ff[jj + o][tx], t);
-                    nrm = q1::fma_t(yb(0, o), dn, nrm);
+                      dn = col::fma_t(T(bn[l]), Ff[jj + o][c0 + l], dn);
+                    t   = col::fma_t(yb(1, o), Ff[jj + o][tx], t);
+                    nrm = col::fma_t(yb(0, o), dn, nrm);
                   }
-                fc = q1::fma_t(hdt2 * (lo ? -ex : ex), q1::fma_t(ex, nrm, ey * t), fc);
+                fc = col::fma_t(hdt2 * (lo ? -ex : ex), col::fma_t(ex, nrm, ey * t), fc);
               }
           }
         return fc;
@@ -664,10 +565,10 @@ namespace lbfem_amd
         case kMarch4:
         case kMarch8:
         case kMarch16:
-          if constexpr (p == 1)
+          if constexpr (march_runs(p, sizeof(T) == 4))
             {
               if (standard_directions(S) && march_fits(G, sizeof(T)))
-                return launch_march<T, MODE>(march_rows(variant), G, S, fin, feq, out, rhs, stream);
+                return launch_march<T, MODE, p>(march_rows(variant), G, S, fin, feq, out, rhs, stream);
               return launch_tile<tile::Cfg<T, p, 32, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
             }
           else
@@ -704,9 +605,9 @@ namespace lbfem_amd
         case kMarch4:
         case kMarch8:
         case kMarch16:
-          // the table holds each member's march decomposition (Q1) or its 32 x 8 tiles
-          if constexpr (p == 1)
-            return launch_march_batch<T, MODE>(table, counts, parity, grid, stream);
+          // the table holds each member's march decomposition (march_runs) or its 32 x 8 tiles
+          if constexpr (march_runs(p, sizeof(T) == 4))
+            return launch_march_batch<T, MODE, p>(table, counts, parity, grid, stream);
           else
             return launch_batch<tile::Cfg<T, p, 32, 8, 1, MODE>>(table, counts, parity, grid, stream);
         default:

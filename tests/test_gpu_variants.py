@@ -39,6 +39,14 @@ CASES = [
     ("tgv", 20, 1, A.LEELIN, A.MASS_LUMPED, A.MIXED),
     ("couette", 70, 1, A.LEELIN, A.MASS_LUMPED, A.FP64),
     ("couette", 70, 1, A.BARDOW, A.MASS_LUMPED, A.MIXED),
+    # Q2 in FP32, where the row-march kernel runs (strips of 60 columns): walls on four sides, a
+    # last strip of one column (61 nodes), periodic lines, and the advection mode under CG
+    ("cavity", 37, 2, A.LEELIN, A.MASS_LUMPED, A.MIXED),
+    ("cavity", 30, 2, A.BARDOW, A.MASS_LUMPED, A.MIXED),
+    ("tgv", 31, 2, A.BARDOW, A.MASS_LUMPED, A.MIXED),
+    ("tgv", 31, 2, A.LEELIN, A.MASS_LUMPED, A.MIXED),
+    ("couette", 40, 2, A.BARDOW, A.MASS_LUMPED, A.MIXED),
+    ("cavity", 30, 2, A.BARDOW, A.MASS_CG, A.MIXED),
 ]
 
 
