@@ -138,6 +138,9 @@ namespace lbfem_amd
       // constants (its bands' row or column sums vanish), and FP32 then carries the
       // small differences instead of cancelling large values
       constexpr bool CEN        = sizeof(T) == 4;
+      // the column form wherever the row-march kernel may run for this form (the same results bit
+      // for bit under every variant)
+      constexpr bool COL        = march_runs(p, sizeof(T) == 4);
       constexpr int  PX         = C::PX;
       T(&F)[NF][HY][PX]         = sm.F;
       // the population buffers are device (global) memory: say so, so that the loads and
@@ -243,12 +246,11 @@ namespace lbfem_amd
       // barrier-free populations and spill registers. A population at rest in the sweep
       // gets r = 0 (streaming.h:118-122).
       // the column form shared with the row-march kernel (column.hpp), centred in FP32: the whole
-      // increment of population al
-      auto popA = [&](const int al, T &res) {
+      // increment of population al (the forms the row-march kernel runs on: Q1, Q2 in FP32)
+      auto col_pop = [&](const int al) -> T {
         const col::Coef<T> k = col::coef<T>(T(S.e[al][0]), T(S.e[al][1]), sa, c_c);
-        res                  = T(0);
         if (!k.nx && !k.ny)
-          return;
+          return T(0);
         const T(*Ff)[PX] = F[al];
         const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
         // FP32: the centre values of the node's row, and the row sums of its M and D^T bands
@@ -275,7 +277,104 @@ namespace lbfem_amd
             const T w = col::xcontract(xc[0][c], xc[1][c], xc[2][c], xc[3][c], u, k);
             acc       = c == 0 ? w : acc + w; // ((w_0 + w_1) + ...) + w_2p
           }
-        res = acc;
+        return acc;
+      };
+      // the other forms (Q2 in FP64, Q3): y bands over the columns' strips, then the column's x bands,
+      // centred on the node in FP32
+      auto popA = [&](const int al, T &res) {
+        if constexpr (COL)
+          res = col_pop(al);
+        else
+          {
+            const T    ex = T(S.e[al][0]), ey = T(S.e[al][1]);
+            const bool nx = ex != T(0), ny = ey != T(0);
+            res           = T(0);
+            const T(*Ff)[PX] = F[al];
+            const T(*Fh)[PX] = F[LL ? Q + al - 1 : al];
+            const T cf       = CEN ? Ff[jj + p][ii + p] : T(0);
+            const T ch       = (CEN && LL) ? Fh[jj + p][ii + p] : T(0);
+              {
+                if (!nx && !ny)
+                  return;
+                // Q2, Q3: y bands over the columns' strips, then the column's x bands
+                T acc = 0;
+                if (nx && !ny)
+                  {
+                    const T wd = ex * sa, wk = c_c * ex * ex;
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
+                      {
+                        T My = 0, Myh = 0;
+    #pragma unroll
+                        for (int o = 0; o < W; ++o)
+                          {
+                            My += yc[0][o] * (Ff[jj + o][ii + c] - cf);
+                            if constexpr (LL)
+                              Myh += yc[0][o] * (Fh[jj + o][ii + c] - ch);
+                          }
+                        if constexpr (!LL)
+                          Myh = My;
+                        acc += (xc[1][c] * wd) * Myh + (xc[2][c] * wk) * My;
+                      }
+                  }
+                else if (!nx && ny)
+                  {
+                    const T wd = ey * sa, wk = c_c * ey * ey;
+                    T       y1[W], y2[W];
+    #pragma unroll
+                    for (int o = 0; o < W; ++o)
+                      {
+                        y1[o] = yc[1][o] * wd;
+                        y2[o] = yc[2][o] * wk;
+                      }
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
+                      {
+                        T s = 0;
+    #pragma unroll
+                        for (int o = 0; o < W; ++o)
+                          {
+                            const T fv = Ff[jj + o][ii + c] - cf;
+                            const T hv = LL ? Fh[jj + o][ii + c] - ch : fv;
+                            s += y1[o] * hv + y2[o] * fv;
+                          }
+                        acc += xc[0][c] * s;
+                      }
+                  }
+                else
+                  {
+                    const T wdx = ex * sa, wdy = ey * sa, wk = c_c * ex * ex, wky = c_c * ey * ey, wx = c_c * ex * ey;
+    #pragma unroll
+                    for (int c = 0; c < W; ++c)
+                      {
+                        T My = 0, Dy = 0, Ky = 0, DTy = 0, Myh = 0, Dyh = 0;
+    #pragma unroll
+                        for (int o = 0; o < W; ++o)
+                          {
+                            const T fv = Ff[jj + o][ii + c] - cf;
+                            My += yc[0][o] * fv;
+                            Dy += yc[1][o] * fv;
+                            Ky += yc[2][o] * fv;
+                            DTy += yc[3][o] * fv;
+                            if constexpr (LL)
+                              {
+                                const T hv = Fh[jj + o][ii + c] - ch;
+                                Myh += yc[0][o] * hv;
+                                Dyh += yc[1][o] * hv;
+                              }
+                          }
+                        if constexpr (!LL)
+                          {
+                            Myh = My;
+                            Dyh = Dy;
+                          }
+                        acc += xc[1][c] * (wdx * Myh + wx * DTy) + xc[2][c] * (wk * My) +
+                               xc[0][c] * (wdy * Dyh + wky * Ky) + xc[3][c] * (wx * Dy);
+                      }
+                  }
+                res = acc;
+              }
+          }
       };
       // the wall surface term of population al at the owned node
       auto face = [&](const int al) -> T {
