@@ -425,6 +425,15 @@ namespace lbfem_amd
       if (ywall || xwall_wave)
         {
           T mp = T(0);
+          // on a wall row, the node's own rows of Dx and Mx (the tangential derivative and the mass
+          // along the wall), read once for all populations
+          T xDr[W], xMr[W];
+#pragma unroll
+          for (int o = 0; o < W; ++o)
+            {
+              xDr[o] = ywall ? T(band_at<W>(G.ax.D, opaque(x), o, P)) : T(0);
+              xMr[o] = ywall ? T(band_at<W>(G.ax.M, opaque(x), o, P)) : T(0);
+            }
           for_pops<1, Q>([&](auto I) {
             constexpr int      al = decltype(I)::value;
             const col::Coef<T> k  = col::coef<T>(T(ex_of(al)), T(ey_of(al)), sa, c_c);
@@ -440,52 +449,54 @@ namespace lbfem_amd
 #pragma unroll
                 for (int l = 0; l <= p; ++l)
                   dn = col::fma_t(T(bn[l]), lo_w ? rw[(B + p + l) % W].f[al - 1] : rw[(B + l) % W].f[al - 1], dn);
-                T rv[W], dv[W];
+                // column by column (few registers live): t = sum_o Dx[o] f(x - p + o, j), nrm likewise
+                T t = T(0), nrm = T(0);
                 for_pops<0, W>([&](auto O) {
                   constexpr int o = decltype(O)::value;
-                  rv[o]           = shifted<p - o>(mid.f[al - 1]);
-                  dv[o]           = shifted<p - o>(dn);
+                  const T       rv = shifted<p - o>(mid.f[al - 1]), dv = shifted<p - o>(dn);
+                  t                = col::fma_t(xDr[o], rv, t);
+                  nrm              = col::fma_t(xMr[o], dv, nrm);
                 });
-                T t = T(0), nrm = T(0);
-#pragma unroll
-                for (int o = 0; o < W; ++o)
-                  {
-                    t   = col::fma_t(T(band_at<W>(G.ax.D, opaque(x), o, P)), rv[o], t);
-                    nrm = col::fma_t(T(band_at<W>(G.ax.M, opaque(x), o, P)), dv[o], nrm);
-                  }
                 fc = col::fma_t(hdt2 * (lo_w ? -ey : ey), col::fma_t(ex, t, ey * nrm), fc);
               }
             if (xwall_wave && k.nx)
               {
                 // lanes at x = 0 take columns 1 .. p from the lanes above, lanes at x = P columns
-                // P - p .. P - 1 from below: column c_l = x + l (x = 0) or x - p + l (x = P)
-                const bool lo_w = x == 0;
-                T          cl[W][p + 1];
+                // P - p .. P - 1 from below: column c_l = x + l (x = 0) or x - p + l (x = P); one
+                // pass for each side a strip holds (uniform), with only that side's lane shifts
+                T t = T(0);
                 for_pops<0, W>([&](auto O) {
                   constexpr int o = decltype(O)::value;
-                  const T       f = rw[(B + o) % W].f[al - 1];
-                  for_pops<0, p + 1>([&](auto L) {
-                    constexpr int l = decltype(L)::value;
-                    const T       up = shifted<-l>(f), dw = shifted<p - l>(f);
-                    cl[o][l]         = lo_w ? up : dw;
-                  });
+                  t               = col::fma_t(yb[1][o], rw[(B + o) % W].f[al - 1], t);
                 });
-                if (xwall_strip && outl)
+                auto side = [&](auto LO, const double *bn) -> T {
+                  constexpr bool lo = decltype(LO)::value;
+                  T              nrm = T(0);
+                  for_pops<0, W>([&](auto O) {
+                    constexpr int o  = decltype(O)::value;
+                    const T       f  = rw[(B + o) % W].f[al - 1];
+                    T             dn = T(0);
+                    for_pops<0, p + 1>([&](auto L) {
+                      constexpr int l = decltype(L)::value;
+                      dn              = col::fma_t(T(bn[l]), lo ? shifted<-l>(f) : shifted<p - l>(f), dn);
+                    });
+                    nrm = col::fma_t(yb[0][o], dn, nrm);
+                  });
+                  return col::fma_t(hdt2 * (lo ? -ex : ex), col::fma_t(ex, nrm, ey * t), fc);
+                };
+                T fx = fc;
+                if (strip == 0)
                   {
-                    const double *bn = lo_w ? G.ax.b0 : G.ax.b1;
-                    T             t = T(0), nrm = T(0);
-#pragma unroll
-                    for (int o = 0; o < W; ++o)
-                      {
-                        T dn = T(0);
-#pragma unroll
-                        for (int l = 0; l <= p; ++l)
-                          dn = col::fma_t(T(bn[l]), cl[o][l], dn);
-                        t   = col::fma_t(yb[1][o], rw[(B + o) % W].f[al - 1], t);
-                        nrm = col::fma_t(yb[0][o], dn, nrm);
-                      }
-                    fc = col::fma_t(hdt2 * (lo_w ? -ex : ex), col::fma_t(ex, nrm, ey * t), fc);
+                    const T f0 = side(std::true_type{}, G.ax.b0);
+                    fx         = x == 0 ? f0 : fx;
                   }
+                if (strip * KO + KO > P)
+                  {
+                    const T fP = side(std::false_type{}, G.ax.b1);
+                    fx         = x == P ? fP : fx;
+                  }
+                if (xwall_strip && outl)
+                  fc = fx;
               }
             xinc[al - 1] = xinc[al - 1] + fc;
             if constexpr (al & 1)
