@@ -215,7 +215,7 @@ int lbfem_scheme_stage_times(const lbfem_scheme *scheme, double *seconds4);
  * 6: 40 x 8 and 48 x 8 tiles of 320 and 384 threads, 7: 40 x 8 tiles split over
  * 2 threads per node (640 threads); 8-11: the row-march kernel (Q1 meshes, and
  * Q2 meshes of the mixed form; wavefronts own strips of 64 - 2p columns and
- * march up segments of 1, 4, 8 or 16 node rows with the stencil rows in
+ * march up segments of 3, 4, 8 or 16 node rows with the stencil rows in
  * registers; elsewhere variant 0); -1 restores the size-based default). Every
  * variant gives bitwise identical populations. */
 int lbfem_scheme_set_tile_variant(lbfem_scheme *scheme, int variant);

@@ -42,7 +42,7 @@ namespace lbfem_amd
     kTileLdsVariants = 8, // the variants above stage the halo'd tile in LDS (the batch kernel's shapes)
     // the row-march kernel (Q1; elsewhere these run as kTileWide): wavefront strips of 62
     // columns marching up segments of 1, 4, 8 or 16 node rows
-    kMarch1       = 8,
+    kMarch3       = 8,
     kMarch4       = 9,
     kMarch8       = 10,
     kMarch16      = 11,
@@ -77,7 +77,7 @@ namespace lbfem_amd
   inline int
   march_rows(const int variant)
   {
-    return variant == kMarch1 ? 1 : variant == kMarch4 ? 4 : variant == kMarch8 ? 8 : 16;
+    return variant == kMarch3 ? 3 : variant == kMarch4 ? 4 : variant == kMarch8 ? 8 : 16;
   }
 
   template <typename T, int p>
@@ -181,12 +181,11 @@ namespace lbfem_amd
   // The variant used until a measurement picks one, by the node count of the launch: below
   // about four wide-tile wavefronts per SIMD of the device (256 CUs x 4 SIMDs) the 40 x 8 tile
   // split over two threads per node (the fastest measured shape at 257^2 nodes, Q1 and Q2,
-  // FP64 and FP32); above, the row-march kernel with about 256 row segments per strip (the
-  // fastest height measured: 1 row at 513^2 nodes, 4 at 1025^2, 8 at 2049^2, 16 at 4097^2;
-  // Lee-Lin FP64 6 %, 20 %, 33 % and 42 % ahead of the 32 x 8 tile there, FP32 28-49 % ahead
-  // of 32 x 16 from 1025^2).
-  // On Q2/Q3 the march variants run as the 32 x 8 tile, 20-30 % ahead of 32 x 16 in FP64 and
-  // 2 % behind it in FP32.
+  // FP64 and FP32); above, the row-march kernel (Q1 Lee-Lin FP64 segment heights measured: 3
+  // rows fastest at 513^2 and 1025^2 nodes (22.0 and 56.1 us against 25.1 at one row and 59.1 at
+  // four), 8 at 2049^2, 16 at 4097^2; 30-42 % ahead of the 32 x 8 tile from 1025^2, FP32 28-49 %
+  // ahead of 32 x 16). Where the row-march kernel does not run (Q2 in FP64, Q3) the march variants
+  // run as the 32 x 8 tile, 20-30 % ahead of 32 x 16 in FP64.
   inline int
   size_default_variant(const long long nodes, const bool, const int p = 1)
   {
@@ -195,7 +194,7 @@ namespace lbfem_amd
     // Q2 (FP32) below 768^2 nodes: the tile kernel (513^2: 26 us against 30-34 for the row march)
     if (p == 2 && nodes < 768LL * 768)
       return kTileWide;
-    return nodes < 768LL * 768 ? kMarch1 : nodes < 1536LL * 1536 ? kMarch4 : nodes < 3072LL * 3072 ? kMarch8 : kMarch16;
+    return nodes < 1536LL * 1536 ? kMarch3 : nodes < 3072LL * 3072 ? kMarch8 : kMarch16;
   }
   // The default shape of a batch group: where the row-march kernel runs (march_runs), over 4-row
   // segments (on the 96-instance seeded batch 19 % ahead of the best tile shape in FP32 on Q1,

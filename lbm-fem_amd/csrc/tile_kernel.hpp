@@ -660,7 +660,7 @@ namespace lbfem_amd
           return launch_tile<tile::Cfg<T, p, 48, 8, 1, MODE>>(G, S, fin, feq, out, rhs, stream);
         case kTileSplit2W40:
           return launch_tile<tile::Cfg<T, p, 40, 8, 2, MODE>>(G, S, fin, feq, out, rhs, stream);
-        case kMarch1:
+        case kMarch3:
         case kMarch4:
         case kMarch8:
         case kMarch16:
@@ -700,7 +700,7 @@ namespace lbfem_amd
           return launch_batch<tile::Cfg<T, p, 48, 8, 1, MODE>>(table, counts, parity, grid, stream);
         case kTileSplit2W40:
           return launch_batch<tile::Cfg<T, p, 40, 8, 2, MODE>>(table, counts, parity, grid, stream);
-        case kMarch1:
+        case kMarch3:
         case kMarch4:
         case kMarch8:
         case kMarch16:

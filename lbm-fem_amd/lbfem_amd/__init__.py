@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "liblbfem_hip.so")
 
 OK, EINVAL, ENOCONV, EHIP = 0, 1, 2, 3
 TILE_WIDE, TILE_TALL, TILE_SPLIT2, TILE_SPLIT4, TILE_SMALL4, TILE_WIDE40, TILE_WIDE48, TILE_SPLIT2_W40 = range(8)
-MARCH1, MARCH4, MARCH8, MARCH16 = range(8, 12)  # the row-march kernel (Q1; Q2 mixed), rows per segment
+MARCH3, MARCH4, MARCH8, MARCH16 = range(8, 12)  # the row-march kernel (Q1; Q2 mixed), rows per segment
 TILE_VARIANTS = 12
 TILE_LDS_VARIANTS = 8  # the shapes a batch runs
 LEELIN, BARDOW = 0, 1
