@@ -39,7 +39,7 @@ N_VARIANTS = 12
 
 def tile_kernel(variant, t="double"):
     if variant >= len(TILE_SHAPES):
-        return f"k_march<{t}, 1>"
+        return f"k_march<{t}, 1, 1>"  # Lee-Lin step, Q1
     tx, ty, ps = TILE_SHAPES[variant]
     return f"k_tile<Cfg<{t}, 1, {tx}, {ty}, {ps}, 1> >"
 
