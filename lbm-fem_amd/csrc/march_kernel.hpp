@@ -620,28 +620,48 @@ namespace lbfem_amd
 
     // ------------------------------------------------------------------ the march
     // start-up: the first W rows' loads go out together (one memory round trip, not W)
-    R rw[W];
-    T v[Q];
+    // PF2: two rows of loads in flight (v: row j + p, v2: row j + p + 1) on Q1 in FP64, where the
+    // second buffer fits beside the window at two wavefronts per SIMD (Lee-Lin 2049^2 nodes 190
+    // against 195 us, 4097^2 676 against 688); in FP32 one (Lee-Lin 142 against 138 us with two)
+    constexpr bool PF2 = p == 1 && sizeof(T) == 8;
+    R              rw[W];
+    T              v[Q], v2[PF2 ? Q : 1];
     {
       T vs[W - 1][Q];
 #pragma unroll
       for (int o = 0; o < W - 1; ++o)
         load(jb - p + o, vs[o]);
       load(jb + p, v);
+      if constexpr (PF2)
+        if (jb + 1 < je)
+          load(jb + p + 1, v2);
 #pragma unroll
       for (int o = 0; o < W - 1; ++o)
         prologue(jb - p + o, vs[o], rw[o]);
     }
-    // one row (window base B): the prologue of row j + p from its loads (issued during the
-    // previous row), the operator of row j, the loads of row j + p + 1, the epilogue and stores
-    // of row j
+    // one row (window base B): the prologue of row j + p from its loads (issued one or two rows
+    // before), the operator of row j, the loads of the row needed next (j + p + 1, or j + p + 2
+    // with two in flight), the epilogue and stores of row j
     auto advance = [&](auto Bc, const int j) {
       constexpr int B = decltype(Bc)::value;
       prologue(j + p, v, rw[(B + W - 1) % W]);
       output(Bc, j, rw, [&]() {
-        if (j + 1 < je)
+        if constexpr (PF2)
+          {
+            if (j + 2 < je)
+              load(j + p + 2, v);
+          }
+        else if (j + 1 < je)
           load(j + p + 1, v);
       });
+      if constexpr (PF2)
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+          {
+            const T t = v[a];
+            v[a]      = v2[a];
+            v2[a]     = t;
+          }
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
