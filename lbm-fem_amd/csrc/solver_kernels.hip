@@ -283,8 +283,8 @@ namespace lbfem_amd
   // A thread keeps only the 1-D bands its OP uses (registers set the occupancy).
   // One vector per block (blockIdx.y): its operand tile is staged in LDS (one barrier), then
   // each thread applies the bands at its RO owned nodes of one column.
-  // V: the vector type, double (the FP64 solves and the refinement residuals) or
-  // float (the inner solves of the mixed form); scalars and partial sums in FP64.
+  // V: the vector type, double (the FP64 solves) or float (the mixed form's solves);
+  // scalars and partial sums in FP64.
   enum
   {
     kOpMass      = 0,

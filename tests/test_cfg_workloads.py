@@ -7,7 +7,7 @@
   and every CG solve of every step within +-1 iteration of the oracle's.
 - Precision: the Taylor-Green vortex (1,1), U0 = 0.01, Re 100, Bardow with the consistent mass
   (CG 1e-10) on Q1 1024^2 and Q2 512^2 cells for 5000 steps: the mixed form (FP32 deviations,
-  FP32 inner CG refined with FP64 residuals) within 1e-5 relative L2 velocity of the FP64 form,
+  CG on FP32 increments with FP64 scalars and sums) within 1e-5 relative L2 velocity of the FP64 form,
   and the FP64 form's velocity decaying as exp(-2 nu k^2 t) (its L2 error against the analytic
   field at the level the oracle's 64^2 run and the reference's tables put it at this mesh).
 - Batching: the 96 seeded uneven instances (base seed 20261203), FP64 and mixed, every member

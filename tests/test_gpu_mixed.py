@@ -1,7 +1,7 @@
 """The mixed-precision form (FP32 deviations f - w rho0; FP64 moments,
-equilibria and corrector; with the consistent mass, FP32 inner CG solves refined
-with FP64 residuals to the same tolerance as the FP64 CG) against the FP64 form
-and the oracle. Tolerance: relative L2 velocity error <= 1e-5 against FP64 after
+equilibria and corrector; with the consistent mass, CG on FP32 increments with
+FP64 scalars and sums, stopped at max(cg_tolerance, the FP32 right-hand side's
+round-off level)) against the FP64 form and the oracle. Tolerance: relative L2 velocity error <= 1e-5 against FP64 after
 the run."""
 import numpy as np
 import pytest
@@ -105,7 +105,7 @@ def test_mixed_rejected_with_richardson():
         A.Bardow(d, mass=A.MASS_RICHARDSON, precision=A.MIXED)
 
 
-# ------------------------------------------------------------------ consistent mass (refined CG)
+# ------------------------------------------------------------------ consistent mass (FP32 CG)
 @pytest.mark.parametrize("case,p,scheme,streaming", [("tgv", 1, A.BARDOW, A.TG2), ("cavity", 1, A.LEELIN, A.TG2),
                                                      ("cavity", 2, A.BARDOW, A.TG2), ("couette", 2, A.LEELIN, A.TG2),
                                                      ("tgv", 1, A.BARDOW, A.TG3), ("tgv", 2, A.BARDOW, A.TG3_SPLIT)])
