@@ -8,11 +8,12 @@
 // of p nodes (the reach of a Q_p row) for all nine populations once, does the
 // nodal prologue, and stages the results in LDS: g* (Bardow), or f and
 // h = a f + b feq (Lee-Lin). The operator of each population is then applied
-// at the owned nodes straight from LDS. On Q1 meshes a thread first forms the
-// 3 x 3 weights of its node for the five parts of the operator (Kronecker
-// products of its assembled 1-D bands, the scheme coefficients folded in) and
-// applies them to each population; on Q2 and Q3 it contracts the y bands over
-// the 2p + 1 neighbouring columns and then the x bands of its column.
+// at the owned nodes straight from LDS. Where the row-march kernel also runs (Q1,
+// and Q2 in FP32) a thread evaluates the column form of column.hpp, the same
+// operation sequence as that kernel: at each of the 2p + 1 neighbouring columns
+// the y contractions grouped by the x band they meet, then the x bands; in FP64
+// on Q2 and on Q3 it contracts the y bands over the neighbouring columns on the
+// node-centred values and then the x bands of its column.
 //
 // With PS > 1 the eight populations of a node are split over PS threads (PS
 // wavefronts of the block hold the same nodes, each wavefront a fixed set of
