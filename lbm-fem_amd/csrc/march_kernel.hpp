@@ -716,9 +716,10 @@ namespace lbfem_amd
 
   // Many meshes of one degree and form (a batch group): the wavefronts of all members form one
   // list (member k owns waves [tile_begin_k, tile_begin_k+1), its S.tiles_x strips by S.tiles_y
-  // row segments of S.rows rows); wavefront w of the grid takes list entries w, w + 4 gridDim.x,
-  // ... in the XCD-aware block order, so that any grid covers the list and a captured launch keeps
-  // working after members join or leave. Each member flips its own population buffers by parity.
+  // row segments of S.rows rows); block b of the grid takes the list's blocks b, b + gridDim.x, ...
+  // (four wavefronts each) in the XCD-aware order of the list, so that any grid covers the list and
+  // a captured launch keeps working after members join or leave. Each member flips its own
+  // population buffers by parity.
   template <typename T, int MODE, int p>
   __global__ void __launch_bounds__(64 * march::kWaves)
   k_march_batch(const BatchEntry<T> *__restrict__ E, const BatchCounts *__restrict__ counts, const int parity)
@@ -728,10 +729,16 @@ namespace lbfem_amd
     __shared__ Lds<T, MODE == kStepLeeLin || MODE == kAdvectLeeLin, p> lds;
     const int wid    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n_inst = counts->n_inst, n_waves = counts->n_tiles;
-    const int nb8    = gridDim.x; // a multiple of 8
-    const int tb     = (blockIdx.x & 7) * (nb8 >> 3) + (blockIdx.x >> 3);
-    for (int gw = tb * kWaves + wid; gw < n_waves; gw += nb8 * kWaves)
+    // the list's blocks (four wavefronts each, a multiple of 8) in the XCD-aware order of the list
+    // itself, so that a grid larger than the list spreads it over every XCD (its extra blocks leave)
+    // and a smaller one strides over it; the grid is a multiple of 8, so block b stays on XCD b mod 8
+    const int nbl = ((n_waves + kWaves - 1) / kWaves + 7) & ~7;
+    for (int b = blockIdx.x; b < nbl; b += gridDim.x)
       {
+        const int tb = (b & 7) * (nbl >> 3) + (b >> 3);
+        const int gw = tb * kWaves + wid;
+        if (gw >= n_waves)
+          continue;
         int lo = 0, hi = n_inst - 1; // the member: last tile_begin <= gw
         while (lo < hi)
           {
