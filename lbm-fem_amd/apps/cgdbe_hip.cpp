@@ -6,8 +6,9 @@
 //
 //   cgdbe_hip --case cavity --refine 4 --reynolds 100,400 --mass lumped
 //
-// Exit status: 0, 1 on an error (the message on stderr), 2 if a steady-state
-// search ran out of time (cgdbe.cc:583-590).
+// Exit status: 0, 1 on an error (the message on stderr; also a run whose moments
+// stop being finite under --steady-max-change), 2 if a steady-state search ran
+// out of time (cgdbe.cc:583-590).
 #include "lbfem_hip.h"
 
 #include <algorithm>
@@ -55,6 +56,10 @@ namespace
     std::array<int, 2>  modes = {{1, 4}};
     std::vector<double> reynolds;
     unsigned            max_steps = 0, n_diagnostic = 20;
+    // steady test by the largest velocity change over a window of steps, max|u - u_prev|/U0 < eps
+    // (0: the reference's mean-rate test, cgdbe.cc:391-393)
+    double              steady_max_change = 0.;
+    unsigned            steady_window     = 1000;
     bool                output = true, fused = false, stage_timers = false;
     std::string         checkpoint, restart;
 
@@ -80,6 +85,7 @@ namespace
     "  --mach Ma [0.1]   --modes n1,n2 [1,4]   --reynolds Re[,Re...] [100 | 10 | 400]\n"
     "  --cfl c [0.25; cavity 0.4, with leelin 0.5]   --tend t [1; cavity 100]   --steps n\n"
     "  --stretch gamma [1.2]   --distort eps [0]   --steady-tol eps [1e-4]   --cg-tol eps [1e-8]   --fused\n"
+    "  --steady-max-change eps [0: off]   --steady-window n [1000] (steady when max|du|/U0 < eps over n steps)\n"
     "  --checkpoint name   --restart name   --precision fp64|mixed [fp64]   --no-output\n"
     "  --stage-timers (the summary's stage breakdown; GPU events at every stage, inside the timed run)\n";
 
@@ -145,6 +151,10 @@ namespace
           prm.distort = std::stod(val());
         else if (a == "--steady-tol")
           prm.steady_tol = std::stod(val());
+        else if (a == "--steady-max-change")
+          prm.steady_max_change = std::stod(val());
+        else if (a == "--steady-window")
+          prm.steady_window = unsigned(std::stoul(val()));
         else if (a == "--cg-tol")
           prm.cg_tol = std::stod(val());
         else if (a == "--fused")
@@ -176,6 +186,8 @@ namespace
           (prm.mass.rfind("richardson", 0) == 0 &&
            prm.mass.find_first_not_of("0123456789", 10) == std::string::npos)))
       throw std::runtime_error("--mass: not one of cg, lumped, richardson[k]: " + prm.mass);
+    if (prm.steady_max_change < 0. || prm.steady_window == 0)
+      throw std::runtime_error("--steady-max-change eps >= 0, --steady-window n > 0");
     prm.finish();
     return prm;
   }
@@ -432,16 +444,18 @@ namespace
           if (prm.output)
             {
               history.open(tag + "_history.dat");
-              history << (steady ? "# t  <u^2>/U0^2  -  <|du|>/U0/t_ref  mean_rho-rho0\n" :
-                                   "# t  E/E0  E/E0(exact)  rel_L2_error_u  mean_rho-rho0\n");
+              history << (!steady                      ? "# t  E/E0  E/E0(exact)  rel_L2_error_u  mean_rho-rho0\n" :
+                          prm.steady_max_change > 0. ? "# t  <u^2>/U0^2  -  max|du|/U0  <|du|>/U0\n" :
+                                                       "# t  <u^2>/U0^2  -  <|du|>/U0/t_ref  mean_rho-rho0\n");
             }
           diagnostics();
           if (prm.output && !steady)
             write_gnuplot(tag + "_initial.dat", Re);
 
           const unsigned interval =
-            !steady ? std::max(1u, n_steps / std::max(1u, prm.n_diagnostic)) :
-                      std::max(1u, unsigned(std::lround(t_ref / dt)));
+            !steady                      ? std::max(1u, n_steps / std::max(1u, prm.n_diagnostic)) :
+            prm.steady_max_change > 0. ? prm.steady_window :
+                                           std::max(1u, unsigned(std::lround(t_ref / dt)));
           bool converged = false;
           while (step_no < n_steps && !converged)
             {
@@ -568,11 +582,30 @@ namespace
         }
       else
         {
-          if (step_no == 0)
-            std::cout << "    step     t/t_ref  <u^2>/U0^2           -  <|du|>/U0/t_ref    <rho>-rho0\n";
-          const double rate = step_no == 0 ? 1. : s[5] / s[4] / flow.U0 / ((time - time_previous) / t_ref);
-          line({{s[2] / s[4] / E0, 0., rate, mass_drift}});
-          converged = rate < prm.steady_tol;
+          if (prm.steady_max_change > 0.)
+            {
+              // the largest velocity change since the previous line (the device monitor; a new Reynolds
+              // number starts from the last line of the previous one, so its first line only records)
+              double  dmax = 0., dmean = 0.;
+              int64_t bad  = 0;
+              check(lbfem_scheme_velocity_change(scheme, &dmax, &dmean, &bad));
+              if (bad > 0)
+                throw std::runtime_error("diverged at step " + std::to_string(step_no) + ": " +
+                                         std::to_string(bad) + " nodes with non-finite moments");
+              if (step_no == 0)
+                std::cout << "    step     t/t_ref  <u^2>/U0^2           -      max|du|/U0     <|du|>/U0\n";
+              const double m = step_no == 0 ? 1. : dmax / flow.U0;
+              line({{s[2] / s[4] / E0, 0., m, step_no == 0 ? 1. : dmean / flow.U0}});
+              converged = step_no > 0 && m < prm.steady_max_change;
+            }
+          else
+            {
+              if (step_no == 0)
+                std::cout << "    step     t/t_ref  <u^2>/U0^2           -  <|du|>/U0/t_ref    <rho>-rho0\n";
+              const double rate = step_no == 0 ? 1. : s[5] / s[4] / flow.U0 / ((time - time_previous) / t_ref);
+              line({{s[2] / s[4] / E0, 0., rate, mass_drift}});
+              converged = rate < prm.steady_tol;
+            }
         }
       time_previous = time;
       return converged;

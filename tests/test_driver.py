@@ -29,6 +29,10 @@ def test_help_and_bad_options(tmp_path):
     assert r.returncode == 1 and "--case" in r.stderr
     r = run(["--mass", "richardsonX"], tmp_path)
     assert r.returncode == 1
+    r = run(["--steady-window", "0"], tmp_path)
+    assert r.returncode == 1 and "--steady-window" in r.stderr
+    r = run(["--steady-max-change", "-1"], tmp_path)
+    assert r.returncode == 1 and "--steady-max-change" in r.stderr
 
 
 def psi_max(path):
@@ -47,6 +51,72 @@ def test_ci_cavity_ghia(tmp_path):
     for Re, ghia in ((100, 0.1034), (400, 0.1139)):
         p = psi_max(tmp_path / f"cavity_Re{Re}.dat")
         assert abs(p - ghia) / ghia < 0.1, (Re, p, ghia)
+
+
+def _steady_lines(stdout):
+    """(step, max|du|/U0) of the diagnostics lines of each Reynolds number, as printed"""
+    runs = []
+    for ln in stdout.splitlines():
+        if ln.strip().startswith("Re ="):
+            runs.append([])
+        x = ln.split()
+        if runs and len(x) == 6 and x[0].isdigit():
+            runs[-1].append((int(x[0]), float(x[4])))
+    return runs
+
+
+@pytest.mark.gpu
+def test_steady_by_largest_velocity_change(tmp_path):
+    """The steady test by the largest velocity change over a window of steps (max|u - u_prev|/U0 < eps,
+    the stopping rule of the plain-method GPU configuration): the CI cavity (16^2 Q1, Re 100 then 400,
+    lumped) stops at the first line under eps at each Reynolds number, psi_max within 10 % of Ghia, and
+    the last line's change equals the velocities of two runs' populations (the run to steady and a run
+    of one window fewer steps, both written as checkpoints) compared on the host."""
+    eps, W = 1e-7, 500
+    common = ["--case", "cavity", "--refine", "4", "--mass", "lumped", "--steady-max-change", str(eps),
+              "--steady-window", str(W)]
+    r = run([*common, "--reynolds", "100,400"], tmp_path)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    assert r.stdout.count("-> steady state") == 2
+    runs = _steady_lines(r.stdout)
+    assert len(runs) == 2
+    for lines in runs:
+        assert lines[0][1] == 1.0 and lines[-1][1] < eps  # the first line of each Re only records
+        assert all(m >= eps for _, m in lines[1:-1])
+        assert all(b - a == W for (a, _), (b, _) in zip(lines, lines[1:]))
+    for Re, ghia in ((100, 0.1034), (400, 0.1139)):
+        p = psi_max(tmp_path / f"cavity_Re{Re}.dat")
+        assert abs(p - ghia) / ghia < 0.1, (Re, p, ghia)
+
+    S, m_last = runs[0][-1]
+    r = run([*common, "--reynolds", "100", "--no-output", "--checkpoint", "end"], tmp_path)
+    assert r.returncode == 0 and _steady_lines(r.stdout)[0][-1] == (S, m_last), r.stdout[-1500:]
+    r = run([*common, "--reynolds", "100", "--no-output", "--steps", str(S - W), "--checkpoint", "before"], tmp_path)
+    assert r.returncode == 0, r.stderr
+
+    ex = np.array([0, 1, 1, 0, -1, -1, -1, 0, 1], dtype=np.float64)
+    ey = np.array([0, 0, 1, 1, 1, 0, -1, -1, -1], dtype=np.float64)
+
+    def velocity(name):
+        f = np.fromfile(tmp_path / f"{name}.0").reshape(9, -1)
+        rho = f.sum(axis=0)
+        return ex @ f / rho, ey @ f / rho
+
+    (u1, v1), (u0, v0) = velocity("end"), velocity("before")
+    m = np.hypot(u1 - u0, v1 - v0).max() / (0.1 / np.sqrt(3.0))
+    assert abs(m - m_last) <= 1e-5 * m_last, (m, m_last)
+
+
+@pytest.mark.gpu
+def test_steady_search_reports_divergence(tmp_path):
+    """Lee-Lin on the stretched cavity (Q1 256^2, lid speed 0.1, Re 400) diverges at CFL 0.6, as the
+    oracle does at CFL 0.5; under the largest-change test the driver stops at the first diagnostics
+    line with non-finite moments and exits with status 1."""
+    r = run(["--case", "cavity", "--scheme", "leelin", "--mass", "lumped", "--cells", "256", "--mach",
+             repr(float(0.1 * np.sqrt(3.0))), "--reynolds", "400", "--cfl", "0.6", "--tend", "2", "--no-output",
+             "--steady-max-change", "1e-8", "--steady-window", "2000"], tmp_path)
+    assert r.returncode == 1 and "diverged at step" in r.stderr, (r.returncode, r.stderr, r.stdout[-1500:])
 
 
 def load_rows(path):
