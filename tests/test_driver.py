@@ -160,6 +160,43 @@ def test_cavity_matches_the_reference_result_files(tmp_path):
         assert (np.abs(va - vb) <= 1.01 * unit + 1e-11).all(), (Re, np.abs(va - vb).max(axis=0))
 
 
+def vortices(path):
+    """{label: (psi, x, y)} of the vortex extrema lines of a cavity file"""
+    out = {}
+    for line in open(path):
+        if line.startswith("#") and " vortex: psi = " in line:
+            label, rest = line[1:].split(": psi = ")
+            v, _, x, y = rest.split()
+            out[label.strip()] = (float(v), float(x), float(y))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,mass,res,pub", [
+    ("leelin", "cg", "400,1000", (0.1232, -2.20e-4, -1.69e-3, 0.5225, 0.5672)),
+    ("bardow", "cg", "400", (0.1148, -1.02e-5, -6.20e-4, 0.5449, 0.6112)),
+    ("bardow", "lumped", "400,1000", (0.1247, -2.10e-4, -1.68e-3, 0.5225, 0.5672)),
+])
+def test_cavity_q1_table_published(tmp_path, scheme, mass, res, pub):
+    """The Q1 rows of the reference's cavity table (docs/results.md:265-285: 64^2 Q1, gamma 1.2,
+    Ma 0.1, CFL 0.4, steady-state tolerance 1e-4; Lee-Lin consistent and Bardow lumped at Re 1000,
+    Bardow consistent at Re 400), run by continuation from Re 400. The page does not state the
+    continuation, and the loose steady test leaves the primary vortex path-dependent by a few units
+    of its fourth digit (from rest straight to Re 1000 Lee-Lin stops at 0.1221), so the primary
+    vortex is checked to 4e-4, its node to the printed digits and the two corner vortices to 5 %."""
+    r = run(["--case", "cavity", "--refine", "6", "--cfl", "0.4", "--scheme", scheme, "--mass", mass,
+             "--reynolds", res], tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    Re = res.split(",")[-1]
+    v = vortices(tmp_path / f"cavity_Re{Re}.dat")
+    psi, x, y = v["primary vortex"]
+    print(scheme, mass, Re, v)
+    assert abs(psi - pub[0]) <= 4e-4, (psi, pub)
+    assert (round(x, 4), round(y, 4)) == (pub[3], pub[4]), (x, y, pub)
+    for label, want in (("lower left vortex", pub[1]), ("lower right vortex", pub[2])):
+        assert abs(v[label][0] - want) <= 0.05 * abs(want), (label, v[label], want)
+
+
 @pytest.mark.gpu
 def test_cavity_startup_cg_iterations_match_published(tmp_path):
     """docs/results.md:334-341: the mean CG iterations per mass solve of the cavity start-up
