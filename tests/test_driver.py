@@ -109,6 +109,27 @@ def test_steady_by_largest_velocity_change(tmp_path):
 
 
 @pytest.mark.gpu
+def test_plain_method_cavity_to_steady_q1_256(tmp_path):
+    """The plain-method GPU configuration run to its steady state: cavity Q1 256^2 (stretch 1.2, lid
+    speed 0.1), Lee-Lin, lumped mass, FP64, Re 400 and then Re 1000 by continuation, each until
+    max|du|/U0 < 1e-8 over 1000 steps (CFL 1.0: at CFL 0.5-0.75 Lee-Lin diverges at Re 400, dt/lambda
+    1.15-1.73). Both reach it (487 000 and 904 000 steps when recorded), and the primary vortex is
+    within 5 % of Ghia, Ghia & Shin (0.1139, 0.1179)."""
+    r = run(["--case", "cavity", "--scheme", "leelin", "--mass", "lumped", "--cells", "256", "--mach",
+             repr(float(0.1 * np.sqrt(3.0))), "--reynolds", "400,1000", "--cfl", "1.0", "--tend", "400",
+             "--steady-max-change", "1e-8", "--steady-window", "1000"], tmp_path, timeout=600)
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    runs = _steady_lines(r.stdout)
+    assert r.stdout.count("-> steady state") == 2 and len(runs) == 2
+    for lines in runs:
+        assert lines[-1][1] < 1e-8 and all(m >= 1e-8 for _, m in lines[1:-1])
+    for Re, ghia in ((400, 0.1139), (1000, 0.1179)):
+        p = psi_max(tmp_path / f"cavity_Re{Re}.dat")
+        assert abs(p - ghia) / ghia < 0.05, (Re, p, ghia)
+
+
+@pytest.mark.gpu
 def test_steady_search_reports_divergence(tmp_path):
     """Lee-Lin on the stretched cavity (Q1 256^2, lid speed 0.1, Re 400) diverges at CFL 0.6, as the
     oracle does at CFL 0.5; under the largest-change test the driver stops at the first diagnostics
