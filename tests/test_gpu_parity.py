@@ -1,8 +1,7 @@
 """Parity of the HIP path (through the C ABI of liblbfem_hip.so) with the CPU
 oracle, on the same meshes, flows and seeded inputs. FP64 throughout; the
-tolerances are those the north-star parity asks for: <= 1e-12 relative in the
-populations per step, <= 1e-10 after many steps of a stable configuration,
-CG iteration counts within +-1 per solve.
+tolerances: <= 1e-12 relative in the populations per step, <= 1e-10 after many
+steps of a stable configuration, CG iteration counts within +-1 per solve.
 """
 import numpy as np
 import pytest
