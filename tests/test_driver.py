@@ -130,6 +130,23 @@ def test_plain_method_cavity_to_steady_q1_256(tmp_path):
 
 
 @pytest.mark.gpu
+def test_mixed_cavity_to_served_steady_tolerance(tmp_path):
+    """The mixed form on the same cavity (Q1 256^2, lid 0.1, Lee-Lin lumped, CFL 1.0, Re 400 then 1000)
+    under the served steady rule, max|du|/U0 < 1e-6 over 1000 steps: both Reynolds numbers reach it,
+    and the primary vortex agrees with the FP64 steady state's (0.118152, 0.119688 when recorded) to
+    2e-4. (Its round-off floor, about 1e-6 U0 per 1000 steps, keeps the 1e-8 rule out of reach.)"""
+    r = run(["--case", "cavity", "--scheme", "leelin", "--mass", "lumped", "--cells", "256", "--mach",
+             repr(float(0.1 * np.sqrt(3.0))), "--reynolds", "400,1000", "--cfl", "1.0", "--tend", "400",
+             "--precision", "mixed", "--steady-max-change", "1e-6", "--steady-window", "1000"], tmp_path, timeout=600)
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    assert r.stdout.count("-> steady state") == 2
+    for Re, fp64 in ((400, 0.118152), (1000, 0.119688)):
+        p = psi_max(tmp_path / f"cavity_Re{Re}.dat")
+        assert abs(p - fp64) <= 2e-4, (Re, p, fp64)
+
+
+@pytest.mark.gpu
 def test_steady_search_reports_divergence(tmp_path):
     """Lee-Lin on the stretched cavity (Q1 256^2, lid speed 0.1, Re 400) diverges at CFL 0.6, as the
     oracle does at CFL 0.5; under the largest-change test the driver stops at the first diagnostics
